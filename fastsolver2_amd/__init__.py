@@ -1,0 +1,327 @@
+"""fastsolver2_amd -- MI355X-native HODLR factor-and-solve.
+
+Python host side over the C ABI of ``libhodlr_amd.so`` (include/hodlr.h).
+Device memory and streams come from PyTorch (ROCm); every numerical step
+runs in the library's HIP kernels.  There is no CPU fallback: importing
+works without a GPU, but every compute call raises if the HIP library or a
+GPU is missing.
+
+Matrices are FP64 and column-major, as in the reference
+(src/ptr_matrix.cc:29-37): an N x k operand is a torch tensor of shape
+(N, k) with strides (1, ld), e.g. ``fortran_empty(N, k)``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import torch
+
+__all__ = [
+    "lib", "lib_path", "available", "fortran_empty", "fortran", "HodlrPlan", "hsolve",
+    "gemm_red", "gemm_bro", "node_solve", "matvec", "generate", "workload_gaussian",
+    "dgesv_batched", "HodlrError",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+lib_path = os.path.join(_HERE, "libhodlr_amd.so")
+_lib = None
+
+HODLR_OK = 0
+HODLR_SINGULAR_LEAF = 1
+HODLR_SINGULAR_CORE = 2
+
+
+class HodlrError(RuntimeError):
+    def __init__(self, status, where=""):
+        self.status = status
+        msg = lib().hodlr_strerror(status).decode()
+        detail = lib().hodlr_last_error_detail().decode()
+        super().__init__(f"{where}: status {status} ({msg}){' - ' + detail if detail else ''}")
+
+
+class _Desc(C.Structure):
+    _fields_ = [("n", C.c_int64), ("leaf", C.c_int), ("levels", C.c_int), ("rank", C.c_int),
+                ("nrhs_max", C.c_int), ("precision", C.c_int), ("device", C.c_int),
+                ("flags", C.c_int)]
+
+
+def lib():
+    """Load libhodlr_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(lib_path):
+        raise RuntimeError(f"{lib_path} is missing: build it with `python -m fastsolver2_amd.build`")
+    L = C.CDLL(lib_path)
+    vp, i64, dbl, ci = C.c_void_p, C.c_int64, C.c_double, C.c_int
+    sig = {
+        "hodlr_version": ([], C.c_char_p),
+        "hodlr_strerror": ([ci], C.c_char_p),
+        "hodlr_last_error_detail": ([], C.c_char_p),
+        "hodlr_plan_create": ([C.POINTER(vp), C.POINTER(_Desc)], ci),
+        "hodlr_plan_destroy": ([vp], ci),
+        "hodlr_plan_device_bytes": ([vp], i64),
+        "hodlr_factor": ([vp, vp, i64, vp, i64, vp, i64, vp], ci),
+        "hodlr_solve": ([vp, vp, i64, ci, vp], ci),
+        "hodlr_factor_solve": ([vp, vp, i64, vp, i64, vp, i64, vp, i64, ci, vp], ci),
+        "hodlr_plan_status": ([vp, vp, C.POINTER(ci)], ci),
+        "hodlr_matvec": ([i64, ci, ci, ci, vp, i64, vp, i64, vp, i64, vp, i64, dbl, dbl, vp, i64,
+                          vp], ci),
+        "hodlr_hsolve_ref": ([ci, ci, ci, ci, ci, vp, vp, vp, vp], ci),
+        "hodlr_gemm_red": ([i64, ci, ci, dbl, vp, i64, ci, vp, i64, ci, vp, i64, vp], ci),
+        "hodlr_gemm_bro": ([i64, ci, ci, dbl, vp, i64, ci, vp, i64, vp, i64, ci, vp], ci),
+        "hodlr_node_solve": ([ci, ci, vp, i64, vp, i64, ci, vp], ci),
+        "hodlr_dgetrf_batched": ([ci, vp, i64, i64, ci, vp, vp, vp], ci),
+        "hodlr_dgesv_batched": ([ci, ci, vp, i64, i64, ci, vp, vp, i64, i64, vp], ci),
+        "hodlr_generate": ([C.c_uint64, C.c_uint32, C.c_uint32, ci, i64, ci, dbl, vp, i64, vp],
+                           ci),
+        "hodlr_shift_leaf_diagonal": ([i64, ci, dbl, vp, i64, vp], ci),
+        "hodlr_profile_enable": ([ci], ci),
+        "hodlr_profile_reset": ([], ci),
+        "hodlr_profile_read": ([ci, C.POINTER(i64), C.POINTER(dbl), C.POINTER(dbl),
+                                C.POINTER(dbl)], ci),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def available() -> bool:
+    """True when the HIP library is built and a GPU is visible."""
+    return os.path.exists(lib_path) and torch.cuda.is_available()
+
+
+# ---------------------------------------------------------------- tensors
+def fortran_empty(rows, cols, device="cuda", dtype=torch.float64):
+    """Column-major rows x cols tensor (strides (1, rows))."""
+    return torch.empty((cols, rows), dtype=dtype, device=device).t()
+
+
+def fortran(t: torch.Tensor) -> torch.Tensor:
+    """Column-major copy of a 2-D tensor (no copy when it already is)."""
+    if t.dim() == 1:
+        t = t.reshape(-1, 1)
+    if t.stride(0) == 1 and (t.shape[1] == 1 or t.stride(1) >= t.shape[0]):
+        return t
+    return t.t().contiguous().t()
+
+
+def _ld(t: torch.Tensor) -> int:
+    if t.dim() == 1:
+        return t.shape[0]
+    if t.stride(0) != 1:
+        raise ValueError("operand must be column-major (stride(0) == 1)")
+    return max(t.stride(1), t.shape[0]) if t.shape[1] > 1 else t.shape[0]
+
+
+def _check(t: torch.Tensor, name: str):
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor")
+    if t.dtype != torch.float64:
+        raise ValueError(f"{name} must be float64")
+
+
+def _stream():
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def _call(fn, *args, where=""):
+    st = fn(*args)
+    if st != 0:
+        raise HodlrError(st, where or fn.__name__)
+    return st
+
+
+# ---------------------------------------------------------------- plans
+class HodlrPlan:
+    """Device factorization of one HODLR matrix (the working form of the
+    reference's HMatrix, include/hmatrix.hpp:8-34)."""
+
+    def __init__(self, n, leaf, levels, rank, nrhs_max=1, device=None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("HodlrPlan needs a GPU")
+        dev = torch.cuda.current_device() if device is None else int(device)
+        self.n, self.leaf, self.levels, self.rank, self.nrhs_max = n, leaf, levels, rank, nrhs_max
+        self.device = dev
+        d = _Desc(n, leaf, levels, rank, nrhs_max, 0, dev, 0)
+        h = C.c_void_p()
+        _call(lib().hodlr_plan_create, C.byref(h), C.byref(d), where="hodlr_plan_create")
+        self._h = h
+        self._keep = ()
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().hodlr_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def device_bytes(self):
+        return lib().hodlr_plan_device_bytes(self._h)
+
+    def _ops(self, K, U, V):
+        for t, nm in ((K, "K"), (U, "U"), (V, "V")):
+            _check(t, nm)
+        if K.shape != (self.n, self.leaf) or U.shape != (self.n, self.rank) or V.shape != (self.n, self.rank):
+            raise ValueError("K must be N x leaf, U and V N x rank")
+
+    def factor(self, K, U, V):
+        """K is overwritten by the leaf LU factors (kept referenced)."""
+        self._ops(K, U, V)
+        _call(lib().hodlr_factor, self._h, _ptr(K), _ld(K), _ptr(U), _ld(U), _ptr(V), _ld(V),
+              _stream(), where="hodlr_factor")
+        self._keep = (K, U, V)
+
+    def factor_solve(self, K, U, V, B):
+        """Fused factor + solve; B (N x s) is overwritten by the solution."""
+        self._ops(K, U, V)
+        _check(B, "B")
+        B2 = B if B.dim() == 2 else B.view(-1, 1)
+        _call(lib().hodlr_factor_solve, self._h, _ptr(K), _ld(K), _ptr(U), _ld(U), _ptr(V),
+              _ld(V), _ptr(B2), _ld(B2), B2.shape[1], _stream(), where="hodlr_factor_solve")
+        self._keep = (K, U, V)
+        return B
+
+    def solve(self, B):
+        """B <- A^-1 B with the stored factorization."""
+        _check(B, "B")
+        B2 = B if B.dim() == 2 else B.view(-1, 1)
+        _call(lib().hodlr_solve, self._h, _ptr(B2), _ld(B2), B2.shape[1], _stream(),
+              where="hodlr_solve")
+        return B
+
+    def status(self):
+        w = C.c_int(-1)
+        st = lib().hodlr_plan_status(self._h, _stream(), C.byref(w))
+        return st, w.value
+
+
+# ---------------------------------------------------------------- operators
+def hsolve(nrow, nrhs, rank, nPart, K, X, V):
+    """Device form of the reference's hsolve (src/tasks/leaf_solve.cc:73-154)
+    on column-major device tensors with a common leading dimension."""
+    for t, nm in ((K, "K"), (X, "X"), (V, "V")):
+        _check(t, nm)
+    LD = _ld(K)
+    if _ld(X) != LD or _ld(V) != LD:
+        raise ValueError("K, X and V must share the leading dimension (leaf_solve.cc:60-61)")
+    return lib().hodlr_hsolve_ref(nrow, nrhs, rank, nPart, LD, _ptr(K), _ptr(X), _ptr(V),
+                                  _stream())
+
+
+def gemm_red(A, B, nchunk, nslot, alpha=1.0, C_out=None):
+    """LMatrix::gemmRed: C[slot] = alpha * sum_p A_p^T B_p (lmatrix.cc:713-761)."""
+    _check(A, "A"); _check(B, "B")
+    rows, ka = A.shape
+    kb = B.shape[1]
+    if C_out is None:
+        C_out = fortran_empty(nslot * ka, kb, device=A.device)
+    _call(lib().hodlr_gemm_red, rows, nchunk, nslot, alpha, _ptr(A), _ld(A), ka, _ptr(B), _ld(B),
+          kb, _ptr(C_out), _ld(C_out), _stream(), where="hodlr_gemm_red")
+    return C_out
+
+
+def gemm_bro(A, E, D, nchunk, nslot, alpha=-1.0):
+    """LMatrix::gemmBro: D_p += alpha * A_p E[slot(p)] (lmatrix.cc:814-865)."""
+    _check(A, "A"); _check(E, "E"); _check(D, "D")
+    rows, k = A.shape
+    _call(lib().hodlr_gemm_bro, rows, nchunk, nslot, alpha, _ptr(A), _ld(A), k, _ptr(E), _ld(E),
+          _ptr(D), _ld(D), D.shape[1], _stream(), where="hodlr_gemm_bro")
+    return D
+
+
+def node_solve(VTu, VTd, r):
+    """LMatrix::node_solve (lmatrix.cc:534-581); returns INFO."""
+    _check(VTu, "VTu"); _check(VTd, "VTd")
+    nnode = VTu.shape[0] // (2 * r)
+    return lib().hodlr_node_solve(nnode, r, _ptr(VTu), _ld(VTu), _ptr(VTd), _ld(VTd),
+                                  VTd.shape[1], _stream())
+
+
+def matvec(K, U, V, X, leaf, alpha=1.0, beta=0.0, Y=None):
+    """Y = alpha * A X + beta * Y for the HODLR matrix (K, U, V)."""
+    for t, nm in ((K, "K"), (U, "U"), (V, "V"), (X, "X")):
+        _check(t, nm)
+    X2 = X if X.dim() == 2 else X.view(-1, 1)
+    if Y is None:
+        Y = fortran_empty(X2.shape[0], X2.shape[1], device=X.device)
+        beta = 0.0
+    Y2 = Y if Y.dim() == 2 else Y.view(-1, 1)
+    _call(lib().hodlr_matvec, K.shape[0], leaf, U.shape[1], X2.shape[1], _ptr(K), _ld(K),
+          _ptr(U), _ld(U), _ptr(V), _ld(V), _ptr(X2), _ld(X2), alpha, beta, _ptr(Y2), _ld(Y2),
+          _stream(), where="hodlr_matvec")
+    return Y
+
+
+def dgesv_batched(A, B, n, batch):
+    """Batched dgesv on column-major blocks stacked by columns:
+    A is n x (n*batch), B is n x (nrhs*batch).  Returns (INFO, ipiv)."""
+    _check(A, "A"); _check(B, "B")
+    ipiv = torch.empty(batch * n, dtype=torch.int32, device=A.device)
+    nrhs = B.shape[1] // batch
+    st = lib().hodlr_dgesv_batched(n, nrhs, _ptr(A), _ld(A), _ld(A) * n, batch, _ptr(ipiv),
+                                   _ptr(B), _ld(B), _ld(B) * nrhs, _stream())
+    return st, ipiv
+
+
+def generate(seed, stream_id, system, kind, rows, cols, scale=1.0, out=None, device="cuda"):
+    """Philox4x64-10 generator on the device (kind 0 uniform, 1 normal)."""
+    if out is None:
+        out = fortran_empty(rows, cols, device=device)
+    _call(lib().hodlr_generate, seed, stream_id, system, kind, rows, cols, scale, _ptr(out),
+          _ld(out), _stream(), where="hodlr_generate")
+    return out
+
+
+def workload_gaussian(N, leaf, r, s, seed=0, system=0, shift=None, rhs="uniform",
+                      device="cuda"):
+    """Seeded Gaussian HODLR workload generated on the device:
+    K_i = G_i + shift*I (G iid N(0,1)), U, V iid N(0,1)/sqrt(r), B iid U[0,1)
+    (rhs='uniform') or N(0,1) (rhs='normal').  Streams 0..3 = K, U, V, B."""
+    shift = float(leaf) if shift is None else float(shift)
+    K = generate(seed, 0, system, 1, N, leaf, 1.0, device=device)
+    _call(lib().hodlr_shift_leaf_diagonal, N, leaf, shift, _ptr(K), _ld(K), _stream(),
+          where="hodlr_shift_leaf_diagonal")
+    sc = 1.0 / math.sqrt(float(r))
+    U = generate(seed, 1, system, 1, N, r, sc, device=device)
+    V = generate(seed, 2, system, 1, N, r, sc, device=device)
+    B = generate(seed, 3, system, 0 if rhs == "uniform" else 1, N, s, 1.0, device=device)
+    return K, U, V, B
+
+
+# ---------------------------------------------------------------- profiling
+PROF_KINDS = ["getrf_leaf", "getrs_leaf", "trtri", "reduce", "assemble", "getrf_core",
+              "getrs_core", "update"]
+
+
+def profile_enable(on=True):
+    lib().hodlr_profile_enable(1 if on else 0)
+
+
+def profile_reset():
+    lib().hodlr_profile_reset()
+
+
+def profile_read():
+    """{kind: dict(count, ms, flops, bytes)} over the recorded launches."""
+    out = {}
+    for k, name in enumerate(PROF_KINDS):
+        c, ms, fl, by = C.c_int64(), C.c_double(), C.c_double(), C.c_double()
+        _call(lib().hodlr_profile_read, k, C.byref(c), C.byref(ms), C.byref(fl), C.byref(by),
+              where="hodlr_profile_read")
+        out[name] = dict(count=c.value, ms=ms.value, flops=fl.value, bytes=by.value)
+    return out

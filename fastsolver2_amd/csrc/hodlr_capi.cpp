@@ -1,0 +1,773 @@
+// hodlr_capi.cpp -- C ABI (include/hodlr.h) over the gfx950 kernels.
+//
+// The HODLR solve is level-synchronous: one batched launch per step for all
+// leaves, then per tree level t = L-1..0 one reduce, one core assembly, one
+// batched core LU + solve and one update launch -- the reference's index
+// launches of LeafSolveTask / GemmRedTask / NodeSolveTask / GemmBroTask
+// (spmd_driver/solver_irregular_part.cc:114-140) become stream-ordered
+// kernels on one device.
+//
+// Fast path (hodlr_factor / hodlr_solve / hodlr_factor_solve).  The
+// reference carries, for every level t, its own copy u_t of U through the
+// leaf solve and all deeper levels (UTree column layout, src/tree.cc:64-73,
+// filled with identical copies by init_matrix.cc:64-71).  Every copy receives
+// exactly the same operations below its level, so at level t all of d's
+// u-blocks equal u_t.  The plan therefore carries one block Y_t = u_t per
+// level: Y_{L-1} = K^-1 U at the leaves and Y_{t-1} = Y_t - Y_t E_t, with
+// E_t the u-part of the core solve.  This is the same arithmetic on the same
+// values as the reference, without the redundant copies; Y_t is stored, so
+// later solves reuse the factorization.  hodlr_hsolve_ref keeps the
+// reference's full column layout for callers that pass it.
+#include "../../include/hodlr.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "hodlr_internal.hpp"
+
+using hodlr::ColSeg;
+
+namespace {
+
+thread_local char g_detail[256] = "";
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      snprintf(g_detail, sizeof g_detail, "%s: %s", #expr, hipGetErrorString(e_)); \
+      return HODLR_ERR_HIP;                                                        \
+    }                                                                              \
+  } while (0)
+
+struct DeviceGuard {
+  int prev = -1;
+  bool ok = true;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (dev >= 0 && dev != prev) ok = (hipSetDevice(dev) == hipSuccess);
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+// Row-chunk length of the split-K level reduce.
+int64_t reduce_rc(int64_t n_child) { return n_child < 512 ? n_child : 512; }
+
+// Library workspace for entry points without a plan: grown on demand,
+// one buffer per device, released at process exit.
+struct Workspace {
+  std::mutex mu;
+  std::vector<void*> ptr;
+  std::vector<size_t> size;
+};
+Workspace g_ws;
+
+int workspace(size_t bytes, void** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_ws.mu);
+  if ((int)g_ws.ptr.size() <= dev) {
+    g_ws.ptr.resize(dev + 1, nullptr);
+    g_ws.size.resize(dev + 1, 0);
+  }
+  if (g_ws.size[dev] < bytes) {
+    if (g_ws.ptr[dev]) {
+      HIP_TRY(hipDeviceSynchronize());
+      HIP_TRY(hipFree(g_ws.ptr[dev]));
+      g_ws.ptr[dev] = nullptr;
+      g_ws.size[dev] = 0;
+    }
+    size_t want = bytes + bytes / 4 + 4096;
+    if (hipMalloc(&g_ws.ptr[dev], want) != hipSuccess) {
+      snprintf(g_detail, sizeof g_detail, "workspace allocation of %zu bytes failed", want);
+      g_ws.ptr[dev] = nullptr;
+      return HODLR_ERR_NOMEM;
+    }
+    g_ws.size[dev] = want;
+  }
+  *out = g_ws.ptr[dev];
+  return HODLR_OK;
+}
+
+// Per-launch timing with HIP events on the launch stream, enabled by
+// hodlr_profile_enable(); each record carries the launch's algorithmic
+// FLOPs and minimum HBM bytes so a caller can place it on the roofline.
+struct ProfRec {
+  int kind;
+  hipEvent_t a, b;
+  double flops, bytes;
+};
+struct Profiler {
+  std::mutex mu;
+  bool on = false;
+  std::vector<ProfRec> recs;
+};
+Profiler g_prof;
+
+struct ProfScope {
+  bool on;
+  ProfRec rec;
+  hipStream_t st;
+  ProfScope(int kind, hipStream_t s, double flops, double bytes) : on(g_prof.on), st(s) {
+    if (!on) return;
+    rec.kind = kind; rec.flops = flops; rec.bytes = bytes;
+    if (hipEventCreate(&rec.a) != hipSuccess || hipEventCreate(&rec.b) != hipSuccess) {
+      on = false;
+      return;
+    }
+    (void)hipEventRecord(rec.a, st);
+  }
+  ~ProfScope() {
+    if (!on) return;
+    (void)hipEventRecord(rec.b, st);
+    std::lock_guard<std::mutex> lk(g_prof.mu);
+    g_prof.recs.push_back(rec);
+  }
+};
+
+bool is_pow2(int64_t x) { return x > 0 && (x & (x - 1)) == 0; }
+int ilog2(int64_t x) {
+  int l = 0;
+  while ((int64_t(1) << l) < x) ++l;
+  return l;
+}
+
+// Workspace layout for one batched core step (2^t nodes of 2r x 2r).
+struct CoreBufs {
+  double* S;
+  double* R;
+  int* ipiv;
+  int* perm;
+  int* info;
+  double* dinv;
+};
+
+// Core step shared by every level: S (already assembled) is factored in
+// place, R (2r x nh per node) overwritten by S^-1 R.
+double lu_flops(double n) { return 2.0 / 3.0 * n * n * n; }
+
+int core_factor_solve(int nnode, int r, const CoreBufs& c, int nh, hipStream_t st) {
+  const int n2 = 2 * r;
+  const double blk = 8.0 * n2 * n2 * nnode;
+  {
+    ProfScope ps(HODLR_PROF_GETRF_CORE, st, nnode * lu_flops(n2), 2.0 * blk);
+    HIP_TRY(hodlr::launch_getrf(n2, c.S, n2, (int64_t)n2 * n2, nnode, c.ipiv, c.perm, c.info, st));
+  }
+  {
+    ProfScope ps(HODLR_PROF_TRTRI, st, 0.0, blk);
+    HIP_TRY(hodlr::launch_trtri_diag(n2, c.S, n2, (int64_t)n2 * n2, nnode, c.dinv, st));
+  }
+  if (nh > 0) {
+    ProfScope ps(HODLR_PROF_GETRS_CORE, st, 2.0 * n2 * n2 * nh * nnode,
+                 blk + 16.0 * n2 * nh * nnode);
+    ColSeg x = hodlr::seg_single(c.R, n2);
+    HIP_TRY(hodlr::launch_getrs(n2, c.S, n2, (int64_t)n2 * n2, nnode, c.perm, c.dinv, x,
+                                (int64_t)n2 * nh, nh, st));
+  }
+  return HODLR_OK;
+}
+
+}  // namespace
+
+// =====================================================================
+// plans
+// =====================================================================
+struct hodlr_plan_s {
+  hodlr_desc d;
+  int64_t N = 0;
+  int m = 0, L = 0, r = 0, smax = 0;
+  int64_t P = 0;
+  // leaf factorization
+  int* leaf_ipiv = nullptr;
+  int* leaf_perm = nullptr;
+  int* leaf_info = nullptr;
+  double* leaf_dinv = nullptr;
+  // Y_t blocks, t = 0..L-1, each N x r (ld N)
+  double* Y = nullptr;
+  // cores, all levels; level t at offsets *_off[t]
+  double* coreS = nullptr;
+  int* core_ipiv = nullptr;
+  int* core_perm = nullptr;
+  int* core_info = nullptr;
+  double* core_dinv = nullptr;
+  std::vector<int64_t> s_off, p_off, d_off, i_off;
+  // workspace
+  double* part = nullptr;
+  double* R = nullptr;
+  int64_t part_elems = 0, r_elems = 0;
+  // factored operands
+  const double* K = nullptr;
+  const double* U = nullptr;
+  const double* V = nullptr;
+  int64_t ldk = 0, ldu = 0, ldv = 0;
+  bool factored = false;
+  int64_t bytes = 0;
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+template <typename T>
+int palloc(hodlr_plan p, T** out, int64_t count) {
+  *out = nullptr;
+  if (count <= 0) return HODLR_OK;
+  void* ptr = nullptr;
+  const size_t bytes = sizeof(T) * (size_t)count;
+  if (hipMalloc(&ptr, bytes) != hipSuccess) {
+    snprintf(g_detail, sizeof g_detail, "plan allocation of %zu bytes failed", bytes);
+    return HODLR_ERR_NOMEM;
+  }
+  p->allocs.push_back(ptr);
+  p->bytes += (int64_t)bytes;
+  *out = static_cast<T*>(ptr);
+  return HODLR_OK;
+}
+
+void plan_free(hodlr_plan p) {
+  for (void* a : p->allocs) (void)hipFree(a);
+  p->allocs.clear();
+}
+
+CoreBufs core_level(hodlr_plan p, int t) {
+  CoreBufs c;
+  c.S = p->coreS + p->s_off[t];
+  c.R = p->R;
+  c.ipiv = p->core_ipiv + p->p_off[t];
+  c.perm = p->core_perm + p->p_off[t];
+  c.info = p->core_info + p->i_off[t];
+  c.dinv = p->core_dinv + p->d_off[t];
+  return c;
+}
+
+int check_ld(int64_t ld, int64_t n) { return ld >= n ? HODLR_OK : HODLR_ERR_ARG; }
+
+// Leaf phase of factor (+ solve): LU of every leaf, then
+// [Y_{L-1} | B] = K^-1 [U | B].
+int leaf_phase(hodlr_plan p, double* K, int64_t ldk, const double* U, int64_t ldu, double* B,
+               int64_t ldb, int nrhs, hipStream_t st) {
+  const int m = p->m;
+  const double kb = 8.0 * (double)p->N * m;
+  {
+    ProfScope ps(HODLR_PROF_GETRF_LEAF, st, p->P * lu_flops(m), 2.0 * kb);
+    HIP_TRY(hodlr::launch_getrf(m, K, ldk, m, (int)p->P, p->leaf_ipiv, p->leaf_perm, p->leaf_info,
+                                st));
+  }
+  {
+    ProfScope ps(HODLR_PROF_TRTRI, st, 0.0, kb);
+    HIP_TRY(hodlr::launch_trtri_diag(m, K, ldk, m, (int)p->P, p->leaf_dinv, st));
+  }
+  ColSeg x;
+  int ncols;
+  if (p->L > 0) {
+    double* Ylast = p->Y + (int64_t)(p->L - 1) * p->N * p->r;
+    x.inA = U; x.outA = Ylast; x.ldiA = ldu; x.ldoA = p->N; x.nA = p->r;
+    x.inB = B; x.outB = B; x.ldiB = ldb; x.ldoB = ldb;
+    ncols = p->r + nrhs;
+  } else {
+    x = hodlr::seg_single(B, ldb);
+    ncols = nrhs;
+  }
+  if (ncols > 0) {
+    ProfScope ps(HODLR_PROF_GETRS_LEAF, st, 2.0 * (double)p->P * m * m * ncols,
+                 kb + 16.0 * (double)p->N * ncols);
+    HIP_TRY(hodlr::launch_getrs(m, K, ldk, m, (int)p->P, p->leaf_perm, p->leaf_dinv, x, m, ncols,
+                                st));
+  }
+  return HODLR_OK;
+}
+
+// Level loop of factor (+ solve) over [Y_t | B] (nrhs may be 0).
+int level_phase(hodlr_plan p, double* B, int64_t ldb, int nrhs, hipStream_t st) {
+  const int r = p->r;
+  const int64_t N = p->N;
+  const int nc = r + nrhs;
+  for (int t = p->L - 1; t >= 0; --t) {
+    const int nchild = 1 << (t + 1), nnode = 1 << t;
+    const int64_t n_child = N >> (t + 1);
+    double* Yt = p->Y + (int64_t)t * N * r;
+    ColSeg x;
+    x.inA = Yt; x.outA = Yt; x.ldiA = N; x.ldoA = N; x.nA = r;
+    x.inB = B; x.outB = B; x.ldiB = ldb; x.ldoB = ldb;
+    int cpc = 0;
+    {
+      ProfScope ps(HODLR_PROF_REDUCE, st, 2.0 * (double)N * r * nc, 8.0 * (double)N * (r + nc));
+      HIP_TRY(hodlr::launch_reduce(p->V, p->ldv, r, x, nc, n_child, nchild, reduce_rc(n_child),
+                                   p->part, &cpc, st));
+    }
+    CoreBufs c = core_level(p, t);
+    {
+      ProfScope ps(HODLR_PROF_ASSEMBLE, st, 0.0, 8.0 * nchild * cpc * (double)r * nc);
+      HIP_TRY(hodlr::launch_assemble(p->part, cpc, r, nc, 0, 0, nc, nnode, c.S, c.R, st));
+    }
+    int rc = core_factor_solve(nnode, r, c, nc, st);
+    if (rc) return rc;
+    const int64_t ens = (int64_t)2 * r * nc;
+    if (t > 0) {
+      // Y_{t-1} = Y_t - Y_t E_t ; B -= Y_t eta_B
+      ColSeg y;
+      y.inA = Yt; y.outA = p->Y + (int64_t)(t - 1) * N * r; y.ldiA = N; y.ldoA = N; y.nA = r;
+      y.inB = B; y.outB = B; y.ldiB = ldb; y.ldoB = ldb;
+      ProfScope ps(HODLR_PROF_UPDATE, st, 2.0 * (double)N * r * nc,
+                   8.0 * (double)N * (r + 2.0 * nc));
+      HIP_TRY(hodlr::launch_update(Yt, N, r, c.R, 2 * r, ens, 1, n_child, nchild, -1.0, y, nc, st));
+    } else if (nrhs > 0) {
+      ColSeg y = hodlr::seg_single(B, ldb);
+      ProfScope ps(HODLR_PROF_UPDATE, st, 2.0 * (double)N * r * nrhs,
+                   8.0 * (double)N * (r + 2.0 * nrhs));
+      HIP_TRY(hodlr::launch_update(Yt, N, r, c.R + (int64_t)r * 2 * r, 2 * r, ens, 1, n_child,
+                                   nchild, -1.0, y, nrhs, st));
+    }
+  }
+  return HODLR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hodlr_version(void) { return "hodlr-amd 0.1 (gfx950, FP64 MFMA)"; }
+
+const char* hodlr_strerror(int status) {
+  switch (status) {
+    case HODLR_OK: return "success";
+    case HODLR_ERR_ARG: return "invalid argument";
+    case HODLR_ERR_SHAPE: return "inconsistent HODLR shape";
+    case HODLR_ERR_UNSUPPORTED: return "size not supported by the kernels";
+    case HODLR_ERR_HIP: return "HIP runtime error";
+    case HODLR_ERR_NOMEM: return "device allocation failed";
+    case HODLR_ERR_STATE: return "plan not factored";
+    case HODLR_SINGULAR_LEAF: return "singular leaf block (zero pivot)";
+    case HODLR_SINGULAR_CORE: return "singular core block (zero pivot)";
+    default: return "unknown status";
+  }
+}
+
+const char* hodlr_last_error_detail(void) { return g_detail; }
+
+int hodlr_profile_enable(int on) {
+  std::lock_guard<std::mutex> lk(g_prof.mu);
+  g_prof.on = (on != 0);
+  return HODLR_OK;
+}
+
+int hodlr_profile_reset(void) {
+  std::lock_guard<std::mutex> lk(g_prof.mu);
+  for (auto& r : g_prof.recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  g_prof.recs.clear();
+  return HODLR_OK;
+}
+
+int hodlr_profile_read(int kind, int64_t* count, double* total_ms, double* flops,
+                       double* bytes) {
+  if (kind < 0 || kind >= HODLR_PROF_NKINDS) return HODLR_ERR_ARG;
+  std::lock_guard<std::mutex> lk(g_prof.mu);
+  int64_t c = 0;
+  double ms = 0.0, f = 0.0, b = 0.0;
+  for (auto& r : g_prof.recs) {
+    if (r.kind != kind) continue;
+    HIP_TRY(hipEventSynchronize(r.b));
+    float e = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&e, r.a, r.b));
+    ++c; ms += e; f += r.flops; b += r.bytes;
+  }
+  if (count) *count = c;
+  if (total_ms) *total_ms = ms;
+  if (flops) *flops = f;
+  if (bytes) *bytes = b;
+  return HODLR_OK;
+}
+
+int hodlr_plan_create(hodlr_plan* plan, const hodlr_desc* desc) {
+  if (!plan || !desc) return HODLR_ERR_ARG;
+  *plan = nullptr;
+  const hodlr_desc& d = *desc;
+  if (d.leaf <= 0 || d.rank <= 0 || d.levels < 0 || d.levels > 24 || d.nrhs_max < 0 || d.flags)
+    return HODLR_ERR_ARG;
+  if (d.n != ((int64_t)d.leaf << d.levels)) return HODLR_ERR_SHAPE;
+  if (d.levels > 0 && d.leaf <= d.rank) return HODLR_ERR_SHAPE;  // matrix.cc:225
+  if (d.leaf > hodlr::kMaxBlock || 2 * d.rank > hodlr::kMaxBlock) return HODLR_ERR_UNSUPPORTED;
+  if (d.precision != HODLR_FP64) return HODLR_ERR_UNSUPPORTED;
+  DeviceGuard g(d.device);
+  if (!g.ok) return HODLR_ERR_ARG;
+  hodlr_plan p = new hodlr_plan_s();
+  p->d = d;
+  p->N = d.n; p->m = d.leaf; p->L = d.levels; p->r = d.rank; p->smax = d.nrhs_max;
+  p->P = (int64_t)1 << d.levels;
+  const int r = p->r, n2 = 2 * r;
+  int rc = HODLR_OK;
+  rc = rc ? rc : palloc(p, &p->leaf_ipiv, p->P * p->m);
+  rc = rc ? rc : palloc(p, &p->leaf_perm, p->P * p->m);
+  rc = rc ? rc : palloc(p, &p->leaf_info, p->P);
+  rc = rc ? rc : palloc(p, &p->leaf_dinv, p->P * hodlr::dinv_elems(p->m));
+  rc = rc ? rc : palloc(p, &p->Y, (int64_t)p->L * p->N * r);
+  int64_t so = 0, po = 0, dof = 0, io = 0;
+  for (int t = 0; t < p->L; ++t) {
+    const int64_t nn = (int64_t)1 << t;
+    p->s_off.push_back(so); p->p_off.push_back(po); p->d_off.push_back(dof); p->i_off.push_back(io);
+    so += nn * n2 * n2; po += nn * n2; dof += nn * hodlr::dinv_elems(n2); io += nn;
+  }
+  rc = rc ? rc : palloc(p, &p->coreS, so);
+  rc = rc ? rc : palloc(p, &p->core_ipiv, po);
+  rc = rc ? rc : palloc(p, &p->core_perm, po);
+  rc = rc ? rc : palloc(p, &p->core_info, io > 0 ? io : 1);
+  rc = rc ? rc : palloc(p, &p->core_dinv, dof);
+  // reduce partials and core RHS for the widest operand [Y | B]
+  const int ncmax = r + (p->smax > 0 ? p->smax : 1);
+  int64_t pe = 0, re = 0;
+  for (int t = 0; t < p->L; ++t) {
+    const int64_t n_child = p->N >> (t + 1);
+    const int64_t chunks = ((int64_t)2 << t) * hodlr::reduce_chunks_per_child(n_child, reduce_rc(n_child));
+    pe = std::max<int64_t>(pe, chunks * r * ncmax);
+    re = std::max<int64_t>(re, ((int64_t)1 << t) * n2 * ncmax);
+  }
+  p->part_elems = pe; p->r_elems = re;
+  rc = rc ? rc : palloc(p, &p->part, pe);
+  rc = rc ? rc : palloc(p, &p->R, re);
+  if (rc) {
+    plan_free(p);
+    delete p;
+    return rc;
+  }
+  *plan = p;
+  return HODLR_OK;
+}
+
+int hodlr_plan_destroy(hodlr_plan plan) {
+  if (!plan) return HODLR_ERR_ARG;
+  DeviceGuard g(plan->d.device);
+  (void)hipDeviceSynchronize();
+  plan_free(plan);
+  delete plan;
+  return HODLR_OK;
+}
+
+int64_t hodlr_plan_device_bytes(hodlr_plan plan) { return plan ? plan->bytes : -1; }
+
+int hodlr_factor(hodlr_plan p, double* K, int64_t ldk, const double* U, int64_t ldu,
+                 const double* V, int64_t ldv, hipStream_t st) {
+  if (!p || !K || !U || !V) return HODLR_ERR_ARG;
+  if (check_ld(ldk, p->N) || check_ld(ldu, p->N) || check_ld(ldv, p->N)) return HODLR_ERR_ARG;
+  DeviceGuard g(p->d.device);
+  p->K = K; p->U = U; p->V = V; p->ldk = ldk; p->ldu = ldu; p->ldv = ldv;
+  p->factored = false;
+  int rc = leaf_phase(p, K, ldk, U, ldu, nullptr, 0, 0, st);
+  if (!rc) rc = level_phase(p, nullptr, 0, 0, st);
+  if (!rc) p->factored = true;
+  return rc;
+}
+
+int hodlr_factor_solve(hodlr_plan p, double* K, int64_t ldk, const double* U, int64_t ldu,
+                       const double* V, int64_t ldv, double* B, int64_t ldb, int nrhs,
+                       hipStream_t st) {
+  if (!p || !K || !U || !V || (nrhs > 0 && !B) || nrhs < 0) return HODLR_ERR_ARG;
+  if (check_ld(ldk, p->N) || check_ld(ldu, p->N) || check_ld(ldv, p->N)) return HODLR_ERR_ARG;
+  if (nrhs > 0 && check_ld(ldb, p->N)) return HODLR_ERR_ARG;
+  if (nrhs > p->smax) return HODLR_ERR_ARG;
+  DeviceGuard g(p->d.device);
+  p->K = K; p->U = U; p->V = V; p->ldk = ldk; p->ldu = ldu; p->ldv = ldv;
+  p->factored = false;
+  int rc = leaf_phase(p, K, ldk, U, ldu, B, ldb, nrhs, st);
+  if (!rc) rc = level_phase(p, B, ldb, nrhs, st);
+  if (!rc) p->factored = true;
+  return rc;
+}
+
+int hodlr_solve(hodlr_plan p, double* B, int64_t ldb, int nrhs, hipStream_t st) {
+  if (!p || !B || nrhs < 0) return HODLR_ERR_ARG;
+  if (!p->factored) return HODLR_ERR_STATE;
+  if (check_ld(ldb, p->N)) return HODLR_ERR_ARG;
+  DeviceGuard g(p->d.device);
+  const int r = p->r, m = p->m;
+  const int64_t N = p->N;
+  const int chunk = r + (p->smax > 0 ? p->smax : 1);  // RHS columns per pass
+  for (int c0 = 0; c0 < nrhs; c0 += chunk) {
+    const int nc = std::min(chunk, nrhs - c0);
+    double* Bc = B + (int64_t)c0 * ldb;
+    ColSeg x = hodlr::seg_single(Bc, ldb);
+    HIP_TRY(hodlr::launch_getrs(m, p->K, p->ldk, m, (int)p->P, p->leaf_perm, p->leaf_dinv, x, m,
+                                nc, st));
+    for (int t = p->L - 1; t >= 0; --t) {
+      const int nchild = 1 << (t + 1), nnode = 1 << t;
+      const int64_t n_child = N >> (t + 1);
+      int cpc = 0;
+      HIP_TRY(hodlr::launch_reduce(p->V, p->ldv, r, x, nc, n_child, nchild, reduce_rc(n_child),
+                                   p->part, &cpc, st));
+      CoreBufs c = core_level(p, t);
+      HIP_TRY(hodlr::launch_assemble(p->part, cpc, r, nc, 0, 0, nc, nnode, nullptr, c.R, st));
+      ColSeg rx = hodlr::seg_single(c.R, 2 * r);
+      HIP_TRY(hodlr::launch_getrs(2 * r, c.S, 2 * r, (int64_t)4 * r * r, nnode, c.perm, c.dinv,
+                                  rx, (int64_t)2 * r * nc, nc, st));
+      const double* Yt = p->Y + (int64_t)t * N * r;
+      HIP_TRY(hodlr::launch_update(Yt, N, r, c.R, 2 * r, (int64_t)2 * r * nc, 1, n_child, nchild,
+                                   -1.0, x, nc, st));
+    }
+  }
+  return HODLR_OK;
+}
+
+int hodlr_plan_status(hodlr_plan p, hipStream_t st, int* where) {
+  if (!p) return HODLR_ERR_ARG;
+  DeviceGuard g(p->d.device);
+  HIP_TRY(hipStreamSynchronize(st));
+  std::vector<int> li(p->P), ci;
+  HIP_TRY(hipMemcpy(li.data(), p->leaf_info, sizeof(int) * p->P, hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < p->P; ++i)
+    if (li[i]) {
+      if (where) *where = (int)i;
+      return HODLR_SINGULAR_LEAF;
+    }
+  for (int t = 0; t < p->L; ++t) {
+    const int nn = 1 << t;
+    ci.resize(nn);
+    HIP_TRY(hipMemcpy(ci.data(), p->core_info + p->i_off[t], sizeof(int) * nn,
+                      hipMemcpyDeviceToHost));
+    for (int k = 0; k < nn; ++k)
+      if (ci[k]) {
+        if (where) *where = (t << 20) + k;
+        return HODLR_SINGULAR_CORE;
+      }
+  }
+  if (where) *where = -1;
+  return HODLR_OK;
+}
+
+int hodlr_matvec(int64_t n, int leaf, int rank, int nrhs, const double* K, int64_t ldk,
+                 const double* U, int64_t ldu, const double* V, int64_t ldv, const double* X,
+                 int64_t ldx, double alpha, double beta, double* Y, int64_t ldy, hipStream_t st) {
+  if (n <= 0 || leaf <= 0 || rank <= 0 || nrhs < 0 || n % leaf) return HODLR_ERR_ARG;
+  if (!K || !U || !V || !X || !Y) return HODLR_ERR_ARG;
+  if (ldk < n || ldu < n || ldv < n || ldx < n || ldy < n) return HODLR_ERR_ARG;
+  if (nrhs == 0) return HODLR_OK;
+  void* w = nullptr;
+  int rc = workspace(sizeof(double) * (size_t)hodlr::matvec_work_elems(n, leaf, rank, nrhs), &w);
+  if (rc) return rc;
+  HIP_TRY(hodlr::launch_matvec(n, leaf, rank, nrhs, K, ldk, U, ldu, V, ldv, X, ldx, alpha, beta, Y,
+                               ldy, static_cast<double*>(w), st));
+  return HODLR_OK;
+}
+
+// ---------------------------------------------------------------------
+// Reference-layout hsolve on the device (src/tasks/leaf_solve.cc:73-154).
+// ---------------------------------------------------------------------
+int hodlr_hsolve_ref(int nrow, int nrhs, int rank, int nPart, int LD, double* K, double* X,
+                     const double* V, hipStream_t st) {
+  if (nrow <= 0 || nrhs < 0 || rank <= 0 || nPart <= 0 || LD < nrow) return HODLR_ERR_ARG;
+  if (!K || !X || !V) return HODLR_ERR_ARG;
+  if (!is_pow2(nPart) || nrow % nPart) return HODLR_ERR_SHAPE;  // leaf_solve.cc:64,95-96
+  const int Ls = ilog2(nPart);
+  const int m = nrow / nPart;
+  const int r = rank, n2 = 2 * r;
+  if (m > hodlr::kMaxBlock || n2 > hodlr::kMaxBlock) return HODLR_ERR_UNSUPPORTED;
+  const int wtot = nrhs + r * Ls;
+  // workspace: leaf ipiv/perm/info/dinv, core S/ipiv/perm/info/dinv, R, part
+  int64_t pe = 0, re = 0, se = 0, ce = 0;
+  for (int t = 0; t < Ls; ++t) {
+    const int64_t n_child = (int64_t)nrow >> (t + 1);
+    const int w = nrhs + r * t;
+    const int64_t chunks = ((int64_t)2 << t) * hodlr::reduce_chunks_per_child(n_child, reduce_rc(n_child));
+    pe = std::max<int64_t>(pe, chunks * r * (w + r));
+    re = std::max<int64_t>(re, ((int64_t)1 << t) * n2 * std::max(w, 1));
+    se = std::max<int64_t>(se, ((int64_t)1 << t) * n2 * n2);
+    ce = std::max<int64_t>(ce, ((int64_t)1 << t));
+  }
+  const int64_t dl = (int64_t)nPart * hodlr::dinv_elems(m);
+  const int64_t dc = ce * hodlr::dinv_elems(n2);
+  const int64_t ndbl = dl + pe + re + se + dc;
+  // core INFO slots for every level (level t at offset 2^t - 1)
+  const int64_t nint = (int64_t)nPart * m * 2 + nPart + ce * n2 * 2 + nPart + 64;
+  void* w = nullptr;
+  int rc = workspace(sizeof(double) * (size_t)ndbl + sizeof(int) * (size_t)nint, &w);
+  if (rc) return rc;
+  double* dp = static_cast<double*>(w);
+  double* leaf_dinv = dp; dp += dl;
+  double* part = dp; dp += pe;
+  double* R = dp; dp += re;
+  double* S = dp; dp += se;
+  double* cdinv = dp; dp += dc;
+  int* ip = reinterpret_cast<int*>(dp);
+  int* leaf_ipiv = ip; ip += (int64_t)nPart * m;
+  int* leaf_perm = ip; ip += (int64_t)nPart * m;
+  int* leaf_info = ip; ip += nPart;
+  int* cipiv = ip; ip += ce * n2;
+  int* cperm = ip; ip += ce * n2;
+  int* cinfo_all = ip; ip += nPart;
+  // leaf: dgesv on all columns (leaf_solve.cc:80-92)
+  HIP_TRY(hodlr::launch_getrf(m, K, LD, m, nPart, leaf_ipiv, leaf_perm, leaf_info, st));
+  HIP_TRY(hodlr::launch_trtri_diag(m, K, LD, m, nPart, leaf_dinv, st));
+  ColSeg xall = hodlr::seg_single(X, LD);
+  if (wtot > 0)
+    HIP_TRY(hodlr::launch_getrs(m, K, LD, m, nPart, leaf_perm, leaf_dinv, xall, m, wtot, st));
+  // levels, deepest first (the recursion's post-order, level-synchronous)
+  for (int t = Ls - 1; t >= 0; --t) {
+    const int nchild = 1 << (t + 1), nnode = 1 << t;
+    const int64_t n_child = (int64_t)nrow >> (t + 1);
+    const int wd = nrhs + r * t;  // d = cols [0, wd), u = cols [wd, wd + r)
+    int cpc = 0;
+    HIP_TRY(hodlr::launch_reduce(V, LD, r, xall, wd + r, n_child, nchild, reduce_rc(n_child), part,
+                                 &cpc, st));
+    HIP_TRY(hodlr::launch_assemble(part, cpc, r, wd + r, wd, 0, wd, nnode, S, R, st));
+    CoreBufs c{S, R, cipiv, cperm, cinfo_all + (nnode - 1), cdinv};
+    rc = core_factor_solve(nnode, r, c, wd, st);
+    if (rc) return rc;
+    if (wd > 0)
+      HIP_TRY(hodlr::launch_update(X + (int64_t)wd * LD, LD, r, R, n2, (int64_t)n2 * wd, 1, n_child,
+                                   nchild, -1.0, xall, wd, st));
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  std::vector<int> li(nPart);
+  HIP_TRY(hipMemcpy(li.data(), leaf_info, sizeof(int) * nPart, hipMemcpyDeviceToHost));
+  for (int i = 0; i < nPart; ++i)
+    if (li[i]) return HODLR_SINGULAR_LEAF;
+  if (nPart > 1) {
+    std::vector<int> ci(nPart - 1);
+    HIP_TRY(hipMemcpy(ci.data(), cinfo_all, sizeof(int) * (nPart - 1), hipMemcpyDeviceToHost));
+    for (int k = 0; k < nPart - 1; ++k)
+      if (ci[k]) return HODLR_SINGULAR_CORE;
+  }
+  return HODLR_OK;
+}
+
+// ---------------------------------------------------------------------
+// LMatrix operators
+// ---------------------------------------------------------------------
+int hodlr_gemm_red(int64_t rows, int nchunk, int nslot, double alpha, const double* A, int64_t lda,
+                   int ka, const double* B, int64_t ldb, int kb, double* C, int64_t ldc,
+                   hipStream_t st) {
+  if (rows <= 0 || nchunk <= 0 || nslot <= 0 || ka <= 0 || kb <= 0) return HODLR_ERR_ARG;
+  if (rows % nchunk || nchunk % nslot) return HODLR_ERR_SHAPE;  // lmatrix.cc:725-726
+  if (!A || !B || !C || lda < rows || ldb < rows || ldc < (int64_t)nslot * ka) return HODLR_ERR_ARG;
+  const int64_t blk = rows / nchunk;
+  const int64_t rc = blk < 512 ? blk : 512;
+  const int64_t cpc = hodlr::reduce_chunks_per_child(blk, rc);
+  void* w = nullptr;
+  int rc2 = workspace(sizeof(double) * (size_t)(nchunk * cpc * (int64_t)ka * kb), &w);
+  if (rc2) return rc2;
+  double* part = static_cast<double*>(w);
+  ColSeg x = hodlr::seg_single(const_cast<double*>(B), ldb);
+  int cpc_out = 0;
+  HIP_TRY(hodlr::launch_reduce(A, lda, ka, x, kb, blk, nchunk, rc, part, &cpc_out, st));
+  HIP_TRY(hodlr::launch_sum_slots(part, (nchunk / nslot) * cpc_out, nslot, ka, kb, alpha, C, ldc,
+                                  st));
+  return HODLR_OK;
+}
+
+int hodlr_gemm_bro(int64_t rows, int nchunk, int nslot, double alpha, const double* A, int64_t lda,
+                   int k, const double* E, int64_t lde, double* D, int64_t ldd, int nd,
+                   hipStream_t st) {
+  if (rows <= 0 || nchunk <= 0 || nslot <= 0 || k <= 0 || nd < 0) return HODLR_ERR_ARG;
+  if (rows % nchunk || nchunk % nslot) return HODLR_ERR_SHAPE;  // lmatrix.cc:824-825
+  if (!A || !E || !D || lda < rows || ldd < rows || lde < (int64_t)nslot * k) return HODLR_ERR_ARG;
+  if (nd == 0) return HODLR_OK;
+  const int64_t blk = rows / nchunk;
+  ColSeg x = hodlr::seg_single(D, ldd);
+  // slot s rows [s*k, s*k+k) of E  ==  (s/2)*(2k) + (s%2)*k
+  HIP_TRY(hodlr::launch_update(A, lda, k, E, lde, (int64_t)2 * k, nchunk / nslot, blk, nchunk,
+                               alpha, x, nd, st));
+  return HODLR_OK;
+}
+
+int hodlr_node_solve(int nnode, int r, const double* VTu, int64_t ldu, double* VTd, int64_t ldd,
+                     int nrhs, hipStream_t st) {
+  if (nnode <= 0 || r <= 0 || nrhs < 0 || !VTu || !VTd) return HODLR_ERR_ARG;
+  if (ldu < (int64_t)2 * r * nnode || ldd < (int64_t)2 * r * nnode) return HODLR_ERR_ARG;
+  const int n2 = 2 * r;
+  if (n2 > hodlr::kMaxBlock) return HODLR_ERR_UNSUPPORTED;
+  const int64_t nS = (int64_t)nnode * n2 * n2, nR = (int64_t)nnode * n2 * std::max(nrhs, 1);
+  const int64_t nD = (int64_t)nnode * hodlr::dinv_elems(n2);
+  void* w = nullptr;
+  int rc = workspace(sizeof(double) * (size_t)(nS + nR + nD) + sizeof(int) * (size_t)(nnode * (2 * n2 + 1) + 16), &w);
+  if (rc) return rc;
+  double* S = static_cast<double*>(w);
+  double* R = S + nS;
+  double* dinv = R + nR;
+  int* ipiv = reinterpret_cast<int*>(dinv + nD);
+  int* perm = ipiv + (int64_t)nnode * n2;
+  int* info = perm + (int64_t)nnode * n2;
+  HIP_TRY(hodlr::launch_node_assemble(nnode, r, VTu, ldu, VTd, ldd, nrhs, S, R, st));
+  CoreBufs c{S, R, ipiv, perm, info, dinv};
+  rc = core_factor_solve(nnode, r, c, nrhs, st);
+  if (rc) return rc;
+  // eta (R, node-major 2r x nrhs) back into the VTd slots
+  for (int k = 0; k < nnode; ++k)
+    HIP_TRY(hodlr::launch_copy_cols(n2, nrhs, R + (int64_t)k * n2 * nrhs, n2,
+                                    VTd + (int64_t)k * n2, ldd, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  std::vector<int> ci(nnode);
+  HIP_TRY(hipMemcpy(ci.data(), info, sizeof(int) * nnode, hipMemcpyDeviceToHost));
+  for (int k = 0; k < nnode; ++k)
+    if (ci[k]) return ci[k] + (k << 20);
+  return HODLR_OK;
+}
+
+// ---------------------------------------------------------------------
+// batched LAPACK
+// ---------------------------------------------------------------------
+int hodlr_dgetrf_batched(int n, double* A, int64_t lda, int64_t bstride, int batch, int* ipiv,
+                         int* info, hipStream_t st) {
+  if (n <= 0 || batch <= 0 || !A || !ipiv || !info || lda < n) return HODLR_ERR_ARG;
+  if (n > hodlr::kMaxBlock) return HODLR_ERR_UNSUPPORTED;
+  void* w = nullptr;
+  int rc = workspace(sizeof(int) * (size_t)batch * n, &w);
+  if (rc) return rc;
+  HIP_TRY(hodlr::launch_getrf(n, A, lda, bstride, batch, ipiv, static_cast<int*>(w), info, st));
+  return HODLR_OK;
+}
+
+int hodlr_dgesv_batched(int n, int nrhs, double* A, int64_t lda, int64_t bstride, int batch,
+                        int* ipiv, double* B, int64_t ldb, int64_t bbstride, hipStream_t st) {
+  if (n <= 0 || batch <= 0 || nrhs < 0 || !A || !ipiv || lda < n || (nrhs && (!B || ldb < n)))
+    return HODLR_ERR_ARG;
+  if (n > hodlr::kMaxBlock) return HODLR_ERR_UNSUPPORTED;
+  const int64_t nD = (int64_t)batch * hodlr::dinv_elems(n);
+  void* w = nullptr;
+  int rc = workspace(sizeof(double) * (size_t)nD + sizeof(int) * (size_t)batch * (n + 1) + 64, &w);
+  if (rc) return rc;
+  double* dinv = static_cast<double*>(w);
+  int* perm = reinterpret_cast<int*>(dinv + nD);
+  int* info = perm + (int64_t)batch * n;
+  HIP_TRY(hodlr::launch_getrf(n, A, lda, bstride, batch, ipiv, perm, info, st));
+  HIP_TRY(hodlr::launch_trtri_diag(n, A, lda, bstride, batch, dinv, st));
+  if (nrhs > 0) {
+    ColSeg x = hodlr::seg_single(B, ldb);
+    HIP_TRY(hodlr::launch_getrs(n, A, lda, bstride, batch, perm, dinv, x, bbstride, nrhs, st));
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  std::vector<int> hi(batch);
+  HIP_TRY(hipMemcpy(hi.data(), info, sizeof(int) * batch, hipMemcpyDeviceToHost));
+  for (int b = 0; b < batch; ++b)
+    if (hi[b]) return hi[b] + (b << 20);
+  return HODLR_OK;
+}
+
+// ---------------------------------------------------------------------
+// generators
+// ---------------------------------------------------------------------
+int hodlr_generate(uint64_t seed, uint32_t stream_id, uint32_t system, int kind, int64_t rows,
+                   int cols, double scale, double* out, int64_t ld, hipStream_t st) {
+  if (rows < 0 || cols < 0 || (kind != 0 && kind != 1) || (rows * (int64_t)cols > 0 && !out) ||
+      ld < rows)
+    return HODLR_ERR_ARG;
+  HIP_TRY(hodlr::launch_generate(seed, stream_id, system, kind, rows, cols, scale, out, ld, st));
+  return HODLR_OK;
+}
+
+int hodlr_shift_leaf_diagonal(int64_t n, int leaf, double shift, double* K, int64_t ldk,
+                              hipStream_t st) {
+  if (n <= 0 || leaf <= 0 || n % leaf || !K || ldk < n) return HODLR_ERR_ARG;
+  HIP_TRY(hodlr::launch_shift_diag(n, leaf, shift, K, ldk, st));
+  return HODLR_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,104 @@
+// hodlr_internal.hpp -- launchers shared by the C ABI (hodlr_capi.cpp) and
+// the kernels (hodlr_kernels.hip).  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace hodlr {
+
+// Column view of an operand made of two column segments [A | B]: column j
+// of the concatenation is read from inA (j < nA) or inB, and written to
+// outA / outB.  Either segment may alias its output (in-place update) or
+// point elsewhere (the Y_t -> Y_{t-1} update writes to a different level).
+// Row / batch offsets are added by the kernels.
+struct ColSeg {
+  const double* inA;
+  double* outA;
+  int64_t ldiA, ldoA;
+  int nA;
+  const double* inB;
+  double* outB;
+  int64_t ldiB, ldoB;
+};
+
+inline ColSeg seg_single(double* X, int64_t ld) {
+  ColSeg s;
+  s.inA = X; s.outA = X; s.ldiA = ld; s.ldoA = ld; s.nA = 1 << 30;
+  s.inB = nullptr; s.outB = nullptr; s.ldiB = 0; s.ldoB = 0;
+  return s;
+}
+
+// Largest n the batched LU / solve kernels accept (LDS-resident panel).
+constexpr int kMaxBlock = 1216;
+// Diagonal-block size of the stored triangular inverses.
+constexpr int kTri = 32;
+
+__host__ __device__ inline int ntri(int n) { return (n + kTri - 1) / kTri; }
+// doubles of diagonal-inverse storage per batch item
+__host__ __device__ inline int64_t dinv_elems(int n) { return (int64_t)ntri(n) * 2 * kTri * kTri; }
+
+// Batched LU with partial pivoting; perm receives the row permutation
+// (row i of the factored block is row perm[i] of the input), ipiv the
+// 1-based LAPACK IPIV interchanges, info the first zero pivot (1-based).
+hipError_t launch_getrf(int n, double* A, int64_t lda, int64_t bstride, int batch,
+                        int* ipiv, int* perm, int* info, hipStream_t st);
+// Inverts the 32x32 diagonal blocks of L (unit) and U of each LU.
+hipError_t launch_trtri_diag(int n, const double* LU, int64_t lda, int64_t bstride,
+                             int batch, double* dinv, hipStream_t st);
+// X_b <- U^-1 L^-1 P X_b for batch item b, columns [0, ncols) of X, where
+// item b's rows start at element offset b*xbstride of each segment.
+hipError_t launch_getrs(int n, const double* LU, int64_t lda, int64_t bstride, int batch,
+                        const int* perm, const double* dinv, const ColSeg& X,
+                        int64_t xbstride, int ncols, hipStream_t st);
+// Transposed solve X_b <- P^T L^-T U^-T X_b (dgetrs TRANS='T').
+hipError_t launch_getrs_t(int n, const double* LU, int64_t lda, int64_t bstride, int batch,
+                          const int* perm, const double* dinv, const ColSeg& X,
+                          int64_t xbstride, int ncols, hipStream_t st);
+
+// Split-K reduction over `nchild` consecutive row blocks of n_child rows:
+// part[child*cpc + q] (r x ncols, ld r) = V_q^T X_q over chunk q (of at most
+// rc rows) of that child.  Returns chunks per child in *cpc.
+hipError_t launch_reduce(const double* V, int64_t ldv, int r, const ColSeg& X, int ncols,
+                         int64_t n_child, int nchild, int64_t rc, double* part, int* cpc,
+                         hipStream_t st);
+int64_t reduce_chunks_per_child(int64_t n_child, int64_t rc);
+
+// Per node k (children 2k, 2k+1), summing each child's cpc partials in order:
+//   S_k = [I, G_1; G_0, I] (2r x 2r, ld 2r)  with G_c = part cols [gcol, gcol+r)
+//   R_k = [H_1; H_0]       (2r x nh, ld 2r)  with H_c = part cols [hcol, hcol+nh)
+// S may be null (solve with a stored core factorization).
+hipError_t launch_assemble(const double* part, int cpc, int r, int ncols_part, int gcol,
+                           int hcol, int nh, int nnode, double* S, double* R, hipStream_t st);
+
+// X_c[:, j] += alpha * u_c * eta_s[:, j] for row blocks c of n_child rows,
+// slot s = c / csz, eta_s = eta + (s/2)*eta_node_stride + (s%2)*r (ld eta_ld),
+// j < ncols.  HODLR levels use csz = 1 (slot = child).
+hipError_t launch_update(const double* u, int64_t ldu, int r, const double* eta,
+                         int64_t eta_ld, int64_t eta_node_stride, int csz, int64_t n_child,
+                         int nchild, double alpha, const ColSeg& X, int ncols, hipStream_t st);
+// S and swapped R for LMatrix::node_solve from VTu / VTd slots.
+hipError_t launch_node_assemble(int nnode, int r, const double* VTu, int64_t ldu,
+                                const double* VTd, int64_t ldd, int nrhs, double* S, double* R,
+                                hipStream_t st);
+
+// Slot-mapped reduce / broadcast for the LMatrix operators.
+hipError_t launch_sum_slots(const double* part, int nchunk, int nslot, int ka, int kb,
+                            double alpha, double* C, int64_t ldc, hipStream_t st);
+
+// HODLR matvec pieces.
+hipError_t launch_matvec(int64_t n, int leaf, int rank, int nrhs, const double* K,
+                         int64_t ldk, const double* U, int64_t ldu, const double* V,
+                         int64_t ldv, const double* X, int64_t ldx, double alpha, double beta,
+                         double* Y, int64_t ldy, double* work, hipStream_t st);
+int64_t matvec_work_elems(int64_t n, int leaf, int rank, int nrhs);
+
+hipError_t launch_generate(uint64_t seed, uint32_t stream_id, uint32_t system, int kind,
+                           int64_t rows, int cols, double scale, double* out, int64_t ld,
+                           hipStream_t st);
+hipError_t launch_shift_diag(int64_t n, int leaf, double shift, double* K, int64_t ldk,
+                             hipStream_t st);
+hipError_t launch_copy_cols(int64_t rows, int cols, const double* src, int64_t lds,
+                            double* dst, int64_t ldd, hipStream_t st);
+
+}  // namespace hodlr

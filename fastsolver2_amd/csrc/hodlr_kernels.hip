@@ -1,0 +1,998 @@
+// hodlr_kernels.hip -- gfx950 (MI355X) kernels of the HODLR factor-and-solve
+// hot path.  FP64 throughout; matrix products use the CDNA4 FP64 matrix core
+// instruction v_mfma_f64_16x16x4_f64 on 64-wide wavefronts.
+//
+// MFMA 16x16x4 f64 operand layout (lane l of a 64-lane wave):
+//   A (16x4):  A[l % 16][l / 16]          one double per lane
+//   B (4x16):  B[l / 16][l % 16]          one double per lane
+//   C/D:       C[4*(l/16) + v][l % 16]    v = 0..3, a 4-double vector
+//
+// Reference algorithm: Charles-Chao-Chen/fastSolver2 hsolve
+// (src/tasks/leaf_solve.cc:73-154) and the level loop gemmRed -> node_solve
+// -> gemmBro (spmd_driver/solver_irregular_part.cc:117-140).  The kernels
+// here are the batched, level-synchronous building blocks of that loop.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <climits>
+#include <cstdint>
+
+#include "hodlr_internal.hpp"
+
+namespace hodlr {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ d4 d4zero() {
+  d4 z = {0.0, 0.0, 0.0, 0.0};
+  return z;
+}
+
+__device__ __forceinline__ const double* seg_in(const ColSeg& s, int j) {
+  return j < s.nA ? s.inA + (int64_t)j * s.ldiA : s.inB + (int64_t)(j - s.nA) * s.ldiB;
+}
+__device__ __forceinline__ double* seg_out(const ColSeg& s, int j) {
+  return j < s.nA ? s.outA + (int64_t)j * s.ldoA : s.outB + (int64_t)(j - s.nA) * s.ldoB;
+}
+
+// =====================================================================
+// Batched LU with partial pivoting, one workgroup (4 waves) per block.
+// Right-looking, NB-column panels held in LDS; the panel is factored
+// column by column (LAPACK dgetf2/dgetrf2 pivot rule: first row of largest
+// |a|), interchanges are applied to the whole rows, U12 = L11^-1 A12 is an
+// MFMA product with the inverse of L11, and the trailing update
+// A22 -= L21 U12 runs on MFMA with L21 read from the LDS panel.
+// =====================================================================
+template <int NB>
+__global__ void __launch_bounds__(256)
+k_getrf(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
+        int* __restrict__ ipiv0, int* __restrict__ perm0, int* __restrict__ info0) {
+  extern __shared__ double lds[];
+  double* pan = lds;                 // [NB][n], column c at pan + c*n
+  double* prow = pan + NB * n;       // [NB] pivot row
+  double* orow = prow + NB;          // [NB] row displaced by the pivot
+  double* linv = orow + NB;          // [NB][NB+1] inverse of L11
+  double* redv = linv + NB * (NB + 1);  // [4]
+  int* redi = reinterpret_cast<int*>(redv + 4);  // [4]
+  int* pivs = redi + 4;              // [NB] panel-relative pivot rows
+  int* perm = pivs + NB;             // [n]
+
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  double* A = A0 + (int64_t)b * bstride;
+  int info = 0;  // meaningful in thread 0 only
+
+  for (int i = tid; i < n; i += 256) perm[i] = i;
+
+  for (int j0 = 0; j0 < n; j0 += NB) {
+    const int nb = min(NB, n - j0);
+    const int rows = n - j0;
+    // 1. load the panel rows [j0, n) x cols [j0, j0+nb)
+    for (int idx = tid; idx < nb * rows; idx += 256) {
+      const int c = idx / rows, i = idx - c * rows;
+      pan[c * n + i] = A[(int64_t)(j0 + i) + (int64_t)(j0 + c) * lda];
+    }
+    __syncthreads();
+
+    // 2. unblocked factorization of the panel
+    for (int jj = 0; jj < nb; ++jj) {
+      double best = -1.0;
+      int bi = INT_MAX;
+      for (int i = jj + tid; i < rows; i += 256) {
+        const double v = fabs(pan[jj * n + i]);
+        if (v > best) { best = v; bi = i; }
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_xor(best, off, 64);
+        const int oi = __shfl_xor(bi, off, 64);
+        if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+      }
+      if (lane == 0) { redv[wid] = best; redi[wid] = bi; }
+      __syncthreads();
+      best = redv[0];
+      bi = redi[0];
+#pragma unroll
+      for (int w = 1; w < 4; ++w) {
+        const double ov = redv[w];
+        const int oi = redi[w];
+        if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+      }
+      const int p = (bi < rows) ? bi : jj;
+      if (tid < nb) {
+        prow[tid] = pan[tid * n + p];
+        orow[tid] = pan[tid * n + jj];
+      }
+      if (tid == 0) pivs[jj] = p;
+      __syncthreads();
+      const double pv = prow[jj];
+      if (pv == 0.0) {
+        // whole column is zero: p == jj, nothing to swap or eliminate
+        if (tid == 0 && info == 0) info = j0 + jj + 1;
+        continue;
+      }
+      const bool use_rcp = fabs(pv) >= DBL_MIN;
+      const double rcp = 1.0 / pv;
+      for (int i = jj + tid; i < rows; i += 256) {
+        if (i == jj) {
+          for (int c = 0; c < nb; ++c) pan[c * n + jj] = prow[c];
+        } else {
+          const bool isp = (i == p);
+          double l = isp ? orow[jj] : pan[jj * n + i];
+          l = use_rcp ? l * rcp : l / pv;
+          pan[jj * n + i] = l;
+          if (isp)
+            for (int c = 0; c < jj; ++c) pan[c * n + i] = orow[c];
+          for (int c = jj + 1; c < nb; ++c) {
+            const double a = isp ? orow[c] : pan[c * n + i];
+            pan[c * n + i] = a - l * prow[c];
+          }
+        }
+      }
+      __syncthreads();
+    }
+
+    // 3. write the panel back, record pivots, update the permutation
+    for (int idx = tid; idx < nb * rows; idx += 256) {
+      const int c = idx / rows, i = idx - c * rows;
+      A[(int64_t)(j0 + i) + (int64_t)(j0 + c) * lda] = pan[c * n + i];
+    }
+    if (tid < nb) ipiv0[(int64_t)b * n + j0 + tid] = j0 + pivs[tid] + 1;  // 1-based
+    if (tid == 0) {
+      for (int jj = 0; jj < nb; ++jj) {
+        const int p = pivs[jj];
+        if (p != jj) {
+          const int t = perm[j0 + jj];
+          perm[j0 + jj] = perm[j0 + p];
+          perm[j0 + p] = t;
+        }
+      }
+    }
+
+    // 4. interchanges on the other columns (sequential in jj, as dlaswp)
+    for (int c = tid; c < n - nb; c += 256) {
+      const int col = c < j0 ? c : c + nb;
+      double* Ac = A + (int64_t)col * lda;
+      for (int jj = 0; jj < nb; ++jj) {
+        const int p = pivs[jj];
+        if (p != jj) {
+          const double t = Ac[j0 + jj];
+          Ac[j0 + jj] = Ac[j0 + p];
+          Ac[j0 + p] = t;
+        }
+      }
+    }
+    // 5a. linv = L11^-1 (unit lower, padded with the identity past nb):
+    // lane c < NB solves L11 x = e_c.
+    const int m2 = n - j0 - nb;
+    if (m2 > 0 && tid < NB) {
+      const int c = tid;
+      double* xc = linv + c * (NB + 1);  // column c of the inverse
+      for (int i = 0; i < NB; ++i) {
+        double acc = (i == c) ? 1.0 : 0.0;
+        if (i > c && i < nb)
+          for (int k = c; k < i; ++k) acc -= pan[k * n + i] * xc[k];
+        xc[i] = (i < c) ? 0.0 : acc;
+      }
+    }
+    __syncthreads();
+    // 5b. U12 = linv * A12 on MFMA; each wave owns whole 16-column tiles, so
+    // it reads a tile's 32 source rows before overwriting them.
+    if (m2 > 0) {
+      const int o = j0 + nb;
+      const int nct = (m2 + 15) >> 4;
+      for (int ct = wid; ct < nct; ct += 4) {
+        const int col = ct * 16 + (lane & 15);
+        d4 u0 = d4zero(), u1 = d4zero();
+#pragma unroll
+        for (int kk = 0; kk < NB / 4; ++kk) {
+          const int k = kk * 4 + (lane >> 4);
+          const double bf = (k < nb && col < m2) ? A[(int64_t)(j0 + k) + (int64_t)(o + col) * lda] : 0.0;
+          const double a0 = linv[(lane & 15) + k * (NB + 1)];
+          u0 = mfma(a0, bf, u0);
+          if (NB > 16) {
+            const double a1 = linv[16 + (lane & 15) + k * (NB + 1)];
+            u1 = mfma(a1, bf, u1);
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int r0 = 4 * (lane >> 4) + v;
+          if (col < m2 && r0 < nb) A[(int64_t)(j0 + r0) + (int64_t)(o + col) * lda] = u0[v];
+          if (NB > 16 && col < m2 && 16 + r0 < nb)
+            A[(int64_t)(j0 + 16 + r0) + (int64_t)(o + col) * lda] = u1[v];
+        }
+      }
+    }
+    __syncthreads();
+
+    // 6. trailing update A22 -= L21 * U12 (MFMA), 32x32 blocks per wave
+    if (m2 > 0) {
+      const int t32 = (m2 + 31) >> 5;
+      const int o = j0 + nb;
+      for (int blk = wid; blk < t32 * t32; blk += 4) {
+        const int r0 = (blk % t32) * 32, c0 = (blk / t32) * 32;
+        d4 acc[2][2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int bb = 0; bb < 2; ++bb) {
+            const int col = c0 + bb * 16 + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int row = r0 + a * 16 + 4 * (lane >> 4) + v;
+              acc[a][bb][v] = (row < m2 && col < m2)
+                                  ? A[(int64_t)(o + row) + (int64_t)(o + col) * lda]
+                                  : 0.0;
+            }
+          }
+#pragma unroll
+        for (int kk = 0; kk < NB / 4; ++kk) {
+          const int k = kk * 4 + (lane >> 4);
+          double af[2], bf[2];
+#pragma unroll
+          for (int a = 0; a < 2; ++a) {
+            const int row = r0 + a * 16 + (lane & 15);
+            af[a] = (k < nb && row < m2) ? -pan[k * n + nb + row] : 0.0;
+          }
+#pragma unroll
+          for (int bb = 0; bb < 2; ++bb) {
+            const int col = c0 + bb * 16 + (lane & 15);
+            bf[bb] = (k < nb && col < m2) ? A[(int64_t)(j0 + k) + (int64_t)(o + col) * lda] : 0.0;
+          }
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int bb = 0; bb < 2; ++bb) acc[a][bb] = mfma(af[a], bf[bb], acc[a][bb]);
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int bb = 0; bb < 2; ++bb) {
+            const int col = c0 + bb * 16 + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int row = r0 + a * 16 + 4 * (lane >> 4) + v;
+              if (row < m2 && col < m2)
+                A[(int64_t)(o + row) + (int64_t)(o + col) * lda] = acc[a][bb][v];
+            }
+          }
+      }
+    }
+    __syncthreads();
+  }
+
+  for (int i = tid; i < n; i += 256) perm0[(int64_t)b * n + i] = perm[i];
+  if (tid == 0) info0[b] = info;
+}
+
+static size_t getrf_lds_bytes(int nb, int n) {
+  return (size_t)(nb * n + 2 * nb + nb * (nb + 1) + 4) * sizeof(double) +
+         (size_t)(4 + nb + n) * sizeof(int);
+}
+
+hipError_t launch_getrf(int n, double* A, int64_t lda, int64_t bstride, int batch, int* ipiv,
+                        int* perm, int* info, hipStream_t st) {
+  if (batch <= 0 || n <= 0) return hipSuccess;
+  if (n > kMaxBlock) return hipErrorInvalidValue;
+  const size_t lim = 160 * 1024;
+  if (getrf_lds_bytes(32, n) <= lim) {
+    const size_t bytes = getrf_lds_bytes(32, n);
+    static bool attr = false;
+    if (!attr) {
+      (void)hipFuncSetAttribute((const void*)k_getrf<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)lim);
+      attr = true;
+    }
+    hipLaunchKernelGGL(k_getrf<32>, dim3(batch), dim3(256), bytes, st, A, lda, bstride, n, ipiv,
+                       perm, info);
+  } else {
+    const size_t bytes = getrf_lds_bytes(16, n);
+    if (bytes > lim) return hipErrorInvalidValue;
+    static bool attr = false;
+    if (!attr) {
+      (void)hipFuncSetAttribute((const void*)k_getrf<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)lim);
+      attr = true;
+    }
+    hipLaunchKernelGGL(k_getrf<16>, dim3(batch), dim3(256), bytes, st, A, lda, bstride, n, ipiv,
+                       perm, info);
+  }
+  return hipGetLastError();
+}
+
+// =====================================================================
+// Inverses of the 32x32 diagonal blocks of L (unit lower) and U (upper),
+// stored per block as [Linv | Uinv], each 32x32 column-major.  A partial
+// last block is padded with the identity.  Lane c < 32 computes column c
+// of Linv, lane 32 + c column c of Uinv (substitution on e_c).
+// =====================================================================
+__global__ void __launch_bounds__(64)
+k_trtri_diag(const double* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
+             double* __restrict__ dinv0) {
+  __shared__ double T[kTri][kTri + 1];  // T[col][row]
+  const int nblk = ntri(n);
+  const int b = blockIdx.x / nblk, ib = blockIdx.x % nblk;
+  const int i0 = ib * kTri;
+  const int bs = min(kTri, n - i0);
+  const double* LU = LU0 + (int64_t)b * bstride;
+  const int tid = threadIdx.x;
+  for (int idx = tid; idx < kTri * kTri; idx += 64) {
+    const int c = idx / kTri, r = idx % kTri;
+    double v;
+    if (r < bs && c < bs) v = LU[(int64_t)(i0 + r) + (int64_t)(i0 + c) * lda];
+    else v = (r == c) ? 1.0 : 0.0;
+    T[c][r] = v;
+  }
+  __syncthreads();
+  double* out = dinv0 + (int64_t)b * dinv_elems(n) + (int64_t)ib * 2 * kTri * kTri;
+  const int c = tid & 31;
+  double x[kTri];
+  if (tid < 32) {
+    // L x = e_c, unit lower
+#pragma unroll
+    for (int i = 0; i < kTri; ++i) {
+      double acc = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+      for (int k = 0; k < i; ++k) acc -= T[k][i] * x[k];
+      x[i] = (i < c) ? 0.0 : acc;
+    }
+#pragma unroll
+    for (int i = 0; i < kTri; ++i) out[i + c * kTri] = x[i];
+  } else {
+    // U x = e_c, upper non-unit
+#pragma unroll
+    for (int ii = 0; ii < kTri; ++ii) {
+      const int i = kTri - 1 - ii;
+      double acc = (i == c) ? 1.0 : 0.0;
+#pragma unroll
+      for (int k = i + 1; k < kTri; ++k) acc -= T[k][i] * x[k];
+      x[i] = (i > c) ? 0.0 : acc / T[i][i];
+    }
+#pragma unroll
+    for (int i = 0; i < kTri; ++i) out[kTri * kTri + i + c * kTri] = x[i];
+  }
+}
+
+hipError_t launch_trtri_diag(int n, const double* LU, int64_t lda, int64_t bstride, int batch,
+                             double* dinv, hipStream_t st) {
+  if (batch <= 0 || n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_trtri_diag, dim3(batch * ntri(n)), dim3(64), 0, st, LU, lda, bstride, n,
+                     dinv);
+  return hipGetLastError();
+}
+
+// =====================================================================
+// Batched solve with a stored LU (dgetrs), CW right-hand-side columns per
+// workgroup held in LDS.  Blocked substitution in 32-row blocks: the
+// diagonal block is applied as a product with its stored inverse, the
+// off-diagonal part as an MFMA update of the remaining rows.
+//   TRANS = false: X <- U^-1 L^-1 P^T X   (gather rows by perm)
+//   TRANS = true : X <- P U^-T ... i.e. solves A^T X = B (scatter by perm)
+// =====================================================================
+template <int CW, bool TRANS>
+__global__ void __launch_bounds__(256)
+k_getrs(const double* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
+        const int* __restrict__ perm0, const double* __restrict__ dinv0, ColSeg X,
+        int64_t xbstride, int ncols, int nchunk) {
+  extern __shared__ double lds[];
+  const int npad = ntri(n) * kTri;
+  double* Xl = lds;  // [CW][npad]
+  const int nblk = ntri(n);
+  // XCD-aware order: consecutive logical ids (chunks of one item) stay on
+  // one XCD so the item's LU factors are shared through that XCD's L2.
+  // Workgroup w runs on XCD w % 8; XCD x receives q + (x < rem) of the G
+  // workgroups, numbered contiguously -- a bijection of [0, G).
+  const int G = gridDim.x;
+  const int q = G >> 3, rem = G & 7;
+  const int x = blockIdx.x & 7;
+  const int logical = x * q + min(x, rem) + (blockIdx.x >> 3);
+  const int b = logical / nchunk, chunk = logical % nchunk;
+  const int c0 = chunk * CW;
+  const double* LU = LU0 + (int64_t)b * bstride;
+  const int* perm = perm0 + (int64_t)b * n;
+  const double* dinv = dinv0 + (int64_t)b * dinv_elems(n);
+  const int64_t xoff = (int64_t)b * xbstride;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  // gather
+  for (int idx = tid; idx < CW * npad; idx += 256) {
+    const int c = idx / npad, i = idx - c * npad;
+    const int col = c0 + c;
+    double v = 0.0;
+    if (i < n && col < ncols) {
+      const int src = TRANS ? i : perm[i];
+      v = seg_in(X, col)[xoff + src];
+    }
+    Xl[c * npad + i] = v;
+  }
+  __syncthreads();
+
+  constexpr int NCT = CW / 16;
+  // two triangular sweeps
+  for (int sweep = 0; sweep < 2; ++sweep) {
+    // sweep 0: lower (TRANS: U^T), forward; sweep 1: upper (TRANS: L^T), backward
+    const bool fwd = (sweep == 0);
+    // non-TRANS: sweep0 uses Linv (slot 0) & L; sweep1 uses Uinv (slot 1) & U
+    // TRANS    : sweep0 uses Uinv^T & U^T;         sweep1 uses Linv^T & L^T
+    const int islot = TRANS ? (fwd ? 1 : 0) : (fwd ? 0 : 1);
+    for (int step = 0; step < nblk; ++step) {
+      const int ib = fwd ? step : nblk - 1 - step;
+      const int i0 = ib * kTri;
+      const double* Di = dinv + (int64_t)ib * 2 * kTri * kTri + islot * kTri * kTri;
+      // (a) x_blk <- D * x_blk  (D = inverse of the diagonal block, or its
+      // transpose).  The 32 x CW block is 2*NCT <= 4 MFMA tiles: one per wave.
+      const bool has_tile = wid < 2 * NCT;
+      const int rt_a = wid & 1, ct_a = wid >> 1;
+      d4 a4 = d4zero();
+      if (has_tile) {
+#pragma unroll
+        for (int kk = 0; kk < kTri / 4; ++kk) {
+          const int k = kk * 4 + lk;
+          const int row = rt_a * 16 + lr;
+          const double af = TRANS ? Di[k + row * kTri] : Di[row + k * kTri];
+          const double bf = Xl[(ct_a * 16 + lr) * npad + i0 + k];
+          a4 = mfma(af, bf, a4);
+        }
+      }
+      __syncthreads();
+      if (has_tile) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          Xl[(ct_a * 16 + lr) * npad + i0 + rt_a * 16 + 4 * lk + v] = a4[v];
+      }
+      __syncthreads();
+      // (b) rows outside the block: fwd -> rows [i0+32, npad), bwd -> rows [0, i0)
+      const int rlo = fwd ? i0 + kTri : 0;
+      const int rhi = fwd ? npad : i0;
+      const int nrt = (rhi - rlo) >> 4;
+      for (int t = wid; t < nrt * NCT; t += 4) {
+        const int rt = t % nrt, ct = t / nrt;
+        const int rbase = rlo + rt * 16;
+        d4 a4;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) a4[v] = Xl[(ct * 16 + lr) * npad + rbase + 4 * lk + v];
+#pragma unroll
+        for (int kk = 0; kk < kTri / 4; ++kk) {
+          const int k = kk * 4 + lk;
+          const int row = rbase + lr;      // row of the operand block
+          const int kcol = i0 + k;         // column of the operand block
+          double af = 0.0;
+          if (row < n && kcol < n) {
+            // non-TRANS: L/U[row][kcol];  TRANS: (U/L)^T[row][kcol] = LU[kcol][row]
+            af = TRANS ? LU[(int64_t)kcol + (int64_t)row * lda] : LU[(int64_t)row + (int64_t)kcol * lda];
+          }
+          const double bf = Xl[(ct * 16 + lr) * npad + i0 + k];
+          a4 = mfma(-af, bf, a4);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Xl[(ct * 16 + lr) * npad + rbase + 4 * lk + v] = a4[v];
+      }
+      __syncthreads();
+    }
+  }
+
+  // scatter
+  for (int idx = tid; idx < CW * npad; idx += 256) {
+    const int c = idx / npad, i = idx - c * npad;
+    const int col = c0 + c;
+    if (i < n && col < ncols) {
+      const int dst = TRANS ? perm[i] : i;
+      seg_out(X, col)[xoff + dst] = Xl[c * npad + i];
+    }
+  }
+}
+
+template <bool TRANS>
+static hipError_t launch_getrs_impl(int n, const double* LU, int64_t lda, int64_t bstride,
+                                    int batch, const int* perm, const double* dinv,
+                                    const ColSeg& X, int64_t xbstride, int ncols,
+                                    hipStream_t st) {
+  if (batch <= 0 || n <= 0 || ncols <= 0) return hipSuccess;
+  if (n > kMaxBlock) return hipErrorInvalidValue;
+  const int npad = ntri(n) * kTri;
+  const size_t lim = 160 * 1024;
+  // 32 columns per workgroup when they fit, else 16
+  if ((size_t)32 * npad * sizeof(double) <= lim && ncols > 16) {
+    const int nchunk = (ncols + 31) / 32;
+    static bool attr = false;
+    if (!attr) {
+      (void)hipFuncSetAttribute((const void*)k_getrs<32, TRANS>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim);
+      attr = true;
+    }
+    hipLaunchKernelGGL((k_getrs<32, TRANS>), dim3(batch * nchunk), dim3(256),
+                       (size_t)32 * npad * sizeof(double), st, LU, lda, bstride, n, perm, dinv,
+                       X, xbstride, ncols, nchunk);
+  } else {
+    const int nchunk = (ncols + 15) / 16;
+    if ((size_t)16 * npad * sizeof(double) > lim) return hipErrorInvalidValue;
+    static bool attr = false;
+    if (!attr) {
+      (void)hipFuncSetAttribute((const void*)k_getrs<16, TRANS>,
+                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lim);
+      attr = true;
+    }
+    hipLaunchKernelGGL((k_getrs<16, TRANS>), dim3(batch * nchunk), dim3(256),
+                       (size_t)16 * npad * sizeof(double), st, LU, lda, bstride, n, perm, dinv,
+                       X, xbstride, ncols, nchunk);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_getrs(int n, const double* LU, int64_t lda, int64_t bstride, int batch,
+                        const int* perm, const double* dinv, const ColSeg& X, int64_t xbstride,
+                        int ncols, hipStream_t st) {
+  return launch_getrs_impl<false>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
+}
+hipError_t launch_getrs_t(int n, const double* LU, int64_t lda, int64_t bstride, int batch,
+                          const int* perm, const double* dinv, const ColSeg& X, int64_t xbstride,
+                          int ncols, hipStream_t st) {
+  return launch_getrs_impl<true>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
+}
+
+// =====================================================================
+// Level reduce ("gemmRed", gemm_reduce.cc:58-76): split-K V^T X over row
+// chunks that never straddle a child, one partial per chunk, summed later
+// in fixed chunk order (deterministic, unlike the reference's REDOP_ADD
+// fold).  Workgroup tile: 64 rows of V^T (one 16-row MFMA tile per wave)
+// x NCT*16 columns of X; K staged through LDS 32 rows at a time with
+// coalesced column loads.
+// =====================================================================
+template <int NCT>
+__global__ void __launch_bounds__(256)
+k_reduce(const double* __restrict__ V, int64_t ldv, int r, ColSeg X, int ncols,
+         int64_t n_child, int cpc, int64_t rc, double* __restrict__ part) {
+  constexpr int KS = 32;
+  __shared__ double Vs[64 * (KS + 1)];
+  __shared__ double Xs[NCT * 16 * (KS + 1)];
+  const int chunk = blockIdx.x;
+  const int child = chunk / cpc, q = chunk % cpc;
+  const int64_t rbeg = (int64_t)child * n_child + (int64_t)q * rc;
+  const int64_t cend = (int64_t)(child + 1) * n_child;
+  const int64_t rend = rbeg + rc < cend ? rbeg + rc : cend;
+  const int j0 = blockIdx.y * NCT * 16;
+  const int a0 = blockIdx.z * 64;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  d4 acc[NCT];
+#pragma unroll
+  for (int ct = 0; ct < NCT; ++ct) acc[ct] = d4zero();
+
+  for (int64_t k0 = rbeg; k0 < rend; k0 += KS) {
+    for (int idx = tid; idx < 64 * KS; idx += 256) {
+      const int a = idx / KS, k = idx % KS;
+      const int64_t row = k0 + k;
+      Vs[a * (KS + 1) + k] = (a0 + a < r && row < rend) ? V[row + (int64_t)(a0 + a) * ldv] : 0.0;
+    }
+    for (int idx = tid; idx < NCT * 16 * KS; idx += 256) {
+      const int j = idx / KS, k = idx % KS;
+      const int64_t row = k0 + k;
+      Xs[j * (KS + 1) + k] = (j0 + j < ncols && row < rend) ? seg_in(X, j0 + j)[row] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < KS / 4; ++kk) {
+      const double af = Vs[(wid * 16 + lr) * (KS + 1) + kk * 4 + lk];
+#pragma unroll
+      for (int ct = 0; ct < NCT; ++ct) {
+        if (j0 + ct * 16 < ncols) {
+          const double bf = Xs[(ct * 16 + lr) * (KS + 1) + kk * 4 + lk];
+          acc[ct] = mfma(af, bf, acc[ct]);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  double* P = part + (int64_t)chunk * r * ncols;
+#pragma unroll
+  for (int ct = 0; ct < NCT; ++ct) {
+    const int col = j0 + ct * 16 + lr;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int a = a0 + wid * 16 + 4 * lk + v;
+      if (a < r && col < ncols) P[a + (int64_t)col * r] = acc[ct][v];
+    }
+  }
+}
+
+int64_t reduce_chunks_per_child(int64_t n_child, int64_t rc) { return (n_child + rc - 1) / rc; }
+
+hipError_t launch_reduce(const double* V, int64_t ldv, int r, const ColSeg& X, int ncols,
+                         int64_t n_child, int nchild, int64_t rc, double* part, int* cpc_out,
+                         hipStream_t st) {
+  const int64_t cpc = reduce_chunks_per_child(n_child, rc);
+  if (cpc_out) *cpc_out = (int)cpc;
+  if (nchild <= 0 || ncols <= 0 || r <= 0) return hipSuccess;
+  dim3 grid((unsigned)(nchild * cpc), (unsigned)((ncols + 127) / 128), (unsigned)((r + 63) / 64));
+  hipLaunchKernelGGL(k_reduce<8>, grid, dim3(256), 0, st, V, ldv, r, X, ncols, n_child, (int)cpc,
+                     rc, part);
+  return hipGetLastError();
+}
+
+// =====================================================================
+// Core assembly (node_solve.cc:60-81): sum each child's partials in chunk
+// order, then S = [I, G_1; G_0, I] and R = [H_1; H_0] (halves swapped).
+// =====================================================================
+__device__ __forceinline__ double sum_part(const double* part, int child, int cpc, int64_t pstride,
+                                           int64_t off) {
+  double s = 0.0;
+  const double* p = part + (int64_t)child * cpc * pstride + off;
+  for (int q = 0; q < cpc; ++q) s += p[(int64_t)q * pstride];
+  return s;
+}
+
+__global__ void __launch_bounds__(256)
+k_assemble(const double* __restrict__ part, int cpc, int r, int ncols_part, int gcol, int hcol,
+           int nh, double* __restrict__ S, double* __restrict__ R) {
+  const int node = blockIdx.y;
+  const int n2 = 2 * r;
+  const int64_t pstride = (int64_t)r * ncols_part;
+  const int64_t nS = S ? (int64_t)n2 * n2 : 0;
+  const int64_t total = nS + (int64_t)n2 * nh;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * 256) {
+    if (e < nS) {
+      const int i = (int)(e % n2), j = (int)(e / n2);
+      double v;
+      if ((i < r) == (j < r)) {
+        v = (i == j) ? 1.0 : 0.0;
+      } else if (i >= r) {  // S(r+a, j) = V0^T u0 (a, j)
+        v = sum_part(part, 2 * node, cpc, pstride, (i - r) + (int64_t)(gcol + j) * r);
+      } else {              // S(a, r+j) = V1^T u1 (a, j)
+        v = sum_part(part, 2 * node + 1, cpc, pstride, i + (int64_t)(gcol + j - r) * r);
+      }
+      S[(int64_t)node * nS + i + (int64_t)j * n2] = v;
+    } else {
+      const int64_t f = e - nS;
+      const int i = (int)(f % n2), j = (int)(f / n2);
+      const double v = (i < r)
+          ? sum_part(part, 2 * node + 1, cpc, pstride, i + (int64_t)(hcol + j) * r)
+          : sum_part(part, 2 * node, cpc, pstride, (i - r) + (int64_t)(hcol + j) * r);
+      R[(int64_t)node * n2 * nh + i + (int64_t)j * n2] = v;
+    }
+  }
+}
+
+hipError_t launch_assemble(const double* part, int cpc, int r, int ncols_part, int gcol, int hcol,
+                           int nh, int nnode, double* S, double* R, hipStream_t st) {
+  if (nnode <= 0) return hipSuccess;
+  const int64_t total = (S ? 4LL * r * r : 0) + 2LL * r * nh;
+  int gx = (int)((total + 255) / 256);
+  if (gx > 64) gx = 64;
+  if (gx < 1) gx = 1;
+  hipLaunchKernelGGL(k_assemble, dim3(gx, nnode), dim3(256), 0, st, part, cpc, r, ncols_part,
+                     gcol, hcol, nh, S, R);
+  return hipGetLastError();
+}
+
+// =====================================================================
+// Level update ("gemmBro", gemm_broadcast.cc:58-82):
+//   X_c[:, j] += alpha * u_c * eta_c[:, j]
+// 64-row groups that never straddle a child (one 16-row MFMA tile per
+// wave) x NCT*16 columns; u streamed as the A operand, eta as B.
+// =====================================================================
+template <int NCT>
+__global__ void __launch_bounds__(256)
+k_update(const double* __restrict__ u, int64_t ldu, int r, const double* __restrict__ eta,
+         int64_t eta_ld, int64_t eta_ns, int csz, int64_t n_child, int gpc, double alpha,
+         ColSeg X, int ncols) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int child = blockIdx.x / gpc, g = blockIdx.x % gpc;
+  const int64_t cend = (int64_t)(child + 1) * n_child;
+  const int64_t row0 = (int64_t)child * n_child + (int64_t)g * 64 + wid * 16;
+  const int j0 = blockIdx.y * NCT * 16;
+  const int slot = child / csz;
+  const double* E = eta + (int64_t)(slot >> 1) * eta_ns + (int64_t)(slot & 1) * r;
+  d4 acc[NCT];
+#pragma unroll
+  for (int ct = 0; ct < NCT; ++ct) {
+    const int col = j0 + ct * 16 + lr;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t row = row0 + 4 * lk + v;
+      acc[ct][v] = (row < cend && col < ncols) ? seg_in(X, col)[row] : 0.0;
+    }
+  }
+  for (int k0 = 0; k0 < r; k0 += 4) {
+    const int k = k0 + lk;
+    const int64_t arow = row0 + lr;
+    const double af = (k < r && arow < cend) ? alpha * u[arow + (int64_t)k * ldu] : 0.0;
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) {
+      if (j0 + ct * 16 < ncols) {
+        const int col = j0 + ct * 16 + lr;
+        const double bf = (k < r && col < ncols) ? E[k + (int64_t)col * eta_ld] : 0.0;
+        acc[ct] = mfma(af, bf, acc[ct]);
+      }
+    }
+  }
+#pragma unroll
+  for (int ct = 0; ct < NCT; ++ct) {
+    const int col = j0 + ct * 16 + lr;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t row = row0 + 4 * lk + v;
+      if (row < cend && col < ncols) seg_out(X, col)[row] = acc[ct][v];
+    }
+  }
+}
+
+hipError_t launch_update(const double* u, int64_t ldu, int r, const double* eta, int64_t eta_ld,
+                         int64_t eta_node_stride, int csz, int64_t n_child, int nchild,
+                         double alpha, const ColSeg& X, int ncols, hipStream_t st) {
+  if (nchild <= 0 || ncols <= 0) return hipSuccess;
+  const int64_t gpc = (n_child + 63) / 64;
+  dim3 grid((unsigned)(nchild * gpc), (unsigned)((ncols + 127) / 128));
+  hipLaunchKernelGGL(k_update<8>, grid, dim3(256), 0, st, u, ldu, r, eta, eta_ld,
+                     eta_node_stride, csz, n_child, (int)gpc, alpha, X, ncols);
+  return hipGetLastError();
+}
+
+// LMatrix::node_solve operands (node_solve.cc:60-81): per node slot k,
+// S = [I, VTu(2kr+r : 2kr+2r, :); VTu(2kr : 2kr+r, :), I] and R = VTd node
+// rows with the halves swapped.
+__global__ void k_node_assemble(int r, const double* __restrict__ VTu, int64_t ldu,
+                                const double* __restrict__ VTd, int64_t ldd, int nrhs,
+                                double* __restrict__ S, double* __restrict__ R) {
+  const int node = blockIdx.y;
+  const int n2 = 2 * r;
+  const int64_t nS = (int64_t)n2 * n2;
+  const int64_t total = nS + (int64_t)n2 * nrhs;
+  const int64_t base = (int64_t)node * n2;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * 256) {
+    if (e < nS) {
+      const int i = (int)(e % n2), j = (int)(e / n2);
+      double v;
+      if ((i < r) == (j < r)) v = (i == j) ? 1.0 : 0.0;
+      else if (i >= r) v = VTu[base + (i - r) + (int64_t)j * ldu];
+      else v = VTu[base + r + i + (int64_t)(j - r) * ldu];
+      S[(int64_t)node * nS + i + (int64_t)j * n2] = v;
+    } else {
+      const int64_t f = e - nS;
+      const int i = (int)(f % n2), j = (int)(f / n2);
+      const int src = (i < r) ? r + i : i - r;
+      R[(int64_t)node * n2 * nrhs + i + (int64_t)j * n2] = VTd[base + src + (int64_t)j * ldd];
+    }
+  }
+}
+
+hipError_t launch_node_assemble(int nnode, int r, const double* VTu, int64_t ldu, const double* VTd,
+                                int64_t ldd, int nrhs, double* S, double* R, hipStream_t st) {
+  if (nnode <= 0) return hipSuccess;
+  const int64_t total = 4LL * r * r + 2LL * r * nrhs;
+  int gx = (int)((total + 255) / 256);
+  if (gx > 64) gx = 64;
+  hipLaunchKernelGGL(k_node_assemble, dim3(gx, nnode), dim3(256), 0, st, r, VTu, ldu, VTd, ldd,
+                     nrhs, S, R);
+  return hipGetLastError();
+}
+
+// Slot sums for LMatrix::gemmRed: C[slot*ka + a, j] = alpha * sum of the
+// nper consecutive partials (ka x kb, ld ka) that belong to the slot.
+__global__ void k_sum_slots(const double* __restrict__ part, int nper, int nslot, int ka, int kb,
+                            double alpha, double* __restrict__ C, int64_t ldc) {
+  const int64_t total = (int64_t)nslot * ka * kb;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * 256) {
+    const int a = (int)(e % ka);
+    const int64_t t = e / ka;
+    const int j = (int)(t % kb), slot = (int)(t / kb);
+    const double* p = part + (int64_t)slot * nper * ka * kb + a + (int64_t)j * ka;
+    double s = 0.0;
+    for (int q = 0; q < nper; ++q) s += p[(int64_t)q * ka * kb];
+    C[(int64_t)slot * ka + a + (int64_t)j * ldc] = alpha * s;
+  }
+}
+
+hipError_t launch_sum_slots(const double* part, int nper, int nslot, int ka, int kb, double alpha,
+                            double* C, int64_t ldc, hipStream_t st) {
+  const int64_t total = (int64_t)nslot * ka * kb;
+  if (total <= 0) return hipSuccess;
+  int g = (int)((total + 255) / 256);
+  if (g > 1024) g = 1024;
+  hipLaunchKernelGGL(k_sum_slots, dim3(g), dim3(256), 0, st, part, nper, nslot, ka, kb, alpha, C,
+                     ldc);
+  return hipGetLastError();
+}
+
+// =====================================================================
+// HODLR matvec: part_i = V_i^T x_i (k_reduce with one chunk per leaf),
+// tot = sum_i part_i (fixed order), then per leaf
+//   y_i = alpha * (K_i x_i + U_i (tot - part_i)) + beta * y_i.
+// =====================================================================
+__global__ void k_mv_total(const double* __restrict__ part, int nleaf, int64_t elems,
+                           double* __restrict__ tot) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= elems) return;
+  double s = 0.0;
+  for (int i = 0; i < nleaf; ++i) s += part[(int64_t)i * elems + e];
+  tot[e] = s;
+}
+
+template <int NCT>
+__global__ void __launch_bounds__(256)
+k_mv_apply(int leaf, int r, int nrhs, const double* __restrict__ K, int64_t ldk,
+           const double* __restrict__ U, int64_t ldu, const double* __restrict__ X, int64_t ldx,
+           const double* __restrict__ part, const double* __restrict__ tot, double alpha,
+           double beta, double* __restrict__ Y, int64_t ldy, int gpl) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int li = blockIdx.x / gpl, g = blockIdx.x % gpl;
+  const int64_t l0 = (int64_t)li * leaf;
+  const int rloc = g * 64 + wid * 16;  // row within the leaf
+  const int j0 = blockIdx.y * NCT * 16;
+  const double* P = part + (int64_t)li * r * nrhs;
+  d4 acc[NCT];
+#pragma unroll
+  for (int ct = 0; ct < NCT; ++ct) acc[ct] = d4zero();
+  for (int k0 = 0; k0 < leaf; k0 += 4) {
+    const int k = k0 + lk;
+    const int arow = rloc + lr;
+    const double af = (k < leaf && arow < leaf) ? K[l0 + arow + (int64_t)k * ldk] : 0.0;
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) {
+      if (j0 + ct * 16 < nrhs) {
+        const int col = j0 + ct * 16 + lr;
+        const double bf = (k < leaf && col < nrhs) ? X[l0 + k + (int64_t)col * ldx] : 0.0;
+        acc[ct] = mfma(af, bf, acc[ct]);
+      }
+    }
+  }
+  for (int k0 = 0; k0 < r; k0 += 4) {
+    const int k = k0 + lk;
+    const int arow = rloc + lr;
+    const double af = (k < r && arow < leaf) ? U[l0 + arow + (int64_t)k * ldu] : 0.0;
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) {
+      if (j0 + ct * 16 < nrhs) {
+        const int col = j0 + ct * 16 + lr;
+        double bf = 0.0;
+        if (k < r && col < nrhs) bf = tot[k + (int64_t)col * r] - P[k + (int64_t)col * r];
+        acc[ct] = mfma(af, bf, acc[ct]);
+      }
+    }
+  }
+#pragma unroll
+  for (int ct = 0; ct < NCT; ++ct) {
+    const int col = j0 + ct * 16 + lr;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = rloc + 4 * lk + v;
+      if (row < leaf && col < nrhs) {
+        double* y = Y + l0 + row + (int64_t)col * ldy;
+        const double prev = (beta == 0.0) ? 0.0 : beta * *y;
+        *y = alpha * acc[ct][v] + prev;
+      }
+    }
+  }
+}
+
+int64_t matvec_work_elems(int64_t n, int leaf, int rank, int nrhs) {
+  const int64_t nleaf = n / leaf;
+  return (nleaf + 1) * (int64_t)rank * nrhs;
+}
+
+hipError_t launch_matvec(int64_t n, int leaf, int rank, int nrhs, const double* K, int64_t ldk,
+                         const double* U, int64_t ldu, const double* V, int64_t ldv,
+                         const double* X, int64_t ldx, double alpha, double beta, double* Y,
+                         int64_t ldy, double* work, hipStream_t st) {
+  const int64_t nleaf = n / leaf;
+  double* part = work;
+  double* tot = work + nleaf * (int64_t)rank * nrhs;
+  ColSeg xs = seg_single(const_cast<double*>(X), ldx);
+  hipError_t e = launch_reduce(V, ldv, rank, xs, nrhs, leaf, (int)nleaf, leaf, part, nullptr, st);
+  if (e != hipSuccess) return e;
+  const int64_t elems = (int64_t)rank * nrhs;
+  hipLaunchKernelGGL(k_mv_total, dim3((unsigned)((elems + 255) / 256)), dim3(256), 0, st, part,
+                     (int)nleaf, elems, tot);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  const int gpl = (leaf + 63) / 64;
+  dim3 grid((unsigned)(nleaf * gpl), (unsigned)((nrhs + 63) / 64));
+  hipLaunchKernelGGL(k_mv_apply<4>, grid, dim3(256), 0, st, leaf, rank, nrhs, K, ldk, U, ldu, X,
+                     ldx, part, tot, alpha, beta, Y, ldy, gpl);
+  return hipGetLastError();
+}
+
+// =====================================================================
+// Workload generator: Philox4x64-10 (Salmon et al., SC'11), one counter
+// block (4 words) per thread.
+// =====================================================================
+__device__ __forceinline__ void philox4x64(uint64_t c[4], uint64_t k0, uint64_t k1) {
+  const uint64_t M0 = 0xD2E7470EE14C6C93ULL, M1 = 0xCA5A826395121157ULL;
+  const uint64_t W0 = 0x9E3779B97F4A7C15ULL, W1 = 0xBB67AE8584CAA73BULL;
+#pragma unroll
+  for (int rnd = 0; rnd < 10; ++rnd) {
+    if (rnd > 0) { k0 += W0; k1 += W1; }
+    const uint64_t hi0 = __umul64hi(M0, c[0]), lo0 = M0 * c[0];
+    const uint64_t hi1 = __umul64hi(M1, c[2]), lo1 = M1 * c[2];
+    const uint64_t n0 = hi1 ^ c[1] ^ k0, n1 = lo1, n2 = hi0 ^ c[3] ^ k1, n3 = lo0;
+    c[0] = n0; c[1] = n1; c[2] = n2; c[3] = n3;
+  }
+}
+
+__global__ void k_generate(uint64_t seed, uint32_t sid, uint32_t system, int kind, int64_t rows,
+                           int cols, double scale, double* __restrict__ out, int64_t ld) {
+  const int64_t total = rows * (int64_t)cols;
+  const int64_t nblk = (total + 3) / 4;
+  for (int64_t blk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; blk < nblk;
+       blk += (int64_t)gridDim.x * blockDim.x) {
+    uint64_t w[4] = {(uint64_t)blk, sid, system, 0};
+    philox4x64(w, seed, 0);
+    double z[4];
+    if (kind == 0) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) z[q] = (double)(w[q] >> 11) * 0x1.0p-53;
+    } else {
+      const double two_pi = 6.283185307179586476925286766559;
+#pragma unroll
+      for (int pr = 0; pr < 2; ++pr) {
+        const double u1 = (double)((w[2 * pr] >> 11) + 1) * 0x1.0p-53;
+        const double u2 = (double)(w[2 * pr + 1] >> 11) * 0x1.0p-53;
+        const double rad = sqrt(-2.0 * log(u1));
+        z[2 * pr] = rad * cos(two_pi * u2);
+        z[2 * pr + 1] = rad * sin(two_pi * u2);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int64_t e = blk * 4 + q;
+      if (e < total) out[(e % rows) + (e / rows) * ld] = z[q] * scale;
+    }
+  }
+}
+
+hipError_t launch_generate(uint64_t seed, uint32_t stream_id, uint32_t system, int kind,
+                           int64_t rows, int cols, double scale, double* out, int64_t ld,
+                           hipStream_t st) {
+  const int64_t nblk = (rows * (int64_t)cols + 3) / 4;
+  if (nblk <= 0) return hipSuccess;
+  int64_t g = (nblk + 255) / 256;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(k_generate, dim3((unsigned)g), dim3(256), 0, st, seed, stream_id, system,
+                     kind, rows, cols, scale, out, ld);
+  return hipGetLastError();
+}
+
+__global__ void k_shift_diag(int64_t n, int leaf, double shift, double* __restrict__ K,
+                             int64_t ldk) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) K[i + (i % leaf) * ldk] += shift;
+}
+
+hipError_t launch_shift_diag(int64_t n, int leaf, double shift, double* K, int64_t ldk,
+                             hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_shift_diag, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, leaf,
+                     shift, K, ldk);
+  return hipGetLastError();
+}
+
+__global__ void k_copy_cols(int64_t rows, int cols, const double* __restrict__ src, int64_t lds,
+                            double* __restrict__ dst, int64_t ldd) {
+  const int64_t total = rows * (int64_t)cols;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * 256) {
+    const int64_t i = e % rows, j = e / rows;
+    dst[i + j * ldd] = src[i + j * lds];
+  }
+}
+
+hipError_t launch_copy_cols(int64_t rows, int cols, const double* src, int64_t lds, double* dst,
+                            int64_t ldd, hipStream_t st) {
+  const int64_t total = rows * (int64_t)cols;
+  if (total <= 0) return hipSuccess;
+  int64_t g = (total + 255) / 256;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(k_copy_cols, dim3((unsigned)g), dim3(256), 0, st, rows, cols, src, lds, dst,
+                     ldd);
+  return hipGetLastError();
+}
+
+}  // namespace hodlr

@@ -1,0 +1,204 @@
+/*
+ * hodlr.h -- C ABI of the MI355X-native HODLR factor-and-solve library
+ *            (libhodlr_amd.so, built from fastsolver2_amd/csrc).
+ *
+ * Drop-in boundary for the hot path of Charles-Chao-Chen/fastSolver2: each
+ * entry point names the reference interface it replaces.  Conventions follow
+ * the reference (src/ptr_matrix.cc:29-37, include/lapack_blas.hpp:28-57):
+ *   - FP64, column-major, element (r,c) at ptr[r + c*ld];
+ *   - the HODLR matrix of size N = leaf * 2^levels has dense leaf blocks K_i
+ *     (K is N x leaf, leaf i = rows [i*leaf,(i+1)*leaf)) and every
+ *     off-diagonal block at every level equal to U(I,:) V(J,:)^T with global
+ *     N x rank matrices U, V (src/tree.cc:59, :189-197);
+ *   - the solution overwrites the right-hand side ("b will be overwritten by
+ *     x", src/lmatrix.cc:447-448), and the leaf blocks K are overwritten by
+ *     their LU factors, as dgesv does.
+ * All pointers passed to entry points taking a hipStream_t are DEVICE
+ * pointers; work is enqueued on that stream and the call returns without
+ * synchronizing unless stated.  Nothing here aborts: every entry point
+ * returns a status (0 = success, < 0 = bad argument / runtime error,
+ * > 0 = numerically singular block, see hodlr_plan_status()).
+ */
+#ifndef HODLR_H
+#define HODLR_H
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define HODLR_OK                 0
+#define HODLR_ERR_ARG           -1   /* an argument is out of range          */
+#define HODLR_ERR_SHAPE         -2   /* N != leaf * 2^levels, leaf <= rank...  */
+#define HODLR_ERR_UNSUPPORTED   -3   /* size beyond what the kernels handle   */
+#define HODLR_ERR_HIP           -4   /* a HIP runtime call failed             */
+#define HODLR_ERR_NOMEM         -5   /* device allocation failed              */
+#define HODLR_ERR_STATE         -6   /* solve before factor, etc.             */
+#define HODLR_SINGULAR_LEAF      1   /* zero pivot in a leaf LU               */
+#define HODLR_SINGULAR_CORE      2   /* zero pivot in a 2r x 2r core LU       */
+
+/* Human-readable text for a status code (static storage, never NULL). */
+const char *hodlr_strerror(int status);
+/* Detail of the last HODLR_ERR_HIP / HODLR_ERR_NOMEM on this thread. */
+const char *hodlr_last_error_detail(void);
+
+/* ---------------------------------------------------------------- plans */
+enum hodlr_precision { HODLR_FP64 = 0, HODLR_MIXED_IR = 1 };
+
+typedef struct hodlr_desc {
+  int64_t n;          /* N = leaf << levels                                  */
+  int     leaf;       /* leaf size m (reference -leaf, solver.cc:234)        */
+  int     levels;     /* tree levels L (reference -mtxlvl)                   */
+  int     rank;       /* off-diagonal rank r (reference -rank)               */
+  int     nrhs_max;   /* largest number of right-hand sides solved at once   */
+  int     precision;  /* enum hodlr_precision                                */
+  int     device;     /* HIP device ordinal                                  */
+  int     flags;      /* reserved, must be 0                                 */
+} hodlr_desc;
+
+typedef struct hodlr_plan_s *hodlr_plan;
+
+/* Replaces HMatrix::HMatrix(nProc, level) + UTree/VTree/KTree::init
+ * (include/hmatrix.hpp:12-18, src/hmatrix.cc:22-47): validates the shape and
+ * allocates all factor storage on the device (per-level Y_t = A_c^-1 U
+ * blocks, leaf/core pivots, core LU factors).  The plan keeps pointers to
+ * the K, U, V passed to hodlr_factor; they must stay allocated until the
+ * plan is destroyed or factored again. */
+int hodlr_plan_create(hodlr_plan *plan, const hodlr_desc *desc);
+/* Replaces HMatrix::destroy (include/hmatrix.hpp:23, declared but never
+ * defined in the reference) / LMatrix::clear (src/lmatrix.cc:886-890). */
+int hodlr_plan_destroy(hodlr_plan plan);
+/* Device bytes the plan owns (factor storage + workspace). */
+int64_t hodlr_plan_device_bytes(hodlr_plan plan);
+
+/* Factorization only (leaf LUs, the per-level Y_t blocks, core LUs).
+ * K (N x leaf, ld ldk) is overwritten by the leaf LU factors. */
+int hodlr_factor(hodlr_plan plan, double *K, int64_t ldk,
+                 const double *U, int64_t ldu, const double *V, int64_t ldv,
+                 hipStream_t stream);
+
+/* Solve with a stored factorization: B (N x nrhs, ld ldb) <- A^-1 B.
+ * The factor-reuse form of HMatrix::solve (src/hmatrix.cc:49-96), with
+ * nrhs columns in place of the hard-coded single RHS (src/tree.cc:9). */
+int hodlr_solve(hodlr_plan plan, double *B, int64_t ldb, int nrhs, hipStream_t stream);
+
+/* Fused factor + solve in one pass: the reference's HMatrix::solve /
+ * spmd driver level loop (spmd_driver/solver_irregular_part.cc:104-140),
+ * which factors and solves together.  Leaves the factorization in the plan
+ * for later hodlr_solve calls. */
+int hodlr_factor_solve(hodlr_plan plan, double *K, int64_t ldk,
+                       const double *U, int64_t ldu, const double *V, int64_t ldv,
+                       double *B, int64_t ldb, int nrhs, hipStream_t stream);
+
+/* Synchronizes `stream` and reports the numerical status of the last factor:
+ * HODLR_OK, HODLR_SINGULAR_LEAF (*where = leaf index) or HODLR_SINGULAR_CORE
+ * (*where = level * 2^20 + node).  Replaces the reference's
+ * `assert(INFO == 0)` (src/tasks/leaf_solve.cc:90,141). */
+int hodlr_plan_status(hodlr_plan plan, hipStream_t stream, int *where);
+
+/* FP64 HODLR matrix-vector product with the ORIGINAL K (not its LU):
+ * Y = alpha * A X + beta * Y, per leaf y_i = K_i x_i + U_i (V^T x - V_i^T x_i).
+ * No reference counterpart (the reference has no residual check:
+ * tests/test.cc:651-658 is `#if 0`); used for residuals and refinement. */
+int hodlr_matvec(int64_t n, int leaf, int rank, int nrhs,
+                 const double *K, int64_t ldk, const double *U, int64_t ldu,
+                 const double *V, int64_t ldv, const double *X, int64_t ldx,
+                 double alpha, double beta, double *Y, int64_t ldy,
+                 hipStream_t stream);
+
+/* ------------------------------------------- reference-layout operators */
+/* GPU form of hsolve (src/tasks/leaf_solve.cc:8-10, :73-154), same argument
+ * meaning and in-place semantics: X is nrow x (nrhs + rank*log2(nPart))
+ * with LD rows, columns [0,nrhs) the right-hand sides d and the following
+ * rank-wide blocks the per-level u copies; K (nrow x nrow/nPart, LD) holds
+ * the leaf blocks and is overwritten by their LU; V is nrow x rank.
+ * Synchronizes `stream` and returns INFO (0 or > 0 singular). */
+int hodlr_hsolve_ref(int nrow, int nrhs, int rank, int nPart, int LD,
+                     double *K, double *X, const double *V, hipStream_t stream);
+
+/* LMatrix::gemmRed (src/lmatrix.cc:713-761; gemm_reduce.cc:58-76):
+ * C[slot] = alpha * sum_{p in slot} A_p^T B_p over nchunk equal row chunks of
+ * A (rows x ka) and B (rows x kb), slot = p / (nchunk / nslot); C is
+ * (nslot*ka) x kb (ld ldc), slots stacked by rows (beta = 0 as asserted at
+ * lmatrix.cc:718). */
+int hodlr_gemm_red(int64_t rows, int nchunk, int nslot, double alpha,
+                   const double *A, int64_t lda, int ka,
+                   const double *B, int64_t ldb, int kb,
+                   double *C, int64_t ldc, hipStream_t stream);
+
+/* LMatrix::gemmBro (src/lmatrix.cc:814-865; gemm_broadcast.cc:58-82):
+ * D_p += alpha * A_p * E[slot(p)] for row chunks p of A (rows x k) and
+ * D (rows x nd); E stacked by slots of k rows (ld lde).  beta = 1 (asserted
+ * at lmatrix.cc:820). */
+int hodlr_gemm_bro(int64_t rows, int nchunk, int nslot, double alpha,
+                   const double *A, int64_t lda, int k,
+                   const double *E, int64_t lde, double *D, int64_t ldd, int nd,
+                   hipStream_t stream);
+
+/* LMatrix::node_solve (src/lmatrix.cc:534-581; node_solve.cc:60-81): for
+ * each of nnode node slots of 2r rows, S = [I, VTu(r:2r,:); VTu(0:r,:), I],
+ * the RHS halves of VTd swapped, VTd <- S^-1 VTd.  Synchronizes `stream` and
+ * returns INFO. */
+int hodlr_node_solve(int nnode, int r, const double *VTu, int64_t ldu,
+                     double *VTd, int64_t ldd, int nrhs, hipStream_t stream);
+
+/* ------------------------------------------------------ batched LAPACK */
+/* Batched LU with partial pivoting (dgetrf semantics, dgetrf.f) of `batch`
+ * n x n blocks A_b = A + b*bstride (ld lda) in place.  ipiv (batch x n,
+ * 1-based, LAPACK IPIV) and info (batch) are device arrays.  Replaces the
+ * dgetrf_ calls behind dgesv_ (include/lapack_blas.hpp:45-57). */
+int hodlr_dgetrf_batched(int n, double *A, int64_t lda, int64_t bstride, int batch,
+                         int *ipiv, int *info, hipStream_t stream);
+/* Batched dgesv_ (include/lapack_blas.hpp:45-50): A_b overwritten by LU,
+ * B_b = B + b*bbstride (n x nrhs, ld ldb) by the solution.  Synchronizes and
+ * returns the first nonzero INFO (+ batch index * 2^20). */
+int hodlr_dgesv_batched(int n, int nrhs, double *A, int64_t lda, int64_t bstride,
+                        int batch, int *ipiv, double *B, int64_t ldb,
+                        int64_t bbstride, hipStream_t stream);
+
+/* ------------------------------------------------ workload generators */
+/* Device Philox4x64-10 generator: element e of (seed, stream, system) is
+ * word e%4 of philox(ctr={e/4, stream, system, 0}, key={seed, 0}).
+ * kind 0: uniform (w>>11)*2^-53 * scale; kind 1: Box-Muller normal * scale.
+ * Writes an rows x cols column-major block (ld) with element index
+ * e = row + col*rows. */
+int hodlr_generate(uint64_t seed, uint32_t stream_id, uint32_t system, int kind,
+                   int64_t rows, int cols, double scale, double *out, int64_t ld,
+                   hipStream_t stream);
+/* Adds `shift` to the diagonal of every leaf block of K (N x leaf). */
+int hodlr_shift_leaf_diagonal(int64_t n, int leaf, double shift, double *K,
+                              int64_t ldk, hipStream_t stream);
+
+/* ---------------------------------------------------- kernel profiling */
+/* Kernel classes timed by the profiler. */
+enum hodlr_prof_kind {
+  HODLR_PROF_GETRF_LEAF = 0, /* batched LU of the leaf blocks              */
+  HODLR_PROF_GETRS_LEAF = 1, /* leaf solves K^-1 [U | B]                   */
+  HODLR_PROF_TRTRI = 2,      /* inverses of 32x32 diagonal blocks          */
+  HODLR_PROF_REDUCE = 3,     /* level reduce V^T [Y | B] (gemmRed)         */
+  HODLR_PROF_ASSEMBLE = 4,   /* core assembly S = [I, G1; G0, I]           */
+  HODLR_PROF_GETRF_CORE = 5, /* batched LU of the 2r x 2r cores            */
+  HODLR_PROF_GETRS_CORE = 6, /* core solves                                */
+  HODLR_PROF_UPDATE = 7,     /* level update [Y | B] -= Y eta (gemmBro)    */
+  HODLR_PROF_NKINDS = 8
+};
+/* No reference counterpart (the reference profiles with Legion Prof,
+ * tests/Makefile:119-212).  While enabled, the plan entry points record HIP
+ * events around every launch on its stream, with the launch's algorithmic
+ * FLOPs and minimum HBM bytes. */
+int hodlr_profile_enable(int on);
+int hodlr_profile_reset(void);
+/* Synchronizes on the recorded events; totals over all launches of `kind`. */
+int hodlr_profile_read(int kind, int64_t *count, double *total_ms, double *flops,
+                       double *bytes);
+
+/* Library version string. */
+const char *hodlr_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HODLR_H */

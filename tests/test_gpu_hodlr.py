@@ -1,0 +1,296 @@
+"""Parity of the HIP path with the CPU oracle (gpu marker).
+
+Every test calls the device library through the C ABI (via the ctypes layer
+in fastsolver2_amd) and compares with the oracle -- the restated reference
+solve (oracle/hodlr_oracle.c) -- on the same inputs.  FP64 tolerance: the
+solution matches the oracle to 1e-10 relative (max-norm), the bar the
+reference comparison is held to.
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def to_dev(F, a):
+    a = np.asfortranarray(a, dtype=np.float64)
+    if a.ndim == 1:
+        a = a.reshape(-1, 1, order="F")
+    t = F.fortran_empty(a.shape[0], a.shape[1])
+    t.copy_(torch.from_numpy(a))
+    return t
+
+
+def to_host(t):
+    torch.cuda.synchronize()
+    return np.asfortranarray(t.cpu().numpy())
+
+
+def relerr(x, ref):
+    return float(np.abs(x - ref).max() / max(np.abs(ref).max(), 1e-300))
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _blas(orc):
+    orc.use_openblas()
+    yield
+    orc.use_netlib()
+
+
+# ------------------------------------------------------------------ generators
+def test_device_generator_matches_oracle(gpu, orc):
+    say("case: generator uniform/normal 1000x7")
+    F = gpu
+    u = to_host(F.generate(9, 3, 1, 0, 1000, 7, 1.5))
+    n = to_host(F.generate(9, 4, 1, 1, 1000, 7, 0.25))
+    uo = orc.gen_uniform(9, 3, 1, 7000, 1.5).reshape((1000, 7), order="F")
+    no = orc.gen_normal(9, 4, 1, 7000, 0.25).reshape((1000, 7), order="F")
+    assert np.array_equal(u, uo)                       # integer-exact stream
+    assert np.abs(n - no).max() <= 4e-16 * np.abs(no).max() * 8
+
+
+def test_workload_generator_definition(gpu, orc):
+    say("case: workload_gaussian N=512 m=64 r=8 s=3")
+    F = gpu
+    K, U, V, B = F.workload_gaussian(512, 64, 8, 3, seed=4)
+    Ko, Uo, Vo, Bo = orc.workload_gaussian(512, 64, 8, 3, seed=4)
+    for d, o in ((K, Ko), (U, Uo), (V, Vo), (B, Bo)):
+        h = to_host(d)
+        assert h.shape == o.shape
+        assert np.abs(h - o).max() <= 1e-14 * max(1.0, np.abs(o).max())
+
+
+# ------------------------------------------------------------------ batched LAPACK
+@pytest.mark.parametrize("n", [1, 3, 16, 31, 32, 33, 64, 100, 128, 257, 512, 600, 1000])
+def test_dgesv_batched_pivoting(gpu, orc, n):
+    """Batched LU with partial pivoting on unshifted Gaussian blocks (pivots
+    active) vs numpy, and pivot vector vs the restated dgetrf."""
+    say(f"case: dgesv_batched n={n}")
+    F = gpu
+    batch, nrhs = 3, 5
+    rng = np.random.default_rng(n)
+    As = [rng.standard_normal((n, n)) for _ in range(batch)]
+    Bs = [rng.standard_normal((n, nrhs)) for _ in range(batch)]
+    A = to_dev(F, np.hstack(As))
+    B = to_dev(F, np.hstack(Bs))
+    info, ipiv = F.dgesv_batched(A, B, n, batch)
+    assert info == 0
+    X = to_host(B)
+    LU = to_host(A)
+    ipiv = ipiv.cpu().numpy().reshape(batch, n)
+    for b in range(batch):
+        xb = X[:, b * nrhs:(b + 1) * nrhs]
+        ref = np.linalg.solve(As[b], Bs[b])
+        cond = np.linalg.cond(As[b])
+        assert relerr(xb, ref) <= 1e-15 * cond * n + 1e-13, (b, cond)
+        if n <= 128:
+            LUo = np.asfortranarray(As[b].copy())
+            _, po = orc.dgetrf(LUo)
+            assert np.array_equal(ipiv[b], po)
+            assert np.abs(LU[:, b * n:(b + 1) * n] - LUo).max() <= 1e-11 * np.abs(LUo).max()
+
+
+def test_dgesv_batched_singular(gpu):
+    say("case: dgesv_batched singular")
+    F = gpu
+    n = 40
+    rng = np.random.default_rng(1)
+    A0 = rng.standard_normal((n, n))
+    A1 = rng.standard_normal((n, n))
+    A1[:, 9] = 0.0
+    A = to_dev(F, np.hstack([A0, A1]))
+    B = to_dev(F, rng.standard_normal((n, 2)))
+    info, _ = F.dgesv_batched(A, B, n, 2)
+    assert info == 10 + (1 << 20)          # INFO 10 in batch item 1
+
+
+# ------------------------------------------------------------------ full solve
+SHAPES = [
+    # N, leaf, rank, nrhs
+    (256, 256, 8, 2),        # L = 0: a single dgesv
+    (512, 64, 8, 3),
+    (2048, 128, 16, 2),
+    (800, 100, 7, 5),        # leaf not a multiple of 16 / 32
+    (1600, 400, 20, 1),      # the drivers' default leaf size 400
+    (16384, 256, 30, 1),     # config 1 of the benchmark set
+    (8192, 512, 64, 8),
+    (4096, 64, 50, 3),       # 2r = 100
+]
+
+
+@pytest.mark.parametrize("N,m,r,s", SHAPES)
+def test_factor_solve_matches_oracle(gpu, orc, N, m, r, s):
+    say(f"case: factor_solve N={N} m={m} r={r} s={s}")
+    F = gpu
+    L = int(np.log2(N // m))
+    K, U, V, B = F.workload_gaussian(N, m, r, s, seed=N + r)
+    Kh, Uh, Vh, Bh = to_host(K), to_host(U), to_host(V), to_host(B)
+    xo = orc.hodlr_solve(Kh, Uh, Vh, Bh, m, task_level=min(L, 3), nthreads=8)
+    plan = F.HodlrPlan(N, m, L, r, nrhs_max=s)
+    plan.factor_solve(K, U, V, B)
+    assert plan.status()[0] == 0
+    x = to_host(B)
+    assert relerr(x, xo) <= RTOL
+    # factor reuse: a second right-hand side through hodlr_solve
+    B2h = np.asfortranarray(np.random.default_rng(1).standard_normal((N, s)))
+    B2 = to_dev(F, B2h)
+    plan.solve(B2)
+    xo2 = orc.hodlr_solve(Kh, Uh, Vh, B2h, m, task_level=min(L, 3), nthreads=8)
+    assert relerr(to_host(B2), xo2) <= RTOL
+    plan.close()
+
+
+def test_factor_then_solve_many_rhs(gpu, orc):
+    """hodlr_factor + hodlr_solve with more columns than nrhs_max (column
+    batches), against the oracle."""
+    say("case: factor + solve 40 rhs N=4096 m=128 r=16")
+    F = gpu
+    N, m, r = 4096, 128, 16
+    K, U, V, _ = F.workload_gaussian(N, m, r, 1, seed=3)
+    Kh, Uh, Vh = to_host(K), to_host(U), to_host(V)
+    plan = F.HodlrPlan(N, m, 5, r, nrhs_max=4)
+    plan.factor(K, U, V)
+    Bh = np.asfortranarray(np.random.default_rng(2).random((N, 40)))
+    B = to_dev(F, Bh)
+    plan.solve(B)
+    xo = orc.hodlr_solve(Kh, Uh, Vh, Bh, m, task_level=3, nthreads=8)
+    assert relerr(to_host(B), xo) <= RTOL
+    plan.close()
+
+
+def test_reference_generator_solve(gpu, orc):
+    """The reference's own generator (drand48 seeds, K = U V^T + diag(D+1e3),
+    dense_block.cc:54-75) through the device solve."""
+    say("case: reference generator N=4096 m=256 r=16")
+    F = gpu
+    npart, m, r = 16, 256, 16
+    N = npart * m
+    U = orc.ref_matrix(101, npart, m, r)
+    V = orc.ref_matrix(102, npart, m, r)
+    K = orc.ref_dense_blocks(npart, m, r, 1000, orc.ref_seeds(101, npart),
+                             orc.ref_seeds(102, npart), orc.ref_seeds(103, npart))
+    b = np.asfortranarray(orc.ref_matrix(104, npart, m, 1))
+    xo = orc.hodlr_solve(K, U, V, b, m)
+    Kd, Ud, Vd, Bd = to_dev(F, K), to_dev(F, U), to_dev(F, V), to_dev(F, b)
+    plan = F.HodlrPlan(N, m, 4, r, nrhs_max=1)
+    plan.factor_solve(Kd, Ud, Vd, Bd)
+    assert relerr(to_host(Bd), xo) <= RTOL
+    plan.close()
+
+
+def test_singular_leaf_status(gpu):
+    say("case: singular leaf N=1024 m=128")
+    F = gpu
+    K, U, V, B = F.workload_gaussian(1024, 128, 8, 1, seed=5)
+    K[3 * 128:4 * 128, 17] = 0.0
+    plan = F.HodlrPlan(1024, 128, 3, 8, nrhs_max=1)
+    plan.factor_solve(K, U, V, B)
+    st, where = plan.status()
+    assert st == 1 and where == 3
+    plan.close()
+
+
+def test_bad_shapes_rejected(gpu):
+    F = gpu
+    with pytest.raises(F.HodlrError):
+        F.HodlrPlan(1000, 64, 4, 8)        # N != leaf * 2^levels
+    with pytest.raises(F.HodlrError):
+        F.HodlrPlan(1024, 8, 7, 8)         # leaf <= rank (matrix.cc:225)
+
+
+# ------------------------------------------------------------------ reference layout
+@pytest.mark.parametrize("N,m,r,s", [(512, 64, 4, 2), (2048, 256, 16, 1), (1200, 150, 10, 3)])
+def test_hsolve_ref_full_layout(gpu, orc, N, m, r, s):
+    """Device hsolve on the reference's column layout X = [b | u_0..u_{L-1}]:
+    every column (solution and the per-level u blocks) vs the oracle hsolve."""
+    say(f"case: hsolve_ref N={N} m={m} r={r} s={s}")
+    F = gpu
+    L = int(np.log2(N // m))
+    K, U, V, B = orc.workload_gaussian(N, m, r, s, seed=7)
+    X = orc.build_x(B, U, L)
+    Ko, Xo = K.copy(order="F"), X.copy(order="F")
+    assert orc.hsolve(N, s, r, N // m, Ko, Xo, V) == 0
+    Kd, Xd, Vd = to_dev(F, K), to_dev(F, X), to_dev(F, V)
+    assert F.hsolve(N, s, r, N // m, Kd, Xd, Vd) == 0
+    Xg = to_host(Xd)
+    assert relerr(Xg[:, :s], Xo[:, :s]) <= RTOL
+    for t in range(L):
+        blk = slice(s + r * t, s + r * (t + 1))
+        assert relerr(Xg[:, blk], Xo[:, blk]) <= RTOL, t
+
+
+# ------------------------------------------------------------------ LMatrix operators
+def test_gemm_red_bro_node_solve(gpu, orc):
+    say("case: gemm_red / gemm_bro / node_solve")
+    F = gpu
+    rng = np.random.default_rng(3)
+    rows, ka, kb = 4096, 12, 7
+    A = np.asfortranarray(rng.standard_normal((rows, ka)))
+    Bm = np.asfortranarray(rng.standard_normal((rows, kb)))
+    for nchunk, nslot in [(8, 2), (16, 16), (4, 1)]:
+        Cg = to_host(F.gemm_red(to_dev(F, A), to_dev(F, Bm), nchunk, nslot, alpha=0.5))
+        Co = orc.gemm_red(A, Bm, nchunk, nslot, alpha=0.5)
+        assert relerr(Cg, Co) <= 1e-13
+    k, nd = 6, 9
+    u = np.asfortranarray(rng.standard_normal((rows, k)))
+    for nchunk, nslot in [(8, 2), (4, 4)]:
+        E = np.asfortranarray(rng.standard_normal((nslot * k, nd)))
+        D = np.asfortranarray(rng.standard_normal((rows, nd)))
+        Dg = to_dev(F, D)
+        F.gemm_bro(to_dev(F, u), to_dev(F, E), Dg, nchunk, nslot, alpha=-1.0)
+        Do = orc.gemm_bro(u, E, D.copy(order="F"), nchunk, nslot, alpha=-1.0)
+        assert relerr(to_host(Dg), Do) <= 1e-13
+    r, nnode, nrhs = 5, 4, 3
+    VTu = np.asfortranarray(rng.random((nnode * 2 * r, r)))
+    VTd = np.asfortranarray(rng.random((nnode * 2 * r, nrhs)))
+    VTd_o = VTd.copy(order="F")
+    assert orc.node_solve(VTu, VTd_o, r) == 0
+    VTd_g = to_dev(F, VTd)
+    assert F.node_solve(to_dev(F, VTu), VTd_g, r) == 0
+    assert relerr(to_host(VTd_g), VTd_o) <= 1e-12
+
+
+def test_matvec_matches_oracle(gpu, orc):
+    say("case: matvec N=4096 m=128 r=12 s=5")
+    F = gpu
+    N, m, r, s = 4096, 128, 12, 5
+    K, U, V, B = F.workload_gaussian(N, m, r, s, seed=8)
+    y = to_host(F.matvec(K, U, V, B, m))
+    yo = orc.hodlr_matvec(to_host(K), to_host(U), to_host(V), to_host(B), m)
+    assert relerr(y, yo) <= 1e-13
+
+
+# ------------------------------------------------------------------ full size
+def test_config2_size_residual_and_oracle_sample(gpu, orc):
+    """N = 2^18, leaf 512, rank 64, 8 RHS (the benchmark workload): residual
+    through the device HODLR matvec, and the solution on the first leaves
+    against the oracle run on the same arrays."""
+    say("case: full size N=262144 m=512 r=64 s=8")
+    F = gpu
+    N, m, r, s, L = 262144, 512, 64, 8, 9
+    K, U, V, B = F.workload_gaussian(N, m, r, s, seed=0)
+    K0, B0 = K.clone(), B.clone()
+    plan = F.HodlrPlan(N, m, L, r, nrhs_max=s)
+    plan.factor_solve(K, U, V, B)
+    assert plan.status()[0] == 0
+    R = F.matvec(K0, U, V, B, m, alpha=-1.0)
+    R.add_(B0)
+    torch.cuda.synchronize()
+    rel = (R.abs().max() / (B0.abs().max())).item()
+    say(f"  relative residual {rel:.3e}")
+    assert rel <= 1e-12
+    # oracle on the same arrays (reference algorithm, OpenBLAS, 16 threads)
+    xo = orc.hodlr_solve(to_host(K0), to_host(U), to_host(V), to_host(B0), m, task_level=4,
+                         nthreads=16)
+    assert relerr(to_host(B), xo) <= RTOL
+    plan.close()
