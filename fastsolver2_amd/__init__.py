@@ -78,6 +78,8 @@ def lib():
         "hodlr_generate": ([C.c_uint64, C.c_uint32, C.c_uint32, ci, i64, ci, dbl, vp, i64, vp],
                            ci),
         "hodlr_shift_leaf_diagonal": ([i64, ci, dbl, vp, i64, vp], ci),
+        "hodlr_selftest_mfma": ([vp, vp], ci),
+        "hodlr_dense_blocks": ([i64, ci, ci, vp, i64, vp, i64, vp, dbl, vp, i64, vp], ci),
         "hodlr_profile_enable": ([ci], ci),
         "hodlr_profile_reset": ([], ci),
         "hodlr_profile_read": ([ci, C.POINTER(i64), C.POINTER(dbl), C.POINTER(dbl),

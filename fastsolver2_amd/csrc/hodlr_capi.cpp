@@ -763,6 +763,21 @@ int hodlr_generate(uint64_t seed, uint32_t stream_id, uint32_t system, int kind,
   return HODLR_OK;
 }
 
+int hodlr_selftest_mfma(double* out, hipStream_t st) {
+  if (!out) return HODLR_ERR_ARG;
+  HIP_TRY(hodlr::launch_selftest_mfma(out, st));
+  return HODLR_OK;
+}
+
+int hodlr_dense_blocks(int64_t n, int leaf, int rank, const double* U, int64_t ldu,
+                       const double* V, int64_t ldv, const double* D, double offset, double* K,
+                       int64_t ldk, hipStream_t st) {
+  if (n <= 0 || leaf <= 0 || rank < 0 || n % leaf || !K || !D || ldk < n) return HODLR_ERR_ARG;
+  if (rank > 0 && (!U || !V || ldu < n || ldv < n)) return HODLR_ERR_ARG;
+  HIP_TRY(hodlr::launch_dense_blocks(n, leaf, rank, U, ldu, V, ldv, D, offset, K, ldk, st));
+  return HODLR_OK;
+}
+
 int hodlr_shift_leaf_diagonal(int64_t n, int leaf, double shift, double* K, int64_t ldk,
                               hipStream_t st) {
   if (n <= 0 || leaf <= 0 || n % leaf || !K || ldk < n) return HODLR_ERR_ARG;

@@ -98,6 +98,12 @@ hipError_t launch_generate(uint64_t seed, uint32_t stream_id, uint32_t system, i
                            hipStream_t st);
 hipError_t launch_shift_diag(int64_t n, int leaf, double shift, double* K, int64_t ldk,
                              hipStream_t st);
+hipError_t launch_dense_blocks(int64_t n, int leaf, int rank, const double* U, int64_t ldu,
+                               const double* V, int64_t ldv, const double* D, double offset,
+                               double* K, int64_t ldk, hipStream_t st);
+hipError_t launch_fill_scale(int64_t rows, int cols, double* A, int64_t ld, int mode, double v,
+                             hipStream_t st);
+hipError_t launch_selftest_mfma(double* out, hipStream_t st);
 hipError_t launch_copy_cols(int64_t rows, int cols, const double* src, int64_t lds,
                             double* dst, int64_t ldd, hipStream_t st);
 

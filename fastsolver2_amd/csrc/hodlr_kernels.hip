@@ -5,7 +5,10 @@
 // MFMA 16x16x4 f64 operand layout (lane l of a 64-lane wave):
 //   A (16x4):  A[l % 16][l / 16]          one double per lane
 //   B (4x16):  B[l / 16][l % 16]          one double per lane
-//   C/D:       C[4*(l/16) + v][l % 16]    v = 0..3, a 4-double vector
+//   C/D:       C[(l/16) + 4*v][l % 16]    v = 0..3, a 4-double vector
+// (the f64 16x16 variant interleaves rows across the four lane groups;
+// crow() below is the one place that encodes it, and
+// hodlr_selftest_mfma checks it on the device).
 //
 // Reference algorithm: Charles-Chao-Chen/fastSolver2 hsolve
 // (src/tasks/leaf_solve.cc:73-154) and the level loop gemmRed -> node_solve
@@ -26,6 +29,9 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
+// row of accumulator element v held by `lane`
+__device__ __forceinline__ int crow(int lane, int v) { return (lane >> 4) + 4 * v; }
+
 __device__ __forceinline__ d4 d4zero() {
   d4 z = {0.0, 0.0, 0.0, 0.0};
   return z;
@@ -200,7 +206,7 @@ k_getrf(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
         }
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const int r0 = 4 * (lane >> 4) + v;
+          const int r0 = crow(lane, v);
           if (col < m2 && r0 < nb) A[(int64_t)(j0 + r0) + (int64_t)(o + col) * lda] = u0[v];
           if (NB > 16 && col < m2 && 16 + r0 < nb)
             A[(int64_t)(j0 + 16 + r0) + (int64_t)(o + col) * lda] = u1[v];
@@ -223,7 +229,7 @@ k_getrf(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
             const int col = c0 + bb * 16 + (lane & 15);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-              const int row = r0 + a * 16 + 4 * (lane >> 4) + v;
+              const int row = r0 + a * 16 + crow(lane, v);
               acc[a][bb][v] = (row < m2 && col < m2)
                                   ? A[(int64_t)(o + row) + (int64_t)(o + col) * lda]
                                   : 0.0;
@@ -255,7 +261,7 @@ k_getrf(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
             const int col = c0 + bb * 16 + (lane & 15);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-              const int row = r0 + a * 16 + 4 * (lane >> 4) + v;
+              const int row = r0 + a * 16 + crow(lane, v);
               if (row < m2 && col < m2)
                 A[(int64_t)(o + row) + (int64_t)(o + col) * lda] = acc[a][bb][v];
             }
@@ -443,7 +449,7 @@ k_getrs(const double* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
       if (has_tile) {
 #pragma unroll
         for (int v = 0; v < 4; ++v)
-          Xl[(ct_a * 16 + lr) * npad + i0 + rt_a * 16 + 4 * lk + v] = a4[v];
+          Xl[(ct_a * 16 + lr) * npad + i0 + rt_a * 16 + crow(lane, v)] = a4[v];
       }
       __syncthreads();
       // (b) rows outside the block: fwd -> rows [i0+32, npad), bwd -> rows [0, i0)
@@ -455,7 +461,7 @@ k_getrs(const double* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
         const int rbase = rlo + rt * 16;
         d4 a4;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) a4[v] = Xl[(ct * 16 + lr) * npad + rbase + 4 * lk + v];
+        for (int v = 0; v < 4; ++v) a4[v] = Xl[(ct * 16 + lr) * npad + rbase + crow(lane, v)];
 #pragma unroll
         for (int kk = 0; kk < kTri / 4; ++kk) {
           const int k = kk * 4 + lk;
@@ -470,7 +476,7 @@ k_getrs(const double* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
           a4 = mfma(-af, bf, a4);
         }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) Xl[(ct * 16 + lr) * npad + rbase + 4 * lk + v] = a4[v];
+        for (int v = 0; v < 4; ++v) Xl[(ct * 16 + lr) * npad + rbase + crow(lane, v)] = a4[v];
       }
       __syncthreads();
     }
@@ -594,7 +600,7 @@ k_reduce(const double* __restrict__ V, int64_t ldv, int r, ColSeg X, int ncols,
     const int col = j0 + ct * 16 + lr;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int a = a0 + wid * 16 + 4 * lk + v;
+      const int a = a0 + wid * 16 + crow(lane, v);
       if (a < r && col < ncols) P[a + (int64_t)col * r] = acc[ct][v];
     }
   }
@@ -695,7 +701,7 @@ k_update(const double* __restrict__ u, int64_t ldu, int r, const double* __restr
     const int col = j0 + ct * 16 + lr;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int64_t row = row0 + 4 * lk + v;
+      const int64_t row = row0 + crow(lane, v);
       acc[ct][v] = (row < cend && col < ncols) ? seg_in(X, col)[row] : 0.0;
     }
   }
@@ -717,7 +723,7 @@ k_update(const double* __restrict__ u, int64_t ldu, int r, const double* __restr
     const int col = j0 + ct * 16 + lr;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int64_t row = row0 + 4 * lk + v;
+      const int64_t row = row0 + crow(lane, v);
       if (row < cend && col < ncols) seg_out(X, col)[row] = acc[ct][v];
     }
   }
@@ -864,7 +870,7 @@ k_mv_apply(int leaf, int r, int nrhs, const double* __restrict__ K, int64_t ldk,
     const int col = j0 + ct * 16 + lr;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int row = rloc + 4 * lk + v;
+      const int row = rloc + crow(lane, v);
       if (row < leaf && col < nrhs) {
         double* y = Y + l0 + row + (int64_t)col * ldy;
         const double prev = (beta == 0.0) ? 0.0 : beta * *y;
@@ -974,6 +980,61 @@ hipError_t launch_shift_diag(int64_t n, int leaf, double shift, double* K, int64
   return hipGetLastError();
 }
 
+// DenseBlockTask (dense_block.cc:54-75): K_i = U_i V_i^T + diag(D_i + offset)
+// for every leaf i (K is n x leaf, leaf i = rows [i*leaf, (i+1)*leaf)).
+// Setup work, one element per thread, the rank-r sum in index order.
+__global__ void k_dense_blocks(int64_t n, int leaf, int rank, const double* __restrict__ U,
+                               int64_t ldu, const double* __restrict__ V, int64_t ldv,
+                               const double* __restrict__ D, double offset,
+                               double* __restrict__ K, int64_t ldk) {
+  const int64_t total = n * (int64_t)leaf;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * 256) {
+    const int64_t row = e % n;
+    const int col = (int)(e / n);
+    const int64_t vrow = (row / leaf) * leaf + col;
+    double acc = 0.0;
+    for (int k = 0; k < rank; ++k) acc += U[row + (int64_t)k * ldu] * V[vrow + (int64_t)k * ldv];
+    if ((int)(row % leaf) == col) acc += D[row] + offset;
+    K[row + (int64_t)col * ldk] = acc;
+  }
+}
+
+hipError_t launch_dense_blocks(int64_t n, int leaf, int rank, const double* U, int64_t ldu,
+                               const double* V, int64_t ldv, const double* D, double offset,
+                               double* K, int64_t ldk, hipStream_t st) {
+  const int64_t total = n * (int64_t)leaf;
+  if (total <= 0) return hipSuccess;
+  int64_t g = (total + 255) / 256;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(k_dense_blocks, dim3((unsigned)g), dim3(256), 0, st, n, leaf, rank, U, ldu,
+                     V, ldv, D, offset, K, ldk);
+  return hipGetLastError();
+}
+
+// LMatrix::clear / LMatrix::scale (clear_matrix.cc, scale_matrix.cc,
+// ptr_matrix.cc:50-60): mode 0 stores v, mode 1 multiplies by v (so a NaN
+// survives a scale by 0, as in the reference).
+__global__ void k_fill_scale(int64_t rows, int cols, double* __restrict__ A, int64_t ld, int mode,
+                             double v) {
+  const int64_t total = rows * (int64_t)cols;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * 256) {
+    double* a = A + (e % rows) + (e / rows) * ld;
+    *a = mode == 0 ? v : *a * v;
+  }
+}
+
+hipError_t launch_fill_scale(int64_t rows, int cols, double* A, int64_t ld, int mode, double v,
+                             hipStream_t st) {
+  const int64_t total = rows * (int64_t)cols;
+  if (total <= 0) return hipSuccess;
+  int64_t g = (total + 255) / 256;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(k_fill_scale, dim3((unsigned)g), dim3(256), 0, st, rows, cols, A, ld, mode, v);
+  return hipGetLastError();
+}
+
 __global__ void k_copy_cols(int64_t rows, int cols, const double* __restrict__ src, int64_t lds,
                             double* __restrict__ dst, int64_t ldd) {
   const int64_t total = rows * (int64_t)cols;
@@ -982,6 +1043,23 @@ __global__ void k_copy_cols(int64_t rows, int cols, const double* __restrict__ s
     const int64_t i = e % rows, j = e / rows;
     dst[i + j * ldd] = src[i + j * lds];
   }
+}
+
+// Lane-layout self-test of v_mfma_f64_16x16x4_f64 (one wave, in bounds):
+// A[i][k] = i + 100 k, B[k][j] = (k == j % 4), so D[i][j] = i + 100 (j % 4);
+// out[4*lane + v] receives accumulator element v of each lane.
+__global__ void k_selftest_mfma(double* __restrict__ out) {
+  const int l = threadIdx.x;
+  const double a = (double)(l & 15) + 100.0 * (double)(l >> 4);
+  const double b = ((l >> 4) == ((l & 15) & 3)) ? 1.0 : 0.0;
+  d4 c = mfma(a, b, d4zero());
+#pragma unroll
+  for (int v = 0; v < 4; ++v) out[4 * l + v] = c[v];
+}
+
+hipError_t launch_selftest_mfma(double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_selftest_mfma, dim3(1), dim3(64), 0, st, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_copy_cols(int64_t rows, int cols, const double* src, int64_t lds, double* dst,

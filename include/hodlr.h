@@ -168,6 +168,12 @@ int hodlr_dgesv_batched(int n, int nrhs, double *A, int64_t lda, int64_t bstride
 int hodlr_generate(uint64_t seed, uint32_t stream_id, uint32_t system, int kind,
                    int64_t rows, int cols, double scale, double *out, int64_t ld,
                    hipStream_t stream);
+/* DenseBlockTask::cpu_task + LMatrix::init_dense_blocks (src/tasks/
+ * dense_block.cc:54-75, src/lmatrix.cc:273-297): K_i = U_i V_i^T +
+ * diag(D_i + offset) for every leaf i of K (N x leaf); D has N entries. */
+int hodlr_dense_blocks(int64_t n, int leaf, int rank, const double *U, int64_t ldu,
+                       const double *V, int64_t ldv, const double *D, double offset,
+                       double *K, int64_t ldk, hipStream_t stream);
 /* Adds `shift` to the diagonal of every leaf block of K (N x leaf). */
 int hodlr_shift_leaf_diagonal(int64_t n, int leaf, double shift, double *K,
                               int64_t ldk, hipStream_t stream);
@@ -194,6 +200,12 @@ int hodlr_profile_reset(void);
 /* Synchronizes on the recorded events; totals over all launches of `kind`. */
 int hodlr_profile_read(int kind, int64_t *count, double *total_ms, double *flops,
                        double *bytes);
+
+/* Device self-test of the FP64 matrix-core lane layout the kernels assume:
+ * writes 256 doubles (64 lanes x 4 accumulator elements) to device `out`;
+ * element 4*l + v must equal row + 100*(col % 4) with row = l/16 + 4v,
+ * col = l%16.  No reference counterpart. */
+int hodlr_selftest_mfma(double *out, hipStream_t stream);
 
 /* Library version string. */
 const char *hodlr_version(void);

@@ -47,6 +47,20 @@ def _blas(orc):
     orc.use_netlib()
 
 
+# ------------------------------------------------------------------ MFMA layout
+def test_mfma_f64_lane_layout(gpu):
+    """The FP64 matrix-core accumulator layout every kernel relies on."""
+    say("case: mfma f64 16x16x4 lane layout")
+    F = gpu
+    out = torch.zeros(256, dtype=torch.float64, device="cuda")
+    assert F.lib().hodlr_selftest_mfma(F._ptr(out), F._stream()) == 0
+    got = out.cpu().numpy()
+    for l in range(64):
+        for v in range(4):
+            row, col = l // 16 + 4 * v, l % 16
+            assert got[4 * l + v] == row + 100 * (col % 4), (l, v, got[4 * l + v])
+
+
 # ------------------------------------------------------------------ generators
 def test_device_generator_matches_oracle(gpu, orc):
     say("case: generator uniform/normal 1000x7")
