@@ -18,6 +18,8 @@ ARCH = "gfx950"
 # Packed-FP32 VALU forms are disabled for every HIP translation unit.
 FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         # fully unroll the register-resident panel loops (static register indexing)
+         "-mllvm", "-pragma-unroll-threshold=1000000",
          "-Wall", "-Wno-unused-result"]
 
 

@@ -22,6 +22,10 @@
 
 #include "hodlr_internal.hpp"
 
+#ifndef GETRF_NB
+#define GETRF_NB 32
+#endif
+
 namespace hodlr {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -275,6 +279,250 @@ k_getrf(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
   if (tid == 0) info0[b] = info;
 }
 
+// =====================================================================
+// Batched LU for n <= 512: one workgroup of 512 threads (8 waves) per
+// block, thread t owning row j0 + t of the current NB-column panel in
+// registers.  Per pivot column: a wave argmax (first row of largest |a|,
+// as idamax) + one cross-wave step, the pivot and displaced rows broadcast
+// through LDS, and the rank-1 update in registers -- two barriers per
+// column.  U12 = L11^-1 A12 is a per-column register substitution (one
+// thread per column), and the trailing update A22 -= L21 U12 runs on MFMA
+// with both operands read from L2.
+// =====================================================================
+template <int NB>
+__global__ void __launch_bounds__(512)
+k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
+            int* __restrict__ ipiv0, int* __restrict__ perm0, int* __restrict__ info0) {
+  __shared__ double prow[NB];
+  __shared__ double orow[NB];
+  __shared__ double redv[8];
+  __shared__ int redi[8];
+  __shared__ int pivs[NB];
+  __shared__ double L11[NB * (NB + 1)];   // L11[i*(NB+1) + k]
+  __shared__ double Xi[NB * (NB + 1)];    // L11^-1, same layout
+  __shared__ double Tt[NB * (NB + 1)];    // doubling scratch
+  __shared__ int perm[512];
+  const int b = blockIdx.x;
+  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  double* A = A0 + (int64_t)b * bstride;
+  int info = 0;  // thread 0
+  if (t < n) perm[t] = t;
+
+  for (int j0 = 0; j0 < n; j0 += NB) {
+    const int nb = min(NB, n - j0);
+    const int rows = n - j0;
+    const bool own = t < rows;
+    double* Arow = A + (int64_t)(j0 + t) + (int64_t)j0 * lda;
+    double p[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) p[c] = (own && c < nb) ? Arow[(int64_t)c * lda] : 0.0;
+
+#pragma unroll
+    for (int jj = 0; jj < NB; ++jj) {
+      if (jj < nb) {
+        double v = (own && t >= jj) ? fabs(p[jj]) : -1.0;
+        int vi = t;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          const double ov = __shfl_xor(v, off, 64);
+          const int oi = __shfl_xor(vi, off, 64);
+          if (ov > v || (ov == v && oi < vi)) { v = ov; vi = oi; }
+        }
+        if (lane == 0) { redv[wid] = v; redi[wid] = vi; }
+        __syncthreads();
+        double best = redv[0];
+        int bi = redi[0];
+#pragma unroll
+        for (int w = 1; w < 8; ++w) {
+          const double ov = redv[w];
+          const int oi = redi[w];
+          if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+        }
+        const int piv = (best >= 0.0) ? bi : jj;
+        if (t == piv) {
+#pragma unroll
+          for (int c = 0; c < NB; ++c) prow[c] = p[c];
+        }
+        if (t == jj) {
+#pragma unroll
+          for (int c = 0; c < NB; ++c) orow[c] = p[c];
+        }
+        if (t == 0) pivs[jj] = piv;
+        __syncthreads();
+        const double pv = prow[jj];
+        if (pv != 0.0) {
+          if (t == jj) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c) p[c] = prow[c];
+          } else if (own && t > jj) {
+            if (t == piv) {
+#pragma unroll
+              for (int c = 0; c < NB; ++c) p[c] = orow[c];
+            }
+            const double l = (fabs(pv) >= DBL_MIN) ? p[jj] * (1.0 / pv) : p[jj] / pv;
+            p[jj] = l;
+#pragma unroll
+            for (int c = jj + 1; c < NB; ++c) p[c] -= l * prow[c];
+          }
+        } else if (t == 0 && info == 0) {
+          info = j0 + jj + 1;  // zero column: p == jj, nothing to eliminate
+        }
+      }
+    }
+
+    // write the panel back; L11 to LDS; pivots; permutation
+    if (own) {
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        if (c < nb) Arow[(int64_t)c * lda] = p[c];
+    }
+    if (t < NB) {
+      // rows past nb are identity rows of the padded L11
+#pragma unroll
+      for (int c = 0; c < NB; ++c) L11[t * (NB + 1) + c] = (t < nb) ? p[c] : 0.0;
+    }
+    if (t < nb) ipiv0[(int64_t)b * n + j0 + t] = j0 + pivs[t] + 1;  // 1-based
+    if (t == 0) {
+      for (int jj = 0; jj < nb; ++jj) {
+        const int pp = pivs[jj];
+        if (pp != jj) {
+          const int tmp = perm[j0 + jj];
+          perm[j0 + jj] = perm[j0 + pp];
+          perm[j0 + pp] = tmp;
+        }
+      }
+    }
+    __syncthreads();
+
+    // interchanges on the other columns (sequential in jj, as dlaswp)
+    const int m2 = n - j0 - nb;
+    if (t < n - nb) {
+      const int col = t < j0 ? t : t + nb;
+      double* Ac = A + (int64_t)col * lda;
+      for (int jj = 0; jj < nb; ++jj) {
+        const int pp = pivs[jj];
+        if (pp != jj) {
+          const double tmp = Ac[j0 + jj];
+          Ac[j0 + jj] = Ac[j0 + pp];
+          Ac[j0 + pp] = tmp;
+        }
+      }
+    }
+    // Xi = L11^-1 by doubling: after the step of size h the diagonal blocks
+    // of size 2h of Xi invert those of L11; the off-diagonal h x h block of
+    // a 2h block is -B^-1 (C A^-1), C the matching block of L11.
+    if (m2 > 0) {
+      for (int e = t; e < NB * NB; e += 512) {
+        const int i = e / NB, j = e % NB;
+        Xi[i * (NB + 1) + j] = (i == j) ? 1.0 : 0.0;
+      }
+      __syncthreads();
+      for (int h = 1; h < NB; h <<= 1) {
+        // Tt[i][j] = sum_k C[i][k] A^-1[k][j], i in lower half, j in upper half
+        for (int e = t; e < NB * h / 2; e += 512) {
+          const int q = e / (h * h), r = e % (h * h);
+          const int s0 = q * 2 * h, i = s0 + h + r / h, j = s0 + r % h;
+          double acc = 0.0;
+          for (int k = j; k < s0 + h; ++k) acc += L11[i * (NB + 1) + k] * Xi[k * (NB + 1) + j];
+          Tt[i * (NB + 1) + j] = acc;
+        }
+        __syncthreads();
+        for (int e = t; e < NB * h / 2; e += 512) {
+          const int q = e / (h * h), r = e % (h * h);
+          const int s0 = q * 2 * h, i = s0 + h + r / h, j = s0 + r % h;
+          double acc = 0.0;
+          for (int k = s0 + h; k <= i; ++k) acc += Xi[i * (NB + 1) + k] * Tt[k * (NB + 1) + j];
+          Xi[i * (NB + 1) + j] = -acc;
+        }
+        __syncthreads();
+      }
+      // U12 = Xi * A12 on MFMA; each wave owns whole 16-column tiles, so it
+      // reads a tile's NB source rows before overwriting them.
+      const int o = j0 + nb;
+      const int nct = (m2 + 15) >> 4;
+      for (int ct = wid; ct < nct; ct += 8) {
+        const int col = ct * 16 + (lane & 15);
+        d4 u[NB / 16];
+#pragma unroll
+        for (int q = 0; q < NB / 16; ++q) u[q] = d4zero();
+#pragma unroll
+        for (int kk = 0; kk < NB / 4; ++kk) {
+          const int k = kk * 4 + (lane >> 4);
+          const double bf = (k < nb && col < m2) ? A[(int64_t)(j0 + k) + (int64_t)(o + col) * lda] : 0.0;
+#pragma unroll
+          for (int q = 0; q < NB / 16; ++q)
+            u[q] = mfma(Xi[(q * 16 + (lane & 15)) * (NB + 1) + k], bf, u[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < NB / 16; ++q)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int r = q * 16 + crow(lane, v);
+            if (col < m2 && r < nb) A[(int64_t)(j0 + r) + (int64_t)(o + col) * lda] = u[q][v];
+          }
+      }
+    }
+    __syncthreads();
+
+    // trailing update A22 -= L21 * U12, 32x32 blocks per wave
+    if (m2 > 0) {
+      const int o = j0 + nb;
+      const int t32 = (m2 + 31) >> 5;
+      for (int blk = wid; blk < t32 * t32; blk += 8) {
+        const int r0 = (blk % t32) * 32, c0 = (blk / t32) * 32;
+        d4 acc[2][2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int bb = 0; bb < 2; ++bb) {
+            const int col = c0 + bb * 16 + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int row = r0 + a * 16 + crow(lane, v);
+              acc[a][bb][v] = (row < m2 && col < m2)
+                                  ? A[(int64_t)(o + row) + (int64_t)(o + col) * lda]
+                                  : 0.0;
+            }
+          }
+#pragma unroll
+        for (int kk = 0; kk < NB / 4; ++kk) {
+          const int k = kk * 4 + (lane >> 4);
+          double af[2], bf[2];
+#pragma unroll
+          for (int a = 0; a < 2; ++a) {
+            const int row = r0 + a * 16 + (lane & 15);
+            af[a] = (k < nb && row < m2) ? -A[(int64_t)(o + row) + (int64_t)(j0 + k) * lda] : 0.0;
+          }
+#pragma unroll
+          for (int bb = 0; bb < 2; ++bb) {
+            const int col = c0 + bb * 16 + (lane & 15);
+            bf[bb] = (k < nb && col < m2) ? A[(int64_t)(j0 + k) + (int64_t)(o + col) * lda] : 0.0;
+          }
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int bb = 0; bb < 2; ++bb) acc[a][bb] = mfma(af[a], bf[bb], acc[a][bb]);
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int bb = 0; bb < 2; ++bb) {
+            const int col = c0 + bb * 16 + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int row = r0 + a * 16 + crow(lane, v);
+              if (row < m2 && col < m2)
+                A[(int64_t)(o + row) + (int64_t)(o + col) * lda] = acc[a][bb][v];
+            }
+          }
+      }
+    }
+    __syncthreads();
+  }
+  if (t < n) perm0[(int64_t)b * n + t] = perm[t];
+  if (t == 0) info0[b] = info;
+}
+
 static size_t getrf_lds_bytes(int nb, int n) {
   return (size_t)(nb * n + 2 * nb + nb * (nb + 1) + 4) * sizeof(double) +
          (size_t)(4 + nb + n) * sizeof(int);
@@ -284,6 +532,11 @@ hipError_t launch_getrf(int n, double* A, int64_t lda, int64_t bstride, int batc
                         int* perm, int* info, hipStream_t st) {
   if (batch <= 0 || n <= 0) return hipSuccess;
   if (n > kMaxBlock) return hipErrorInvalidValue;
+  if (n <= 512) {
+    hipLaunchKernelGGL(k_getrf_reg<GETRF_NB>, dim3(batch), dim3(512), 0, st, A, lda, bstride, n,
+                       ipiv, perm, info);
+    return hipGetLastError();
+  }
   const size_t lim = 160 * 1024;
   if (getrf_lds_bytes(32, n) <= lim) {
     const size_t bytes = getrf_lds_bytes(32, n);
@@ -493,6 +746,144 @@ k_getrs(const double* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
   }
 }
 
+// =====================================================================
+// Batched solve with a stored LU for n <= 512: one workgroup of 512 threads
+// per (block, column chunk of NCT*16 columns).  The right-hand sides stay
+// in MFMA accumulators for the whole solve; wave w owns the 16-row tiles
+// w, w+8, w+16, w+24 (interleaved, so the triangular sweeps stay balanced).
+// Per 32-row diagonal block: the owners stage the block in LDS, the block is
+// multiplied by its stored inverse (one tile per wave), and every wave
+// updates its own tiles on the far side of the block with L (or U) read from
+// L2 -- each factor element is read once per chunk, two barriers per block.
+//   TRANS = false: X <- U^-1 L^-1 P^T X;  TRANS = true: X <- P L^-T U^-T X.
+// =====================================================================
+template <int NCT, bool TRANS>
+__global__ void __launch_bounds__(512)
+k_getrs_reg(const double* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
+            const int* __restrict__ perm0, const double* __restrict__ dinv0, ColSeg X,
+            int64_t xbstride, int ncols, int nchunk) {
+  constexpr int W = NCT * 16;
+  __shared__ double xa[kTri * (W + 1)];  // raw block (owners -> all)
+  __shared__ double xs[kTri * (W + 1)];  // solved block
+  const int G = gridDim.x;
+  const int gq = G >> 3, grem = G & 7;
+  const int xcd = blockIdx.x & 7;
+  const int logical = xcd * gq + min(xcd, grem) + (blockIdx.x >> 3);
+  const int b = logical / nchunk, chunk = logical % nchunk;
+  const int c0 = chunk * W;
+  const double* LU = LU0 + (int64_t)b * bstride;
+  const int* perm = perm0 + (int64_t)b * n;
+  const double* dinv = dinv0 + (int64_t)b * dinv_elems(n);
+  const int64_t xoff = (int64_t)b * xbstride;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int nblk = ntri(n);
+  const int nrt = nblk * 2;  // 16-row tiles
+
+  d4 acc[4][NCT];
+  // gather (rows by perm for the forward solve)
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) {
+      const int col = c0 + ct * 16 + lr;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = (wid + 8 * q) * 16 + crow(lane, v);
+        double val = 0.0;
+        if (row < n && col < ncols) val = seg_in(X, col)[xoff + (TRANS ? row : perm[row])];
+        acc[q][ct][v] = val;
+      }
+    }
+
+  for (int sweep = 0; sweep < 2; ++sweep) {
+    const bool fwd = (sweep == 0);
+    const int islot = TRANS ? (fwd ? 1 : 0) : (fwd ? 0 : 1);
+    for (int step = 0; step < nblk; ++step) {
+      const int ib = fwd ? step : nblk - 1 - step;
+      const int i0 = ib * kTri;
+      const double* Di = dinv + (int64_t)ib * 2 * kTri * kTri + islot * kTri * kTri;
+      // (a1) owners of row tiles 2ib, 2ib+1 stage the raw block
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int rt = wid + 8 * q;
+        if ((rt >> 1) == ib) {
+#pragma unroll
+          for (int ct = 0; ct < NCT; ++ct)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+              xa[((rt & 1) * 16 + crow(lane, v)) * (W + 1) + ct * 16 + lr] = acc[q][ct][v];
+        }
+      }
+      __syncthreads();
+      // (a2) xs = D * xa, 2*NCT tiles spread over the waves
+      for (int tt = wid; tt < 2 * NCT; tt += 8) {
+        const int rtl = tt & 1, ct = tt >> 1;
+        d4 a4 = d4zero();
+#pragma unroll
+        for (int kk = 0; kk < kTri / 4; ++kk) {
+          const int k = kk * 4 + lk;
+          const int row = rtl * 16 + lr;
+          const double af = TRANS ? Di[k + row * kTri] : Di[row + k * kTri];
+          a4 = mfma(af, xa[k * (W + 1) + ct * 16 + lr], a4);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) xs[(rtl * 16 + crow(lane, v)) * (W + 1) + ct * 16 + lr] = a4[v];
+      }
+      __syncthreads();
+      // (b) owners take the solved block back; other tiles on the far side
+      // of the block are updated with the off-diagonal factor block.
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int rt = wid + 8 * q;
+        if (rt < nrt) {
+          if ((rt >> 1) == ib) {
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct)
+#pragma unroll
+              for (int v = 0; v < 4; ++v)
+                acc[q][ct][v] = xs[((rt & 1) * 16 + crow(lane, v)) * (W + 1) + ct * 16 + lr];
+          } else if (fwd ? (rt >> 1) > ib : (rt >> 1) < ib) {
+            const int row = rt * 16 + lr;
+#pragma unroll
+            for (int kk = 0; kk < kTri / 4; ++kk) {
+              const int k = kk * 4 + lk;
+              const int kcol = i0 + k;
+              double af = 0.0;
+              if (row < n && kcol < n)
+                af = TRANS ? LU[(int64_t)kcol + (int64_t)row * lda] : LU[(int64_t)row + (int64_t)kcol * lda];
+#pragma unroll
+              for (int ct = 0; ct < NCT; ++ct)
+                acc[q][ct] = mfma(-af, xs[k * (W + 1) + ct * 16 + lr], acc[q][ct]);
+            }
+          }
+        }
+      }
+    }
+  }
+  // scatter
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) {
+      const int col = c0 + ct * 16 + lr;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = (wid + 8 * q) * 16 + crow(lane, v);
+        if (row < n && col < ncols) seg_out(X, col)[xoff + (TRANS ? perm[row] : row)] = acc[q][ct][v];
+      }
+    }
+}
+
+template <int NCT, bool TRANS>
+static void launch_getrs_reg(int n, const double* LU, int64_t lda, int64_t bstride, int batch,
+                             const int* perm, const double* dinv, const ColSeg& X,
+                             int64_t xbstride, int ncols, hipStream_t st) {
+  const int nchunk = (ncols + NCT * 16 - 1) / (NCT * 16);
+  hipLaunchKernelGGL((k_getrs_reg<NCT, TRANS>), dim3(batch * nchunk), dim3(512), 0, st, LU, lda,
+                     bstride, n, perm, dinv, X, xbstride, ncols, nchunk);
+}
+
 template <bool TRANS>
 static hipError_t launch_getrs_impl(int n, const double* LU, int64_t lda, int64_t bstride,
                                     int batch, const int* perm, const double* dinv,
@@ -500,6 +891,15 @@ static hipError_t launch_getrs_impl(int n, const double* LU, int64_t lda, int64_
                                     hipStream_t st) {
   if (batch <= 0 || n <= 0 || ncols <= 0) return hipSuccess;
   if (n > kMaxBlock) return hipErrorInvalidValue;
+  if (n <= 512) {
+    if (ncols <= 16)
+      launch_getrs_reg<1, TRANS>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
+    else if (ncols <= 32)
+      launch_getrs_reg<2, TRANS>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
+    else
+      launch_getrs_reg<5, TRANS>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
+    return hipGetLastError();
+  }
   const int npad = ntri(n) * kTri;
   const size_t lim = 160 * 1024;
   // 32 columns per workgroup when they fit, else 16
