@@ -33,6 +33,33 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
+// One DPP row operation on a 64-bit value (two 32-bit halves).
+template <int CTRL>
+__device__ __forceinline__ unsigned long long dpp_u64(unsigned long long v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)v, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(v >> 32), CTRL, 0xF, 0xF, false);
+  return ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo;
+}
+__device__ __forceinline__ unsigned long long umax64(unsigned long long a, unsigned long long b) {
+  return a > b ? a : b;
+}
+__device__ __forceinline__ unsigned long long readlane_u64(unsigned long long v, int l) {
+  const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)v, l);
+  const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(v >> 32), l);
+  return ((unsigned long long)hi << 32) | lo;
+}
+// Maximum over the 64 lanes: quad butterflies (xor 1, xor 2), half-row and
+// row mirrors leave every lane of a 16-lane row with the row maximum; the
+// four rows are combined through readlane.
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+  v = umax64(v, dpp_u64<0xB1>(v));   // quad_perm [1,0,3,2]
+  v = umax64(v, dpp_u64<0x4E>(v));   // quad_perm [2,3,0,1]
+  v = umax64(v, dpp_u64<0x141>(v));  // row_half_mirror
+  v = umax64(v, dpp_u64<0x140>(v));  // row_mirror
+  return umax64(umax64(readlane_u64(v, 0), readlane_u64(v, 16)),
+                umax64(readlane_u64(v, 32), readlane_u64(v, 48)));
+}
+
 // row of accumulator element v held by `lane`
 __device__ __forceinline__ int crow(int lane, int v) { return (lane >> 4) + 4 * v; }
 
@@ -289,19 +316,22 @@ k_getrf(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
 // thread per column), and the trailing update A22 -= L21 U12 runs on MFMA
 // with both operands read from L2.
 // =====================================================================
-template <int NB>
-__global__ void __launch_bounds__(512)
+template <int NB, int NT>
+__global__ void __launch_bounds__(NT)
 k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
             int* __restrict__ ipiv0, int* __restrict__ perm0, int* __restrict__ info0) {
-  __shared__ double prow[NB];
-  __shared__ double orow[NB];
-  __shared__ double redv[8];
-  __shared__ int redi[8];
+  // per-column candidates, double-buffered by column parity: each wave
+  // publishes its best row (key, index and the whole panel row) so the
+  // winner's row is readable right after the single barrier of the column
+  constexpr int NW = NT / 64;
+  __shared__ unsigned long long ckey[2][NW];
+  __shared__ int cidx[2][NW];
+  __shared__ double crowv[2][NW][NB];
   __shared__ int pivs[NB];
   __shared__ double L11[NB * (NB + 1)];   // L11[i*(NB+1) + k]
   __shared__ double Xi[NB * (NB + 1)];    // L11^-1, same layout
   __shared__ double Tt[NB * (NB + 1)];    // doubling scratch
-  __shared__ int perm[512];
+  __shared__ int perm[NT];
   const int b = blockIdx.x;
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
   double* A = A0 + (int64_t)b * bstride;
@@ -312,74 +342,112 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
     const int nb = min(NB, n - j0);
     const int rows = n - j0;
     const bool own = t < rows;
-    double* Arow = A + (int64_t)(j0 + t) + (int64_t)j0 * lda;
     double p[NB];
+    if (own) {
+      const double* Arow = A + (int64_t)(j0 + t) + (int64_t)j0 * lda;
 #pragma unroll
-    for (int c = 0; c < NB; ++c) p[c] = (own && c < nb) ? Arow[(int64_t)c * lda] : 0.0;
+      for (int c = 0; c < NB; ++c) p[c] = (c < nb) ? Arow[(int64_t)c * lda] : 0.0;
+    } else {
+#pragma unroll
+      for (int c = 0; c < NB; ++c) p[c] = 0.0;
+    }
+    // Implicit pivoting: thread t keeps its row; `pos` is the row's current
+    // position in LAPACK's interchange order (ties go to the smallest
+    // position, as idamax), `live` until the row becomes a pivot.
+    int pos = t;
+    bool live = own;
 
 #pragma unroll
     for (int jj = 0; jj < NB; ++jj) {
       if (jj < nb) {
-        double v = (own && t >= jj) ? fabs(p[jj]) : -1.0;
-        int vi = t;
+        const int par = jj & 1;
+        // key = bits(|a|) + 1 orders non-negative doubles; a NaN counts as
+        // zero; 0 = not a candidate (retired rows, rows past n)
+        const double aj = p[jj];
+        const unsigned long long mykey =
+            live ? ((aj == aj) ? (unsigned long long)__double_as_longlong(fabs(aj)) + 1ULL : 1ULL)
+                 : 0ULL;
+        const int cpos = mykey ? pos : INT_MAX;
+        // this wave's candidate: largest key, then smallest position
+        const unsigned long long wmax = wave_max_u64(mykey);
+        const unsigned long long bal = __ballot(mykey == wmax);
+        int wl = 0;
+        if (wmax != 0ULL) {
+          if (__popcll(bal) == 1) {
+            wl = __ffsll((long long)bal) - 1;
+          } else {
+            int qq = (mykey == wmax) ? cpos : INT_MAX;
 #pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-          const double ov = __shfl_xor(v, off, 64);
-          const int oi = __shfl_xor(vi, off, 64);
-          if (ov > v || (ov == v && oi < vi)) { v = ov; vi = oi; }
+            for (int off = 32; off > 0; off >>= 1) qq = min(qq, __shfl_xor(qq, off, 64));
+            wl = __ffsll((long long)__ballot(mykey == wmax && cpos == qq)) - 1;
+          }
         }
-        if (lane == 0) { redv[wid] = v; redi[wid] = vi; }
+        if (lane == wl) {
+#pragma unroll
+          for (int c = 0; c < NB; ++c)
+            if (c >= jj) crowv[par][wid][c] = p[c];
+          ckey[par][wid] = wmax;
+          cidx[par][wid] = wmax ? cpos : INT_MAX;
+        }
         __syncthreads();
-        double best = redv[0];
-        int bi = redi[0];
+        // winner across the waves, lane-parallel: lane w < NW holds wave w's
+        // candidate
+        const unsigned long long kw = (lane < NW) ? ckey[par][lane] : 0ULL;
+        const int qw = (lane < NW) ? cidx[par][lane] : INT_MAX;
+        const unsigned long long kbest = wave_max_u64(kw);
+        const unsigned long long tb = __ballot(lane < NW && kw == kbest);
+        int bw;
+        if (__popcll(tb) == 1) {
+          bw = __ffsll((long long)tb) - 1;
+        } else {
+          int qq = (lane < NW && kw == kbest) ? qw : INT_MAX;
 #pragma unroll
-        for (int w = 1; w < 8; ++w) {
-          const double ov = redv[w];
-          const int oi = redi[w];
-          if (ov > best || (ov == best && oi < bi)) { best = ov; bi = oi; }
+          for (int off = 32; off > 0; off >>= 1) qq = min(qq, __shfl_xor(qq, off, 64));
+          bw = __ffsll((long long)__ballot(lane < NW && kw == kbest && qw == qq)) - 1;
         }
-        const int piv = (best >= 0.0) ? bi : jj;
-        if (t == piv) {
-#pragma unroll
-          for (int c = 0; c < NB; ++c) prow[c] = p[c];
+        const int bpos = __builtin_amdgcn_readlane(qw, bw);
+        const bool winner = (lane == wl) && (wmax != 0ULL);
+        const double* prow = crowv[par][bw];
+        if (t == 0) pivs[jj] = bpos;
+        // the global winner retires to position jj; the row at jj takes the
+        // winner's old position
+        if (live && winner && wid == bw) {
+          live = false;
+          pos = jj;
+        } else if (pos == jj) {
+          pos = bpos;
         }
-        if (t == jj) {
-#pragma unroll
-          for (int c = 0; c < NB; ++c) orow[c] = p[c];
-        }
-        if (t == 0) pivs[jj] = piv;
-        __syncthreads();
         const double pv = prow[jj];
         if (pv != 0.0) {
-          if (t == jj) {
-#pragma unroll
-            for (int c = 0; c < NB; ++c) p[c] = prow[c];
-          } else if (own && t > jj) {
-            if (t == piv) {
-#pragma unroll
-              for (int c = 0; c < NB; ++c) p[c] = orow[c];
-            }
+          if (live) {
             const double l = (fabs(pv) >= DBL_MIN) ? p[jj] * (1.0 / pv) : p[jj] / pv;
             p[jj] = l;
 #pragma unroll
-            for (int c = jj + 1; c < NB; ++c) p[c] -= l * prow[c];
+            for (int c = 0; c < NB; ++c)
+              if (c > jj) p[c] -= l * prow[c];
           }
         } else if (t == 0 && info == 0) {
-          info = j0 + jj + 1;  // zero column: p == jj, nothing to eliminate
+          info = j0 + jj + 1;  // the live part of the column is zero
         }
       }
     }
+    __syncthreads();
 
-    // write the panel back; L11 to LDS; pivots; permutation
+    // write the panel rows to their final positions; L11 to LDS
     if (own) {
+      double* Arow = A + (int64_t)(j0 + pos) + (int64_t)j0 * lda;
 #pragma unroll
       for (int c = 0; c < NB; ++c)
         if (c < nb) Arow[(int64_t)c * lda] = p[c];
     }
-    if (t < NB) {
+    if (own && pos < NB) {
+#pragma unroll
+      for (int c = 0; c < NB; ++c) L11[pos * (NB + 1) + c] = p[c];
+    }
+    if (t >= nb && t < NB) {
       // rows past nb are identity rows of the padded L11
 #pragma unroll
-      for (int c = 0; c < NB; ++c) L11[t * (NB + 1) + c] = (t < nb) ? p[c] : 0.0;
+      for (int c = 0; c < NB; ++c) L11[t * (NB + 1) + c] = 0.0;
     }
     if (t < nb) ipiv0[(int64_t)b * n + j0 + t] = j0 + pivs[t] + 1;  // 1-based
     if (t == 0) {
@@ -412,14 +480,14 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
     // of size 2h of Xi invert those of L11; the off-diagonal h x h block of
     // a 2h block is -B^-1 (C A^-1), C the matching block of L11.
     if (m2 > 0) {
-      for (int e = t; e < NB * NB; e += 512) {
+      for (int e = t; e < NB * NB; e += NT) {
         const int i = e / NB, j = e % NB;
         Xi[i * (NB + 1) + j] = (i == j) ? 1.0 : 0.0;
       }
       __syncthreads();
       for (int h = 1; h < NB; h <<= 1) {
         // Tt[i][j] = sum_k C[i][k] A^-1[k][j], i in lower half, j in upper half
-        for (int e = t; e < NB * h / 2; e += 512) {
+        for (int e = t; e < NB * h / 2; e += NT) {
           const int q = e / (h * h), r = e % (h * h);
           const int s0 = q * 2 * h, i = s0 + h + r / h, j = s0 + r % h;
           double acc = 0.0;
@@ -427,7 +495,7 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
           Tt[i * (NB + 1) + j] = acc;
         }
         __syncthreads();
-        for (int e = t; e < NB * h / 2; e += 512) {
+        for (int e = t; e < NB * h / 2; e += NT) {
           const int q = e / (h * h), r = e % (h * h);
           const int s0 = q * 2 * h, i = s0 + h + r / h, j = s0 + r % h;
           double acc = 0.0;
@@ -440,7 +508,7 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
       // reads a tile's NB source rows before overwriting them.
       const int o = j0 + nb;
       const int nct = (m2 + 15) >> 4;
-      for (int ct = wid; ct < nct; ct += 8) {
+      for (int ct = wid; ct < nct; ct += NW) {
         const int col = ct * 16 + (lane & 15);
         d4 u[NB / 16];
 #pragma unroll
@@ -468,7 +536,330 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
     if (m2 > 0) {
       const int o = j0 + nb;
       const int t32 = (m2 + 31) >> 5;
-      for (int blk = wid; blk < t32 * t32; blk += 8) {
+      // software-pipelined over this wave's blocks: the next block's C,
+      // L21 and U12 operands are in flight while the current block's
+      // MFMAs run; partial blocks (only at the matrix edge) take the masked
+      // loads.
+      const int nblk = t32 * t32;
+      auto load_blk = [&](int blk, d4 (&c)[2][2], double (&af)[NB / 4][2], double (&bf)[NB / 4][2]) {
+        const int r0 = (blk % t32) * 32, c0 = (blk / t32) * 32;
+        const bool full = (nb == NB) && (r0 + 32 <= m2) && (c0 + 32 <= m2);
+        if (full) {
+          const double* Cp = A + (int64_t)(o + r0) + (int64_t)(o + c0) * lda;
+          const double* Lp = A + (int64_t)(o + r0) + (int64_t)j0 * lda;
+          const double* Up = A + (int64_t)j0 + (int64_t)(o + c0) * lda;
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+              for (int v = 0; v < 4; ++v)
+                c[a][bb][v] = Cp[(a * 16 + crow(lane, v)) + (int64_t)(bb * 16 + (lane & 15)) * lda];
+#pragma unroll
+          for (int kk = 0; kk < NB / 4; ++kk) {
+            const int k = kk * 4 + (lane >> 4);
+#pragma unroll
+            for (int a = 0; a < 2; ++a) af[kk][a] = Lp[(a * 16 + (lane & 15)) + (int64_t)k * lda];
+#pragma unroll
+            for (int bb = 0; bb < 2; ++bb) bf[kk][bb] = Up[k + (int64_t)(bb * 16 + (lane & 15)) * lda];
+          }
+        } else {
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int bb = 0; bb < 2; ++bb) {
+              const int col = c0 + bb * 16 + (lane & 15);
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                const int row = r0 + a * 16 + crow(lane, v);
+                c[a][bb][v] = (row < m2 && col < m2) ? A[(int64_t)(o + row) + (int64_t)(o + col) * lda] : 0.0;
+              }
+            }
+#pragma unroll
+          for (int kk = 0; kk < NB / 4; ++kk) {
+            const int k = kk * 4 + (lane >> 4);
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+              const int row = r0 + a * 16 + (lane & 15);
+              af[kk][a] = (k < nb && row < m2) ? A[(int64_t)(o + row) + (int64_t)(j0 + k) * lda] : 0.0;
+            }
+#pragma unroll
+            for (int bb = 0; bb < 2; ++bb) {
+              const int col = c0 + bb * 16 + (lane & 15);
+              bf[kk][bb] = (k < nb && col < m2) ? A[(int64_t)(j0 + k) + (int64_t)(o + col) * lda] : 0.0;
+            }
+          }
+        }
+      };
+      auto store_blk = [&](int blk, const d4 (&c)[2][2]) {
+        const int r0 = (blk % t32) * 32, c0 = (blk / t32) * 32;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int bb = 0; bb < 2; ++bb) {
+            const int col = c0 + bb * 16 + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int row = r0 + a * 16 + crow(lane, v);
+              if (row < m2 && col < m2) A[(int64_t)(o + row) + (int64_t)(o + col) * lda] = c[a][bb][v];
+            }
+          }
+      };
+      if (NT > 256) {
+        // 2 waves per SIMD: no register room for a second operand set
+        for (int blk = wid; blk < nblk; blk += NW) {
+          d4 c[2][2];
+          double af[NB / 4][2], bf[NB / 4][2];
+          load_blk(blk, c, af, bf);
+#pragma unroll
+          for (int kk = 0; kk < NB / 4; ++kk)
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+              for (int bb = 0; bb < 2; ++bb) c[a][bb] = mfma(-af[kk][a], bf[kk][bb], c[a][bb]);
+          store_blk(blk, c);
+        }
+      } else if (wid < nblk) {
+        d4 c[2][2];
+        double af[NB / 4][2], bf[NB / 4][2];
+        load_blk(wid, c, af, bf);
+        for (int blk = wid; blk < nblk; blk += NW) {
+          d4 c2[2][2];
+          double af2[NB / 4][2], bf2[NB / 4][2];
+          const bool more = blk + NW < nblk;
+          if (more) load_blk(blk + NW, c2, af2, bf2);
+#pragma unroll
+          for (int kk = 0; kk < NB / 4; ++kk)
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+              for (int bb = 0; bb < 2; ++bb) c[a][bb] = mfma(-af[kk][a], bf[kk][bb], c[a][bb]);
+          store_blk(blk, c);
+          if (more) {
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+              for (int bb = 0; bb < 2; ++bb) c[a][bb] = c2[a][bb];
+#pragma unroll
+            for (int kk = 0; kk < NB / 4; ++kk)
+#pragma unroll
+              for (int a = 0; a < 2; ++a) { af[kk][a] = af2[kk][a]; bf[kk][a] = bf2[kk][a]; }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (t < n) perm0[(int64_t)b * n + t] = perm[t];
+  if (t == 0) info0[b] = info;
+}
+
+// =====================================================================
+// Batched LU for n <= 128 (the 2r x 2r cores at rank <= 64): ONE wave per
+// block, each lane holding panel rows lane and lane + 64 in registers.  The
+// pivot search is a DPP reduction inside the wave and the pivot row is
+// broadcast through LDS, so a pivot column costs no workgroup barrier.
+// Pivot rule, implicit row tracking, U12 and the trailing update are those
+// of k_getrf_reg.
+// =====================================================================
+template <int NB>
+__global__ void __launch_bounds__(64)
+k_getrf_w64(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
+            int* __restrict__ ipiv0, int* __restrict__ perm0, int* __restrict__ info0) {
+  constexpr int R = 2;  // rows per lane
+  __shared__ double prow[NB];
+  __shared__ int pivs[NB];
+  __shared__ double L11[NB * (NB + 1)];
+  __shared__ double Xi[NB * (NB + 1)];
+  __shared__ double Tt[NB * (NB + 1)];
+  __shared__ int perm[128];
+  const int b = blockIdx.x;
+  const int lane = threadIdx.x;
+  double* A = A0 + (int64_t)b * bstride;
+  int info = 0;  // lane 0
+  for (int q = 0; q < R; ++q)
+    if (q * 64 + lane < n) perm[q * 64 + lane] = q * 64 + lane;
+
+  for (int j0 = 0; j0 < n; j0 += NB) {
+    const int nb = min(NB, n - j0);
+    const int rows = n - j0;
+    double p[R][NB];
+    int pos[R];
+    bool live[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int rr = q * 64 + lane;
+      pos[q] = rr;
+      live[q] = rr < rows;
+      const double* Arow = A + (int64_t)(j0 + rr) + (int64_t)j0 * lda;
+#pragma unroll
+      for (int c = 0; c < NB; ++c) p[q][c] = (live[q] && c < nb) ? Arow[(int64_t)c * lda] : 0.0;
+    }
+#pragma unroll
+    for (int jj = 0; jj < NB; ++jj) {
+      if (jj < nb) {
+        // this lane's best row (larger key, then smaller position)
+        unsigned long long lk = 0ULL;
+        int lq = 0, lp = INT_MAX;
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          const double a = p[q][jj];
+          const unsigned long long k =
+              live[q] ? ((a == a) ? (unsigned long long)__double_as_longlong(fabs(a)) + 1ULL : 1ULL) : 0ULL;
+          const int kp = k ? pos[q] : INT_MAX;
+          const bool tk = k > lk || (k == lk && kp < lp);
+          lk = tk ? k : lk;
+          lp = tk ? kp : lp;
+          lq = tk ? q : lq;
+        }
+        const unsigned long long wmax = wave_max_u64(lk);
+        const unsigned long long bal = __ballot(lk == wmax);
+        int wl;
+        if (__popcll(bal) == 1) {
+          wl = __ffsll((long long)bal) - 1;
+        } else {
+          int qq = (lk == wmax) ? lp : INT_MAX;
+#pragma unroll
+          for (int off = 32; off > 0; off >>= 1) qq = min(qq, __shfl_xor(qq, off, 64));
+          wl = __ffsll((long long)__ballot(lk == wmax && lp == qq)) - 1;
+        }
+        const int wq = __builtin_amdgcn_readlane(lq, wl);
+        const int bpos = __builtin_amdgcn_readlane(lp, wl);
+        // two statically indexed stores (a select between p[0] and p[1]
+        // would become a dynamic index and push p to scratch)
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          if (lane == wl && wq == q) {
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+              if (c >= jj) prow[c] = p[q][c];
+          }
+        }
+        if (lane == 0) pivs[jj] = bpos;
+        __syncthreads();  // one wave: orders the LDS broadcast
+        const double pv = prow[jj];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          if (lane == wl && q == wq) {
+            live[q] = false;
+            pos[q] = jj;
+          } else if (pos[q] == jj) {
+            pos[q] = bpos;
+          }
+        }
+        if (pv != 0.0) {
+          const bool use_rcp = fabs(pv) >= DBL_MIN;
+          const double rcp = 1.0 / pv;
+#pragma unroll
+          for (int q = 0; q < R; ++q) {
+            if (live[q]) {
+              const double l = use_rcp ? p[q][jj] * rcp : p[q][jj] / pv;
+              p[q][jj] = l;
+#pragma unroll
+              for (int c = 0; c < NB; ++c)
+                if (c > jj) p[q][c] -= l * prow[c];
+            }
+          }
+        } else if (lane == 0 && info == 0) {
+          info = j0 + jj + 1;
+        }
+        __syncthreads();  // prow is rewritten by the next column
+      }
+    }
+    // rows to their final positions; L11 (padded) to LDS
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int rr = q * 64 + lane;
+      if (rr < rows) {
+        double* Arow = A + (int64_t)(j0 + pos[q]) + (int64_t)j0 * lda;
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+          if (c < nb) Arow[(int64_t)c * lda] = p[q][c];
+        if (pos[q] < NB) {
+#pragma unroll
+          for (int c = 0; c < NB; ++c) L11[pos[q] * (NB + 1) + c] = p[q][c];
+        }
+      }
+    }
+    for (int t = nb + lane; t < NB; t += 64)
+#pragma unroll
+      for (int c = 0; c < NB; ++c) L11[t * (NB + 1) + c] = 0.0;
+    if (lane < nb) ipiv0[(int64_t)b * n + j0 + lane] = j0 + pivs[lane] + 1;
+    if (lane == 0) {
+      for (int jj = 0; jj < nb; ++jj) {
+        const int pp = pivs[jj];
+        if (pp != jj) {
+          const int tmp = perm[j0 + jj];
+          perm[j0 + jj] = perm[j0 + pp];
+          perm[j0 + pp] = tmp;
+        }
+      }
+    }
+    __syncthreads();
+    const int m2 = n - j0 - nb;
+    for (int t = lane; t < n - nb; t += 64) {
+      const int col = t < j0 ? t : t + nb;
+      double* Ac = A + (int64_t)col * lda;
+      for (int jj = 0; jj < nb; ++jj) {
+        const int pp = pivs[jj];
+        if (pp != jj) {
+          const double tmp = Ac[j0 + jj];
+          Ac[j0 + jj] = Ac[j0 + pp];
+          Ac[j0 + pp] = tmp;
+        }
+      }
+    }
+    if (m2 > 0) {
+      for (int e = lane; e < NB * NB; e += 64) {
+        const int i = e / NB, j = e % NB;
+        Xi[i * (NB + 1) + j] = (i == j) ? 1.0 : 0.0;
+      }
+      __syncthreads();
+      for (int h = 1; h < NB; h <<= 1) {
+        for (int e = lane; e < NB * h / 2; e += 64) {
+          const int q = e / (h * h), r = e % (h * h);
+          const int s0 = q * 2 * h, i = s0 + h + r / h, j = s0 + r % h;
+          double acc = 0.0;
+          for (int k = j; k < s0 + h; ++k) acc += L11[i * (NB + 1) + k] * Xi[k * (NB + 1) + j];
+          Tt[i * (NB + 1) + j] = acc;
+        }
+        __syncthreads();
+        for (int e = lane; e < NB * h / 2; e += 64) {
+          const int q = e / (h * h), r = e % (h * h);
+          const int s0 = q * 2 * h, i = s0 + h + r / h, j = s0 + r % h;
+          double acc = 0.0;
+          for (int k = s0 + h; k <= i; ++k) acc += Xi[i * (NB + 1) + k] * Tt[k * (NB + 1) + j];
+          Xi[i * (NB + 1) + j] = -acc;
+        }
+        __syncthreads();
+      }
+      const int o = j0 + nb;
+      const int nct = (m2 + 15) >> 4;
+      for (int ct = 0; ct < nct; ++ct) {
+        const int col = ct * 16 + (lane & 15);
+        d4 u[NB / 16];
+#pragma unroll
+        for (int q = 0; q < NB / 16; ++q) u[q] = d4zero();
+#pragma unroll
+        for (int kk = 0; kk < NB / 4; ++kk) {
+          const int k = kk * 4 + (lane >> 4);
+          const double bf = (k < nb && col < m2) ? A[(int64_t)(j0 + k) + (int64_t)(o + col) * lda] : 0.0;
+#pragma unroll
+          for (int q = 0; q < NB / 16; ++q)
+            u[q] = mfma(Xi[(q * 16 + (lane & 15)) * (NB + 1) + k], bf, u[q]);
+        }
+#pragma unroll
+        for (int q = 0; q < NB / 16; ++q)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int r = q * 16 + crow(lane, v);
+            if (col < m2 && r < nb) A[(int64_t)(j0 + r) + (int64_t)(o + col) * lda] = u[q][v];
+          }
+      }
+      __syncthreads();
+      // trailing update, 32x32 blocks
+      const int t32 = (m2 + 31) >> 5;
+      for (int blk = 0; blk < t32 * t32; ++blk) {
         const int r0 = (blk % t32) * 32, c0 = (blk / t32) * 32;
         d4 acc[2][2];
 #pragma unroll
@@ -479,30 +870,30 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
               const int row = r0 + a * 16 + crow(lane, v);
-              acc[a][bb][v] = (row < m2 && col < m2)
-                                  ? A[(int64_t)(o + row) + (int64_t)(o + col) * lda]
-                                  : 0.0;
+              acc[a][bb][v] = (row < m2 && col < m2) ? A[(int64_t)(o + row) + (int64_t)(o + col) * lda] : 0.0;
             }
           }
+        double af[NB / 4][2], bf[NB / 4][2];
 #pragma unroll
         for (int kk = 0; kk < NB / 4; ++kk) {
           const int k = kk * 4 + (lane >> 4);
-          double af[2], bf[2];
 #pragma unroll
           for (int a = 0; a < 2; ++a) {
             const int row = r0 + a * 16 + (lane & 15);
-            af[a] = (k < nb && row < m2) ? -A[(int64_t)(o + row) + (int64_t)(j0 + k) * lda] : 0.0;
+            af[kk][a] = (k < nb && row < m2) ? A[(int64_t)(o + row) + (int64_t)(j0 + k) * lda] : 0.0;
           }
 #pragma unroll
           for (int bb = 0; bb < 2; ++bb) {
             const int col = c0 + bb * 16 + (lane & 15);
-            bf[bb] = (k < nb && col < m2) ? A[(int64_t)(j0 + k) + (int64_t)(o + col) * lda] : 0.0;
+            bf[kk][bb] = (k < nb && col < m2) ? A[(int64_t)(j0 + k) + (int64_t)(o + col) * lda] : 0.0;
           }
+        }
+#pragma unroll
+        for (int kk = 0; kk < NB / 4; ++kk)
 #pragma unroll
           for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int bb = 0; bb < 2; ++bb) acc[a][bb] = mfma(af[a], bf[bb], acc[a][bb]);
-        }
+            for (int bb = 0; bb < 2; ++bb) acc[a][bb] = mfma(-af[kk][a], bf[kk][bb], acc[a][bb]);
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -511,16 +902,16 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
               const int row = r0 + a * 16 + crow(lane, v);
-              if (row < m2 && col < m2)
-                A[(int64_t)(o + row) + (int64_t)(o + col) * lda] = acc[a][bb][v];
+              if (row < m2 && col < m2) A[(int64_t)(o + row) + (int64_t)(o + col) * lda] = acc[a][bb][v];
             }
           }
       }
     }
     __syncthreads();
   }
-  if (t < n) perm0[(int64_t)b * n + t] = perm[t];
-  if (t == 0) info0[b] = info;
+  for (int q = 0; q < R; ++q)
+    if (q * 64 + lane < n) perm0[(int64_t)b * n + q * 64 + lane] = perm[q * 64 + lane];
+  if (lane == 0) info0[b] = info;
 }
 
 static size_t getrf_lds_bytes(int nb, int n) {
@@ -532,9 +923,19 @@ hipError_t launch_getrf(int n, double* A, int64_t lda, int64_t bstride, int batc
                         int* perm, int* info, hipStream_t st) {
   if (batch <= 0 || n <= 0) return hipSuccess;
   if (n > kMaxBlock) return hipErrorInvalidValue;
+  if (n <= 128) {
+    hipLaunchKernelGGL(k_getrf_w64<32>, dim3(batch), dim3(64), 0, st, A, lda, bstride, n, ipiv,
+                       perm, info);
+    return hipGetLastError();
+  }
+  if (n <= 256) {
+    hipLaunchKernelGGL((k_getrf_reg<GETRF_NB, 256>), dim3(batch), dim3(256), 0, st, A, lda, bstride,
+                       n, ipiv, perm, info);
+    return hipGetLastError();
+  }
   if (n <= 512) {
-    hipLaunchKernelGGL(k_getrf_reg<GETRF_NB>, dim3(batch), dim3(512), 0, st, A, lda, bstride, n,
-                       ipiv, perm, info);
+    hipLaunchKernelGGL((k_getrf_reg<GETRF_NB, 512>), dim3(batch), dim3(512), 0, st, A, lda, bstride,
+                       n, ipiv, perm, info);
     return hipGetLastError();
   }
   const size_t lim = 160 * 1024;
