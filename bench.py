@@ -85,6 +85,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--n", type=int, default=262144)
     ap.add_argument("--leaf", type=int, default=512)
@@ -208,6 +209,54 @@ def main():
             traffic = None
     roof["traffic"] = traffic
 
+    # differentiable solve (N=65536, leaf 256, rank 32, 16 RHS): forward
+    # factor+solve, then the adjoint solve and the K/U/V gradients
+    adj = None
+    if not args.no_adjoint:
+        aN, am, ar, as_ = 65536, 256, 32, 16
+        aL = int(round(math.log2(aN // am)))
+        aK0, aU, aV, aB0 = F.workload_gaussian(aN, am, ar, as_, seed=0)
+        aplan = F.HodlrPlan(aN, am, aL, ar, nrhs_max=as_)
+        aK = aK0.clone()
+        aX = aB0.clone()
+        Xstar = F.fortran(torch.randn(as_, aN, dtype=torch.float64, device="cuda").t())
+        fw = []
+        for i in range(W + K_):
+            aK.copy_(aK0)
+            aX.copy_(aB0)
+            ev0.record()
+            aplan.factor_solve(aK, aU, aV, aX)
+            ev1.record()
+            torch.cuda.synchronize()
+            if i >= W:
+                fw.append(ev0.elapsed_time(ev1))
+        G = aX - Xstar
+        Lam = F.fortran_empty(aN, as_)
+        for i in range(W):
+            Lam.copy_(G)
+            aplan.solve_adjoint(Lam)
+            F.grad(aU, aV, aX, Lam, am)
+        torch.cuda.synchronize()
+        bw = []
+        for i in range(K_):
+            Lam.copy_(G)
+            torch.cuda.synchronize()
+            ev0.record()
+            aplan.solve_adjoint(Lam)
+            gK, gU, gV = F.grad(aU, aV, aX, Lam, am)
+            ev1.record()
+            torch.cuda.synchronize()
+            bw.append(ev0.elapsed_time(ev1))
+        assert aplan.status()[0] == 0
+        adj = {"workload": f"N={aN} leaf={am} levels={aL} rank={ar} nrhs={as_} fp64, "
+                           "loss 0.5||x - x*||^2",
+               "adjoint_ms": round(statistics.median(bw), 4),
+               "forward_ms": round(statistics.median(fw), 4)}
+        log(f"adjoint (transposed solve + dK, dU, dV): {adj['adjoint_ms']:.3f} ms; "
+            f"forward factor+solve {adj['forward_ms']:.3f} ms")
+        aplan.close()
+        del aK, aK0, aU, aV, aB0, aX, Lam, gK, gU, gV
+
     cpu = None
     if not args.no_cpu:
         nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -244,7 +293,8 @@ def main():
         "solve_only_ms": round(solve_ms, 4),
         "relative_residual": rel_res,
         "mixed_ir": None,
-        "adjoint_ms": None,
+        "adjoint_ms": adj["adjoint_ms"] if adj else None,
+        "adjoint": adj,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -21,7 +21,7 @@ import torch
 __all__ = [
     "lib", "lib_path", "available", "fortran_empty", "fortran", "HodlrPlan", "hsolve",
     "gemm_red", "gemm_bro", "node_solve", "matvec", "generate", "workload_gaussian",
-    "dgesv_batched", "HodlrError",
+    "dgesv_batched", "HodlrError", "grad", "HodlrSolveFunction", "hodlr_solve",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -67,6 +67,9 @@ def lib():
         "hodlr_solve": ([vp, vp, i64, ci, vp], ci),
         "hodlr_factor_solve": ([vp, vp, i64, vp, i64, vp, i64, vp, i64, ci, vp], ci),
         "hodlr_plan_status": ([vp, vp, C.POINTER(ci)], ci),
+        "hodlr_solve_adjoint": ([vp, vp, i64, ci, vp], ci),
+        "hodlr_grad": ([i64, ci, ci, ci, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64,
+                        vp, i64, vp], ci),
         "hodlr_matvec": ([i64, ci, ci, ci, vp, i64, vp, i64, vp, i64, vp, i64, dbl, dbl, vp, i64,
                           vp], ci),
         "hodlr_hsolve_ref": ([ci, ci, ci, ci, ci, vp, vp, vp, vp], ci),
@@ -206,6 +209,14 @@ class HodlrPlan:
               where="hodlr_solve")
         return B
 
+    def solve_adjoint(self, G):
+        """G <- A^-T G with the stored factorization."""
+        _check(G, "G")
+        G2 = G if G.dim() == 2 else G.view(-1, 1)
+        _call(lib().hodlr_solve_adjoint, self._h, _ptr(G2), _ld(G2), G2.shape[1], _stream(),
+              where="hodlr_solve_adjoint")
+        return G
+
     def status(self):
         w = C.c_int(-1)
         st = lib().hodlr_plan_status(self._h, _stream(), C.byref(w))
@@ -267,6 +278,73 @@ def matvec(K, U, V, X, leaf, alpha=1.0, beta=0.0, Y=None):
           _ptr(U), _ld(U), _ptr(V), _ld(V), _ptr(X2), _ld(X2), alpha, beta, _ptr(Y2), _ld(Y2),
           _stream(), where="hodlr_matvec")
     return Y
+
+
+def grad(U, V, X, Lam, leaf, dK=True, dU=True, dV=True):
+    """Gradients of a loss l(X), X = A^-1 B, given Lam = A^-T dl/dX:
+    dK_i = -Lam_i X_i^T, dU_i = -Lam_i (V^T X - V_i^T X_i)^T,
+    dV_i = -X_i (U^T Lam - U_i^T Lam_i)^T.  Returns (dK, dU, dV); a gradient
+    not asked for is None."""
+    for t, nm in ((U, "U"), (V, "V"), (X, "X"), (Lam, "Lam")):
+        _check(t, nm)
+    N, r = U.shape
+    X2 = X if X.dim() == 2 else X.view(-1, 1)
+    L2 = Lam if Lam.dim() == 2 else Lam.view(-1, 1)
+    gK = fortran_empty(N, leaf, device=U.device) if dK else None
+    gU = fortran_empty(N, r, device=U.device) if dU else None
+    gV = fortran_empty(N, r, device=U.device) if dV else None
+    nul = C.c_void_p(None)
+    _call(lib().hodlr_grad, N, leaf, r, X2.shape[1], _ptr(U), _ld(U), _ptr(V), _ld(V), _ptr(X2),
+          _ld(X2), _ptr(L2), _ld(L2), _ptr(gK) if dK else nul, _ld(gK) if dK else N,
+          _ptr(gU) if dU else nul, _ld(gU) if dU else N, _ptr(gV) if dV else nul,
+          _ld(gV) if dV else N, _stream(), where="hodlr_grad")
+    return gK, gU, gV
+
+
+class HodlrSolveFunction(torch.autograd.Function):
+    """X = A^-1 B for the HODLR matrix A(K, U, V), differentiable in K, U, V
+    and B.  Forward: fused factor + solve on a copy of K; backward: the
+    adjoint solve Lam = A^-T dX with the stored factorization, then
+    dB = Lam and the rank-k / leaf outer-product gradients (hodlr_grad)."""
+
+    @staticmethod
+    def forward(ctx, K, U, V, B, leaf):
+        N = K.shape[0]
+        levels = int(round(math.log2(N // leaf)))
+        s = B.shape[1] if B.dim() == 2 else 1
+        Kf = fortran_empty(N, leaf, device=K.device)  # overwritten by the leaf LU
+        Kf.copy_(K.detach())
+        Uf, Vf = fortran(U.detach()), fortran(V.detach())
+        X = fortran_empty(N, s, device=B.device)
+        X.copy_(B.detach().reshape(N, s))
+        plan = HodlrPlan(N, leaf, levels, U.shape[1], nrhs_max=s)
+        plan.factor_solve(Kf, Uf, Vf, X)
+        st, where = plan.status()
+        if st != 0:
+            plan.close()
+            raise HodlrError(st, f"factor (block {where})")
+        ctx.plan = plan
+        ctx.leaf = leaf
+        ctx.bshape = B.shape
+        ctx.save_for_backward(Uf, Vf, X)
+        return X if B.dim() == 2 else X.view(-1)
+
+    @staticmethod
+    def backward(ctx, gX):
+        Uf, Vf, X = ctx.saved_tensors
+        N = X.shape[0]
+        Lam = fortran_empty(N, X.shape[1], device=X.device)
+        Lam.copy_(gX.reshape(N, X.shape[1]))
+        ctx.plan.solve_adjoint(Lam)
+        need = ctx.needs_input_grad
+        gK, gU, gV = grad(Uf, Vf, X, Lam, ctx.leaf, dK=need[0], dU=need[1], dV=need[2])
+        gB = Lam.view(ctx.bshape) if need[3] else None
+        return gK, gU, gV, gB, None
+
+
+def hodlr_solve(K, U, V, B, leaf):
+    """Differentiable HODLR solve X = A(K, U, V)^-1 B (torch.autograd)."""
+    return HodlrSolveFunction.apply(K, U, V, B, leaf)
 
 
 def dgesv_batched(A, B, n, batch):

@@ -517,6 +517,85 @@ int hodlr_solve(hodlr_plan p, double* B, int64_t ldb, int nrhs, hipStream_t st) 
   return HODLR_OK;
 }
 
+// Transpose of hodlr_solve's operator: levels root first, then K^-T.
+int hodlr_solve_adjoint(hodlr_plan p, double* G, int64_t ldg, int nrhs, hipStream_t st) {
+  if (!p || !G || nrhs < 0) return HODLR_ERR_ARG;
+  if (!p->factored) return HODLR_ERR_STATE;
+  if (check_ld(ldg, p->N)) return HODLR_ERR_ARG;
+  DeviceGuard g(p->d.device);
+  const int r = p->r, m = p->m;
+  const int64_t N = p->N;
+  const int chunk = r + (p->smax > 0 ? p->smax : 1);
+  for (int c0 = 0; c0 < nrhs; c0 += chunk) {
+    const int nc = std::min(chunk, nrhs - c0);
+    double* Gc = G + (int64_t)c0 * ldg;
+    ColSeg x = hodlr::seg_single(Gc, ldg);
+    for (int t = 0; t < p->L; ++t) {
+      const int nchild = 1 << (t + 1), nnode = 1 << t;
+      const int64_t n_child = N >> (t + 1);
+      const double* Yt = p->Y + (int64_t)t * N * r;
+      int cpc = 0;
+      // z_c = u_c^T g_c per child, stacked unswapped per node
+      HIP_TRY(hodlr::launch_reduce(Yt, N, r, x, nc, n_child, nchild, reduce_rc(n_child), p->part,
+                                   &cpc, st));
+      CoreBufs c = core_level(p, t);
+      HIP_TRY(hodlr::launch_assemble(p->part, cpc, r, nc, 0, 0, nc, nnode, nullptr, c.R, st,
+                                     /*swap=*/false));
+      ColSeg rx = hodlr::seg_single(c.R, 2 * r);
+      HIP_TRY(hodlr::launch_getrs_t(2 * r, c.S, 2 * r, (int64_t)4 * r * r, nnode, c.perm, c.dinv,
+                                    rx, (int64_t)2 * r * nc, nc, st));
+      // g_c0 -= V_c0 y_1, g_c1 -= V_c1 y_0
+      HIP_TRY(hodlr::launch_update(p->V, p->ldv, r, c.R, 2 * r, (int64_t)2 * r * nc, 1, n_child,
+                                   nchild, -1.0, x, nc, st, /*half_xor=*/1));
+    }
+    HIP_TRY(hodlr::launch_getrs_t(m, p->K, p->ldk, m, (int)p->P, p->leaf_perm, p->leaf_dinv, x, m,
+                                  nc, st));
+  }
+  return HODLR_OK;
+}
+
+int hodlr_grad(int64_t n, int leaf, int rank, int nrhs, const double* U, int64_t ldu,
+               const double* V, int64_t ldv, const double* X, int64_t ldx, const double* Lam,
+               int64_t ldl, double* dK, int64_t lddk, double* dU, int64_t lddu, double* dV,
+               int64_t lddv, hipStream_t st) {
+  if (n <= 0 || leaf <= 0 || rank <= 0 || nrhs < 0 || n % leaf) return HODLR_ERR_ARG;
+  if (!X || !Lam || ldx < n || ldl < n) return HODLR_ERR_ARG;
+  if ((dU || dV) && (!U || !V || ldu < n || ldv < n)) return HODLR_ERR_ARG;
+  if ((dK && lddk < n) || (dU && lddu < n) || (dV && lddv < n)) return HODLR_ERR_ARG;
+  if (nrhs == 0) {
+    if (dK) HIP_TRY(hodlr::launch_fill_scale(n, leaf, dK, lddk, 0, 0.0, st));
+    if (dU) HIP_TRY(hodlr::launch_fill_scale(n, rank, dU, lddu, 0, 0.0, st));
+    if (dV) HIP_TRY(hodlr::launch_fill_scale(n, rank, dV, lddv, 0, 0.0, st));
+    return HODLR_OK;
+  }
+  if (dK)
+    HIP_TRY(hodlr::launch_leaf_outer(n, leaf, leaf, nrhs, -1.0, Lam, ldl, X, ldx, nullptr, nullptr,
+                                     dK, lddk, st));
+  if (!dU && !dV) return HODLR_OK;
+  const int64_t pe = (n / leaf + 1) * (int64_t)rank * nrhs;
+  void* w = nullptr;
+  int rc = workspace(sizeof(double) * (size_t)(2 * pe), &w);
+  if (rc) return rc;
+  double* pw = static_cast<double*>(w);
+  double* vx_part = pw;
+  double* vx_tot = pw + (n / leaf) * (int64_t)rank * nrhs;
+  double* ul_part = pw + pe;
+  double* ul_tot = ul_part + (n / leaf) * (int64_t)rank * nrhs;
+  if (dU) {
+    HIP_TRY(hodlr::launch_leaf_projections(n, leaf, rank, nrhs, V, ldv, X, ldx, vx_part, vx_tot,
+                                           st));
+    HIP_TRY(hodlr::launch_leaf_outer(n, leaf, rank, nrhs, -1.0, Lam, ldl, nullptr, 0, vx_part,
+                                     vx_tot, dU, lddu, st));
+  }
+  if (dV) {
+    HIP_TRY(hodlr::launch_leaf_projections(n, leaf, rank, nrhs, U, ldu, Lam, ldl, ul_part, ul_tot,
+                                           st));
+    HIP_TRY(hodlr::launch_leaf_outer(n, leaf, rank, nrhs, -1.0, X, ldx, nullptr, 0, ul_part,
+                                     ul_tot, dV, lddv, st));
+  }
+  return HODLR_OK;
+}
+
 int hodlr_plan_status(hodlr_plan p, hipStream_t st, int* where) {
   if (!p) return HODLR_ERR_ARG;
   DeviceGuard g(p->d.device);

@@ -67,16 +67,20 @@ int64_t reduce_chunks_per_child(int64_t n_child, int64_t rc);
 // Per node k (children 2k, 2k+1), summing each child's cpc partials in order:
 //   S_k = [I, G_1; G_0, I] (2r x 2r, ld 2r)  with G_c = part cols [gcol, gcol+r)
 //   R_k = [H_1; H_0]       (2r x nh, ld 2r)  with H_c = part cols [hcol, hcol+nh)
+// (R_k = [H_0; H_1] when swap is false: the adjoint's u_c^T g_c stacking).
 // S may be null (solve with a stored core factorization).
 hipError_t launch_assemble(const double* part, int cpc, int r, int ncols_part, int gcol,
-                           int hcol, int nh, int nnode, double* S, double* R, hipStream_t st);
+                           int hcol, int nh, int nnode, double* S, double* R, hipStream_t st,
+                           bool swap = true);
 
 // X_c[:, j] += alpha * u_c * eta_s[:, j] for row blocks c of n_child rows,
-// slot s = c / csz, eta_s = eta + (s/2)*eta_node_stride + (s%2)*r (ld eta_ld),
-// j < ncols.  HODLR levels use csz = 1 (slot = child).
+// slot s = c / csz, eta_s = eta + (s/2)*eta_node_stride + ((s%2)^half_xor)*r
+// (ld eta_ld), j < ncols.  HODLR levels use csz = 1 (slot = child); the
+// adjoint uses half_xor = 1 (child c takes the other child's half).
 hipError_t launch_update(const double* u, int64_t ldu, int r, const double* eta,
                          int64_t eta_ld, int64_t eta_node_stride, int csz, int64_t n_child,
-                         int nchild, double alpha, const ColSeg& X, int ncols, hipStream_t st);
+                         int nchild, double alpha, const ColSeg& X, int ncols, hipStream_t st,
+                         int half_xor = 0);
 // S and swapped R for LMatrix::node_solve from VTu / VTd slots.
 hipError_t launch_node_assemble(int nnode, int r, const double* VTu, int64_t ldu,
                                 const double* VTd, int64_t ldd, int nrhs, double* S, double* R,
@@ -92,6 +96,20 @@ hipError_t launch_matvec(int64_t n, int leaf, int rank, int nrhs, const double* 
                          int64_t ldv, const double* X, int64_t ldx, double alpha, double beta,
                          double* Y, int64_t ldy, double* work, hipStream_t st);
 int64_t matvec_work_elems(int64_t n, int leaf, int rank, int nrhs);
+// part (nleaf x r x nrhs) = V_i^T X_i per leaf, tot (r x nrhs) = their sum
+// in leaf order.
+hipError_t launch_leaf_projections(int64_t n, int leaf, int rank, int nrhs, const double* V,
+                                   int64_t ldv, const double* X, int64_t ldx, double* part,
+                                   double* tot, hipStream_t st);
+
+// Per leaf i (rows [i*leaf, (i+1)*leaf)), C_i (leaf x nc, ld ldc) =
+//   alpha * A_i (leaf x k, ld lda) * B_i^T
+// with B_i (nc x k) = B rows of leaf i (ld ldb) when tot is null, else
+// B_i = tot - part_i with tot, part_i (nc x k, ld nc) from
+// launch_leaf_projections (nc = rank).
+hipError_t launch_leaf_outer(int64_t n, int leaf, int nc, int k, double alpha, const double* A,
+                             int64_t lda, const double* B, int64_t ldb, const double* part,
+                             const double* tot, double* C, int64_t ldc, hipStream_t st);
 
 hipError_t launch_generate(uint64_t seed, uint32_t stream_id, uint32_t system, int kind,
                            int64_t rows, int cols, double scale, double* out, int64_t ld,

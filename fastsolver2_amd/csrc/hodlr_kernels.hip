@@ -1435,7 +1435,7 @@ __device__ __forceinline__ double sum_part(const double* part, int child, int cp
 
 __global__ void __launch_bounds__(256)
 k_assemble(const double* __restrict__ part, int cpc, int r, int ncols_part, int gcol, int hcol,
-           int nh, double* __restrict__ S, double* __restrict__ R) {
+           int nh, double* __restrict__ S, double* __restrict__ R, int swap) {
   const int node = blockIdx.y;
   const int n2 = 2 * r;
   const int64_t pstride = (int64_t)r * ncols_part;
@@ -1457,23 +1457,24 @@ k_assemble(const double* __restrict__ part, int cpc, int r, int ncols_part, int 
     } else {
       const int64_t f = e - nS;
       const int i = (int)(f % n2), j = (int)(f / n2);
-      const double v = (i < r)
-          ? sum_part(part, 2 * node + 1, cpc, pstride, i + (int64_t)(hcol + j) * r)
-          : sum_part(part, 2 * node, cpc, pstride, (i - r) + (int64_t)(hcol + j) * r);
+      const int hi = i >= r;  // lower half of R
+      const int child = 2 * node + (swap ? 1 - hi : hi);
+      const double v = sum_part(part, child, cpc, pstride, (i - hi * r) + (int64_t)(hcol + j) * r);
       R[(int64_t)node * n2 * nh + i + (int64_t)j * n2] = v;
     }
   }
 }
 
 hipError_t launch_assemble(const double* part, int cpc, int r, int ncols_part, int gcol, int hcol,
-                           int nh, int nnode, double* S, double* R, hipStream_t st) {
+                           int nh, int nnode, double* S, double* R, hipStream_t st,
+                           bool swap) {
   if (nnode <= 0) return hipSuccess;
   const int64_t total = (S ? 4LL * r * r : 0) + 2LL * r * nh;
   int gx = (int)((total + 255) / 256);
   if (gx > 64) gx = 64;
   if (gx < 1) gx = 1;
   hipLaunchKernelGGL(k_assemble, dim3(gx, nnode), dim3(256), 0, st, part, cpc, r, ncols_part,
-                     gcol, hcol, nh, S, R);
+                     gcol, hcol, nh, S, R, swap ? 1 : 0);
   return hipGetLastError();
 }
 
@@ -1487,7 +1488,7 @@ template <int NCT>
 __global__ void __launch_bounds__(256)
 k_update(const double* __restrict__ u, int64_t ldu, int r, const double* __restrict__ eta,
          int64_t eta_ld, int64_t eta_ns, int csz, int64_t n_child, int gpc, double alpha,
-         ColSeg X, int ncols) {
+         ColSeg X, int ncols, int half_xor) {
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int child = blockIdx.x / gpc, g = blockIdx.x % gpc;
@@ -1495,7 +1496,7 @@ k_update(const double* __restrict__ u, int64_t ldu, int r, const double* __restr
   const int64_t row0 = (int64_t)child * n_child + (int64_t)g * 64 + wid * 16;
   const int j0 = blockIdx.y * NCT * 16;
   const int slot = child / csz;
-  const double* E = eta + (int64_t)(slot >> 1) * eta_ns + (int64_t)(slot & 1) * r;
+  const double* E = eta + (int64_t)(slot >> 1) * eta_ns + (int64_t)((slot & 1) ^ half_xor) * r;
   d4 acc[NCT];
 #pragma unroll
   for (int ct = 0; ct < NCT; ++ct) {
@@ -1532,12 +1533,13 @@ k_update(const double* __restrict__ u, int64_t ldu, int r, const double* __restr
 
 hipError_t launch_update(const double* u, int64_t ldu, int r, const double* eta, int64_t eta_ld,
                          int64_t eta_node_stride, int csz, int64_t n_child, int nchild,
-                         double alpha, const ColSeg& X, int ncols, hipStream_t st) {
+                         double alpha, const ColSeg& X, int ncols, hipStream_t st,
+                         int half_xor) {
   if (nchild <= 0 || ncols <= 0) return hipSuccess;
   const int64_t gpc = (n_child + 63) / 64;
   dim3 grid((unsigned)(nchild * gpc), (unsigned)((ncols + 127) / 128));
   hipLaunchKernelGGL(k_update<8>, grid, dim3(256), 0, st, u, ldu, r, eta, eta_ld,
-                     eta_node_stride, csz, n_child, (int)gpc, alpha, X, ncols);
+                     eta_node_stride, csz, n_child, (int)gpc, alpha, X, ncols, half_xor & 1);
   return hipGetLastError();
 }
 
@@ -1684,6 +1686,79 @@ k_mv_apply(int leaf, int r, int nrhs, const double* __restrict__ K, int64_t ldk,
 int64_t matvec_work_elems(int64_t n, int leaf, int rank, int nrhs) {
   const int64_t nleaf = n / leaf;
   return (nleaf + 1) * (int64_t)rank * nrhs;
+}
+
+hipError_t launch_leaf_projections(int64_t n, int leaf, int rank, int nrhs, const double* V,
+                                   int64_t ldv, const double* X, int64_t ldx, double* part,
+                                   double* tot, hipStream_t st) {
+  const int64_t nleaf = n / leaf;
+  ColSeg xs = seg_single(const_cast<double*>(X), ldx);
+  hipError_t e = launch_reduce(V, ldv, rank, xs, nrhs, leaf, (int)nleaf, leaf, part, nullptr, st);
+  if (e != hipSuccess) return e;
+  const int64_t elems = (int64_t)rank * nrhs;
+  hipLaunchKernelGGL(k_mv_total, dim3((unsigned)((elems + 255) / 256)), dim3(256), 0, st, part,
+                     (int)nleaf, elems, tot);
+  return hipGetLastError();
+}
+
+// =====================================================================
+// Per-leaf outer products for the gradients: C_i = alpha * A_i * B_i^T
+// with a short inner dimension k (the number of right-hand sides).
+// Workgroup tile: 64 rows (one 16-row MFMA tile per wave) x NCT*16
+// columns; the k loop runs in steps of 4 with zero padding.
+// =====================================================================
+template <int NCT>
+__global__ void __launch_bounds__(256)
+k_leaf_outer(int leaf, int nc, int k, double alpha, const double* __restrict__ A, int64_t lda,
+             const double* __restrict__ B, int64_t ldb, const double* __restrict__ part,
+             const double* __restrict__ tot, double* __restrict__ C, int64_t ldc, int gpl) {
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int li = blockIdx.x / gpl, g = blockIdx.x % gpl;
+  const int64_t l0 = (int64_t)li * leaf;
+  const int rloc = g * 64 + wid * 16;
+  const int j0 = blockIdx.y * NCT * 16;
+  const double* P = tot ? part + (int64_t)li * nc * k : nullptr;
+  d4 acc[NCT];
+#pragma unroll
+  for (int ct = 0; ct < NCT; ++ct) acc[ct] = d4zero();
+  for (int k0 = 0; k0 < k; k0 += 4) {
+    const int kk = k0 + lk;
+    const int arow = rloc + lr;
+    const double af = (kk < k && arow < leaf) ? alpha * A[l0 + arow + (int64_t)kk * lda] : 0.0;
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) {
+      if (j0 + ct * 16 < nc) {
+        const int col = j0 + ct * 16 + lr;
+        double bf = 0.0;
+        if (kk < k && col < nc)
+          bf = tot ? tot[col + (int64_t)kk * nc] - P[col + (int64_t)kk * nc]
+                   : B[l0 + col + (int64_t)kk * ldb];
+        acc[ct] = mfma(af, bf, acc[ct]);
+      }
+    }
+  }
+#pragma unroll
+  for (int ct = 0; ct < NCT; ++ct) {
+    const int col = j0 + ct * 16 + lr;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = rloc + crow(lane, v);
+      if (row < leaf && col < nc) C[l0 + row + (int64_t)col * ldc] = acc[ct][v];
+    }
+  }
+}
+
+hipError_t launch_leaf_outer(int64_t n, int leaf, int nc, int k, double alpha, const double* A,
+                             int64_t lda, const double* B, int64_t ldb, const double* part,
+                             const double* tot, double* C, int64_t ldc, hipStream_t st) {
+  const int64_t nleaf = n / leaf;
+  if (nleaf <= 0 || nc <= 0) return hipSuccess;
+  const int gpl = (leaf + 63) / 64;
+  dim3 grid((unsigned)(nleaf * gpl), (unsigned)((nc + 63) / 64));
+  hipLaunchKernelGGL(k_leaf_outer<4>, grid, dim3(256), 0, st, leaf, nc, k, alpha, A, lda, B, ldb,
+                     part, tot, C, ldc, gpl);
+  return hipGetLastError();
 }
 
 hipError_t launch_matvec(int64_t n, int leaf, int rank, int nrhs, const double* K, int64_t ldk,

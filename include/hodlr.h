@@ -93,6 +93,27 @@ int hodlr_factor_solve(hodlr_plan plan, double *K, int64_t ldk,
                        const double *U, int64_t ldu, const double *V, int64_t ldv,
                        double *B, int64_t ldb, int nrhs, hipStream_t stream);
 
+/* Adjoint (transposed) solve with the stored factorization, in place:
+ * G <- A^-T G, nrhs columns (any number; processed in chunks).  The
+ * operator is the transpose of hodlr_solve's, applied top-down: per level
+ * t = 0..L-1 and node, y = S_t^-T [u_c0^T g_c0; u_c1^T g_c1], then
+ * g_c0 -= V_c0 y_1, g_c1 -= V_c1 y_0; finally g <- K^-T g per leaf.
+ * No reference counterpart (the reference has no gradient path); it
+ * transposes the reference's level loop (spmd_driver/solver_irregular_part.cc:104-140). */
+int hodlr_solve_adjoint(hodlr_plan plan, double *G, int64_t ldg, int nrhs, hipStream_t stream);
+
+/* Gradients of a scalar loss l(X), X = A^-1 B, with respect to the leaf
+ * blocks, U and V, given X and Lambda = A^-T dl/dX (hodlr_solve_adjoint):
+ *   dK_i = -Lambda_i X_i^T                          (leaf x leaf, rows of leaf i)
+ *   dU_i = -Lambda_i (V^T X - V_i^T X_i)^T          (leaf x rank)
+ *   dV_i = -X_i (U^T Lambda - U_i^T Lambda_i)^T     (leaf x rank)
+ * (dl/dB = Lambda).  dK is N x leaf (ld lddk), dU, dV are N x rank.  Any of
+ * dK, dU, dV may be NULL to skip it.  No reference counterpart. */
+int hodlr_grad(int64_t n, int leaf, int rank, int nrhs, const double *U, int64_t ldu,
+               const double *V, int64_t ldv, const double *X, int64_t ldx,
+               const double *Lambda, int64_t ldl, double *dK, int64_t lddk,
+               double *dU, int64_t lddu, double *dV, int64_t lddv, hipStream_t stream);
+
 /* Synchronizes `stream` and reports the numerical status of the last factor:
  * HODLR_OK, HODLR_SINGULAR_LEAF (*where = leaf index) or HODLR_SINGULAR_CORE
  * (*where = level * 2^20 + node).  Replaces the reference's
