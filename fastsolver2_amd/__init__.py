@@ -385,7 +385,7 @@ def workload_gaussian(N, leaf, r, s, seed=0, system=0, shift=None, rhs="uniform"
 
 # ---------------------------------------------------------------- profiling
 PROF_KINDS = ["getrf_leaf", "getrs_leaf", "trtri", "reduce", "assemble", "getrf_core",
-              "getrs_core", "update"]
+              "getrs_core", "update", "leaf_level"]
 
 
 def profile_enable(on=True):

@@ -11,13 +11,18 @@
 // reference carries, for every level t, its own copy u_t of U through the
 // leaf solve and all deeper levels (UTree column layout, src/tree.cc:64-73,
 // filled with identical copies by init_matrix.cc:64-71).  Every copy receives
-// exactly the same operations below its level, so at level t all of d's
-// u-blocks equal u_t.  The plan therefore carries one block Y_t = u_t per
-// level: Y_{L-1} = K^-1 U at the leaves and Y_{t-1} = Y_t - Y_t E_t, with
-// E_t the u-part of the core solve.  This is the same arithmetic on the same
-// values as the reference, without the redundant copies; Y_t is stored, so
-// later solves reuse the factorization.  hodlr_hsolve_ref keeps the
-// reference's full column layout for callers that pass it.
+// exactly the same operations below its level, so one block per level
+// suffices: Y_{L-1} = K^-1 U and Y_{t-1} = Y_t - Y_t E_t (E_t the u-part of
+// the level-t core solve, per child).  On one leaf i that recursion is a
+// right multiplication by r x r matrices, so Y_t,i = Y_{L-1,i} W_t,i with
+// W_{L-1} = I and W_{t-1,i} = W_t,i (I - E_t,c(i)); likewise the level
+// projections V_i^T [Y_t | B_t] obey P_{t-1,i} = P_t,i - P_t,i(:, 0:r) E_t,c(i)
+// and B_final = B_{L-1} - Y_{L-1} G with G_i = sum_t W_t,i E^B_t,c(i).  The
+// plan therefore touches N-row data only at the leaves (one solve on [U | B],
+// one projection, one final B update); every level is O(leaves * r^2 (r+s))
+// work on r x r blocks.  Same arithmetic as the reference up to association
+// order.  hodlr_hsolve_ref keeps the reference's full column layout for
+// callers that pass it.
 #include "../../include/hodlr.h"
 
 #include <hip/hip_runtime.h>
@@ -191,8 +196,11 @@ struct hodlr_plan_s {
   int* leaf_perm = nullptr;
   int* leaf_info = nullptr;
   double* leaf_dinv = nullptr;
-  // Y_t blocks, t = 0..L-1, each N x r (ld N)
+  // Y = Y_{L-1} = K^-1 U (N x r, ld N)
   double* Y = nullptr;
+  // W_t,i (r x r, ld r) at W + (t*P + i)*r*r; Q_i = V_i^T Y_i at Q + i*r*r
+  double* W = nullptr;
+  double* Q = nullptr;
   // cores, all levels; level t at offsets *_off[t]
   double* coreS = nullptr;
   int* core_ipiv = nullptr;
@@ -200,10 +208,11 @@ struct hodlr_plan_s {
   int* core_info = nullptr;
   double* core_dinv = nullptr;
   std::vector<int64_t> s_off, p_off, d_off, i_off;
-  // workspace
-  double* part = nullptr;
+  // workspace: per-leaf projections (two, ping-pong), accumulators, core RHS
+  double* Pb[3] = {nullptr, nullptr, nullptr};
+  double* G = nullptr;
   double* R = nullptr;
-  int64_t part_elems = 0, r_elems = 0;
+  int ncmax = 0;
   // factored operands
   const double* K = nullptr;
   const double* U = nullptr;
@@ -268,8 +277,7 @@ int leaf_phase(hodlr_plan p, double* K, int64_t ldk, const double* U, int64_t ld
   ColSeg x;
   int ncols;
   if (p->L > 0) {
-    double* Ylast = p->Y + (int64_t)(p->L - 1) * p->N * p->r;
-    x.inA = U; x.outA = Ylast; x.ldiA = ldu; x.ldoA = p->N; x.nA = p->r;
+    x.inA = U; x.outA = p->Y; x.ldiA = ldu; x.ldoA = p->N; x.nA = p->r;
     x.inB = B; x.outB = B; x.ldiB = ldb; x.ldoB = ldb;
     ncols = p->r + nrhs;
   } else {
@@ -285,48 +293,97 @@ int leaf_phase(hodlr_plan p, double* K, int64_t ldk, const double* U, int64_t ld
   return HODLR_OK;
 }
 
-// Level loop of factor (+ solve) over [Y_t | B] (nrhs may be 0).
+double* W_at(hodlr_plan p, int t) { return p->W + (int64_t)t * p->P * p->r * p->r; }
+
+// Per-leaf projections part_i = V_i^T X_i (r x nc, ld r, stride r*nc).
+int leaf_projections(hodlr_plan p, const double* Vm, int64_t ldv, const ColSeg& x, int nc,
+                     double* part, hipStream_t st) {
+  ProfScope ps(HODLR_PROF_REDUCE, st, 2.0 * (double)p->N * p->r * nc,
+               8.0 * (double)p->N * (p->r + nc));
+  int cpc = 0;
+  HIP_TRY(hodlr::launch_reduce(Vm, ldv, p->r, x, nc, p->m, (int)p->P, p->m, part, &cpc, st));
+  return HODLR_OK;
+}
+
+// B_i -= Y_i G_i for every leaf (G_i: r x nc, ld r, stride r*nc).
+int final_update(hodlr_plan p, const double* Ym, int64_t ldy, const double* Gm, int nc, double* B,
+                 int64_t ldb, hipStream_t st) {
+  hodlr::BGemmArgs g;
+  g.batch = (int)p->P; g.M = p->m; g.K = p->r; g.Nc = nc; g.csplit = nc;
+  g.A = Ym; g.lda = ldy; g.sa = p->m;
+  g.B = Gm; g.ldb = p->r; g.bshift = 0; g.bs1 = (int64_t)p->r * nc; g.bs2 = 2 * g.bs1;
+  g.alpha1 = -1.0; g.beta1 = 1.0;
+  g.D1 = B; g.ldd1 = ldb; g.sd1 = p->m;
+  g.C1 = B; g.ldc1 = ldb; g.sc1 = p->m;
+  ProfScope ps(HODLR_PROF_UPDATE, st, 2.0 * (double)p->N * p->r * nc,
+               8.0 * (double)p->N * (p->r + 2.0 * nc));
+  HIP_TRY(hodlr::launch_bgemm(g, st));
+  return HODLR_OK;
+}
+
+// Level loop of factor (+ solve, nrhs > 0) on per-leaf r x r blocks.
+// Input: [Y | B] after the leaf solve.  Output: W_t for every level, Q, the
+// core factorizations, and B overwritten by the solution.
 int level_phase(hodlr_plan p, double* B, int64_t ldb, int nrhs, hipStream_t st) {
-  const int r = p->r;
-  const int64_t N = p->N;
+  const int r = p->r, L = p->L;
+  const int64_t P = p->P;
   const int nc = r + nrhs;
-  for (int t = p->L - 1; t >= 0; --t) {
-    const int nchild = 1 << (t + 1), nnode = 1 << t;
-    const int64_t n_child = N >> (t + 1);
-    double* Yt = p->Y + (int64_t)t * N * r;
-    ColSeg x;
-    x.inA = Yt; x.outA = Yt; x.ldiA = N; x.ldoA = N; x.nA = r;
-    x.inB = B; x.outB = B; x.ldiB = ldb; x.ldoB = ldb;
-    int cpc = 0;
-    {
-      ProfScope ps(HODLR_PROF_REDUCE, st, 2.0 * (double)N * r * nc, 8.0 * (double)N * (r + nc));
-      HIP_TRY(hodlr::launch_reduce(p->V, p->ldv, r, x, nc, n_child, nchild, reduce_rc(n_child),
-                                   p->part, &cpc, st));
-    }
+  if (L == 0) return HODLR_OK;
+  ColSeg x;
+  x.inA = p->Y; x.outA = p->Y; x.ldiA = p->N; x.ldoA = p->N; x.nA = r;
+  x.inB = B; x.outB = B; x.ldiB = ldb; x.ldoB = ldb;
+  double* Pc = p->Pb[0];
+  double* Pn = p->Pb[1];
+  int rc = leaf_projections(p, p->V, p->ldv, x, nc, Pc, st);
+  if (rc) return rc;
+  // Q_i = V_i^T Y_i: the first r columns of each projection
+  HIP_TRY(hodlr::launch_copy_cols((int64_t)r * r, (int)P, Pc, (int64_t)r * nc, p->Q,
+                                  (int64_t)r * r, st));
+  HIP_TRY(hodlr::launch_eye((int)P, r, W_at(p, L - 1), st));
+  for (int t = L - 1; t >= 0; --t) {
+    const int nnode = 1 << t;
+    const int shift = L - 1 - t;  // leaf -> child at level t
     CoreBufs c = core_level(p, t);
     {
-      ProfScope ps(HODLR_PROF_ASSEMBLE, st, 0.0, 8.0 * nchild * cpc * (double)r * nc);
-      HIP_TRY(hodlr::launch_assemble(p->part, cpc, r, nc, 0, 0, nc, nnode, c.S, c.R, st));
+      ProfScope ps(HODLR_PROF_ASSEMBLE, st, 0.0, 8.0 * (double)P * r * nc);
+      HIP_TRY(hodlr::launch_assemble(Pc, 1 << shift, r, nc, 0, 0, nc, nnode, c.S, c.R, st));
     }
-    int rc = core_factor_solve(nnode, r, c, nc, st);
+    rc = core_factor_solve(nnode, r, c, nc, st);
     if (rc) return rc;
     const int64_t ens = (int64_t)2 * r * nc;
+    ProfScope ps(HODLR_PROF_LEAF_LEVEL, st,
+                 (double)P * 2.0 * r * r * ((t > 0 ? 2.0 * nc : 0.0) + nrhs),
+                 8.0 * (double)P * r * (2.0 * nc + 2.0 * r + 2.0 * nrhs));
+    // W_t E_c -> W_{t-1} = W_t - W_t E^Y, G (+)= W_t E^B
+    hodlr::BGemmArgs g;
+    g.batch = (int)P; g.M = r; g.K = r; g.Nc = nc; g.csplit = r;
+    g.A = W_at(p, t); g.lda = r; g.sa = (int64_t)r * r;
+    g.B = c.R; g.ldb = 2 * r; g.bshift = shift; g.bs2 = ens; g.bs1 = r;
     if (t > 0) {
-      // Y_{t-1} = Y_t - Y_t E_t ; B -= Y_t eta_B
-      ColSeg y;
-      y.inA = Yt; y.outA = p->Y + (int64_t)(t - 1) * N * r; y.ldiA = N; y.ldoA = N; y.nA = r;
-      y.inB = B; y.outB = B; y.ldiB = ldb; y.ldoB = ldb;
-      ProfScope ps(HODLR_PROF_UPDATE, st, 2.0 * (double)N * r * nc,
-                   8.0 * (double)N * (r + 2.0 * nc));
-      HIP_TRY(hodlr::launch_update(Yt, N, r, c.R, 2 * r, ens, 1, n_child, nchild, -1.0, y, nc, st));
-    } else if (nrhs > 0) {
-      ColSeg y = hodlr::seg_single(B, ldb);
-      ProfScope ps(HODLR_PROF_UPDATE, st, 2.0 * (double)N * r * nrhs,
-                   8.0 * (double)N * (r + 2.0 * nrhs));
-      HIP_TRY(hodlr::launch_update(Yt, N, r, c.R + (int64_t)r * 2 * r, 2 * r, ens, 1, n_child,
-                                   nchild, -1.0, y, nrhs, st));
+      g.alpha1 = -1.0; g.beta1 = 1.0;
+      g.D1 = W_at(p, t); g.ldd1 = r; g.sd1 = (int64_t)r * r;
+      g.C1 = W_at(p, t - 1); g.ldc1 = r; g.sc1 = (int64_t)r * r;
+    }
+    if (nrhs > 0) {
+      g.alpha2 = 1.0; g.beta2 = (t == L - 1) ? 0.0 : 1.0;
+      g.D2 = p->G; g.ldd2 = r; g.sd2 = (int64_t)r * nrhs;
+      g.C2 = p->G; g.ldc2 = r; g.sc2 = (int64_t)r * nrhs;
+    }
+    if (g.C1 || g.C2) HIP_TRY(hodlr::launch_bgemm(g, st));
+    if (t > 0) {
+      // P_{t-1,i} = P_t,i - P_t,i(:, 0:r) E_c
+      hodlr::BGemmArgs h;
+      h.batch = (int)P; h.M = r; h.K = r; h.Nc = nc; h.csplit = nc;
+      h.A = Pc; h.lda = r; h.sa = (int64_t)r * nc;
+      h.B = c.R; h.ldb = 2 * r; h.bshift = shift; h.bs2 = ens; h.bs1 = r;
+      h.alpha1 = -1.0; h.beta1 = 1.0;
+      h.D1 = Pc; h.ldd1 = r; h.sd1 = (int64_t)r * nc;
+      h.C1 = Pn; h.ldc1 = r; h.sc1 = (int64_t)r * nc;
+      HIP_TRY(hodlr::launch_bgemm(h, st));
+      std::swap(Pc, Pn);
     }
   }
+  if (nrhs > 0) return final_update(p, p->Y, p->N, p->G, nrhs, B, ldb, st);
   return HODLR_OK;
 }
 
@@ -411,7 +468,9 @@ int hodlr_plan_create(hodlr_plan* plan, const hodlr_desc* desc) {
   rc = rc ? rc : palloc(p, &p->leaf_perm, p->P * p->m);
   rc = rc ? rc : palloc(p, &p->leaf_info, p->P);
   rc = rc ? rc : palloc(p, &p->leaf_dinv, p->P * hodlr::dinv_elems(p->m));
-  rc = rc ? rc : palloc(p, &p->Y, (int64_t)p->L * p->N * r);
+  rc = rc ? rc : palloc(p, &p->Y, p->L > 0 ? p->N * r : 0);
+  rc = rc ? rc : palloc(p, &p->W, (int64_t)p->L * p->P * r * r);
+  rc = rc ? rc : palloc(p, &p->Q, p->L > 0 ? p->P * r * r : 0);
   int64_t so = 0, po = 0, dof = 0, io = 0;
   for (int t = 0; t < p->L; ++t) {
     const int64_t nn = (int64_t)1 << t;
@@ -423,17 +482,13 @@ int hodlr_plan_create(hodlr_plan* plan, const hodlr_desc* desc) {
   rc = rc ? rc : palloc(p, &p->core_perm, po);
   rc = rc ? rc : palloc(p, &p->core_info, io > 0 ? io : 1);
   rc = rc ? rc : palloc(p, &p->core_dinv, dof);
-  // reduce partials and core RHS for the widest operand [Y | B]
-  const int ncmax = r + (p->smax > 0 ? p->smax : 1);
-  int64_t pe = 0, re = 0;
-  for (int t = 0; t < p->L; ++t) {
-    const int64_t n_child = p->N >> (t + 1);
-    const int64_t chunks = ((int64_t)2 << t) * hodlr::reduce_chunks_per_child(n_child, reduce_rc(n_child));
-    pe = std::max<int64_t>(pe, chunks * r * ncmax);
-    re = std::max<int64_t>(re, ((int64_t)1 << t) * n2 * ncmax);
-  }
-  p->part_elems = pe; p->r_elems = re;
-  rc = rc ? rc : palloc(p, &p->part, pe);
+  // per-leaf projections / accumulators and core RHS for the widest
+  // operand: [Y | B] with nrhs_max columns, or a solve chunk of that width
+  p->ncmax = r + (p->smax > 0 ? p->smax : 1);
+  const int64_t pe = p->L > 0 ? p->P * r * p->ncmax : 0;
+  const int64_t re = p->L > 0 ? ((int64_t)1 << (p->L - 1)) * n2 * p->ncmax : 0;
+  for (int k = 0; k < 3; ++k) rc = rc ? rc : palloc(p, &p->Pb[k], pe);
+  rc = rc ? rc : palloc(p, &p->G, pe);
   rc = rc ? rc : palloc(p, &p->R, re);
   if (rc) {
     plan_free(p);
@@ -489,66 +544,114 @@ int hodlr_solve(hodlr_plan p, double* B, int64_t ldb, int nrhs, hipStream_t st) 
   if (!p->factored) return HODLR_ERR_STATE;
   if (check_ld(ldb, p->N)) return HODLR_ERR_ARG;
   DeviceGuard g(p->d.device);
-  const int r = p->r, m = p->m;
-  const int64_t N = p->N;
-  const int chunk = r + (p->smax > 0 ? p->smax : 1);  // RHS columns per pass
-  for (int c0 = 0; c0 < nrhs; c0 += chunk) {
-    const int nc = std::min(chunk, nrhs - c0);
+  const int r = p->r, m = p->m, L = p->L;
+  const int64_t P = p->P;
+  for (int c0 = 0; c0 < nrhs; c0 += p->ncmax) {
+    const int nc = std::min(p->ncmax, nrhs - c0);
     double* Bc = B + (int64_t)c0 * ldb;
     ColSeg x = hodlr::seg_single(Bc, ldb);
-    HIP_TRY(hodlr::launch_getrs(m, p->K, p->ldk, m, (int)p->P, p->leaf_perm, p->leaf_dinv, x, m,
+    HIP_TRY(hodlr::launch_getrs(m, p->K, p->ldk, m, (int)P, p->leaf_perm, p->leaf_dinv, x, m,
                                 nc, st));
-    for (int t = p->L - 1; t >= 0; --t) {
-      const int nchild = 1 << (t + 1), nnode = 1 << t;
-      const int64_t n_child = N >> (t + 1);
-      int cpc = 0;
-      HIP_TRY(hodlr::launch_reduce(p->V, p->ldv, r, x, nc, n_child, nchild, reduce_rc(n_child),
-                                   p->part, &cpc, st));
+    if (L == 0) continue;
+    // p_i = V_i^T B_i; per level the current projection is p_i - Q_i G_i
+    int rc = leaf_projections(p, p->V, p->ldv, x, nc, p->Pb[0], st);
+    if (rc) return rc;
+    const int64_t ps = (int64_t)r * nc;
+    for (int t = L - 1; t >= 0; --t) {
+      const int nnode = 1 << t, shift = L - 1 - t;
       CoreBufs c = core_level(p, t);
-      HIP_TRY(hodlr::launch_assemble(p->part, cpc, r, nc, 0, 0, nc, nnode, nullptr, c.R, st));
+      const double* cur = p->Pb[0];
+      if (t < L - 1) {
+        hodlr::BGemmArgs h;
+        h.batch = (int)P; h.M = r; h.K = r; h.Nc = nc; h.csplit = nc;
+        h.A = p->Q; h.lda = r; h.sa = (int64_t)r * r;
+        h.B = p->G; h.ldb = r; h.bs1 = ps; h.bs2 = 2 * ps;
+        h.alpha1 = -1.0; h.beta1 = 1.0;
+        h.D1 = p->Pb[0]; h.ldd1 = r; h.sd1 = ps;
+        h.C1 = p->Pb[1]; h.ldc1 = r; h.sc1 = ps;
+        HIP_TRY(hodlr::launch_bgemm(h, st));
+        cur = p->Pb[1];
+      }
+      HIP_TRY(hodlr::launch_assemble(cur, 1 << shift, r, nc, 0, 0, nc, nnode, nullptr, c.R, st));
       ColSeg rx = hodlr::seg_single(c.R, 2 * r);
       HIP_TRY(hodlr::launch_getrs(2 * r, c.S, 2 * r, (int64_t)4 * r * r, nnode, c.perm, c.dinv,
                                   rx, (int64_t)2 * r * nc, nc, st));
-      const double* Yt = p->Y + (int64_t)t * N * r;
-      HIP_TRY(hodlr::launch_update(Yt, N, r, c.R, 2 * r, (int64_t)2 * r * nc, 1, n_child, nchild,
-                                   -1.0, x, nc, st));
+      // G_i (+)= W_t,i eta_c
+      hodlr::BGemmArgs g2;
+      g2.batch = (int)P; g2.M = r; g2.K = r; g2.Nc = nc; g2.csplit = nc;
+      g2.A = W_at(p, t); g2.lda = r; g2.sa = (int64_t)r * r;
+      g2.B = c.R; g2.ldb = 2 * r; g2.bshift = shift; g2.bs2 = (int64_t)2 * r * nc; g2.bs1 = r;
+      g2.alpha1 = 1.0; g2.beta1 = (t == L - 1) ? 0.0 : 1.0;
+      g2.D1 = p->G; g2.ldd1 = r; g2.sd1 = ps;
+      g2.C1 = p->G; g2.ldc1 = r; g2.sc1 = ps;
+      HIP_TRY(hodlr::launch_bgemm(g2, st));
     }
+    rc = final_update(p, p->Y, p->N, p->G, nc, Bc, ldb, st);
+    if (rc) return rc;
   }
   return HODLR_OK;
 }
 
-// Transpose of hodlr_solve's operator: levels root first, then K^-T.
+// Transpose of hodlr_solve's operator, levels root first, then K^-T:
+// with h_i = Y_i^T g_i and F_i the sum of the y halves applied so far,
+// u_t,i^T g_i = W_t,i^T (h_i - Q_i^T F_i); finally g_i -= V_i F_i.
 int hodlr_solve_adjoint(hodlr_plan p, double* G, int64_t ldg, int nrhs, hipStream_t st) {
   if (!p || !G || nrhs < 0) return HODLR_ERR_ARG;
   if (!p->factored) return HODLR_ERR_STATE;
   if (check_ld(ldg, p->N)) return HODLR_ERR_ARG;
   DeviceGuard g(p->d.device);
-  const int r = p->r, m = p->m;
-  const int64_t N = p->N;
-  const int chunk = r + (p->smax > 0 ? p->smax : 1);
-  for (int c0 = 0; c0 < nrhs; c0 += chunk) {
-    const int nc = std::min(chunk, nrhs - c0);
+  const int r = p->r, m = p->m, L = p->L;
+  const int64_t P = p->P;
+  for (int c0 = 0; c0 < nrhs; c0 += p->ncmax) {
+    const int nc = std::min(p->ncmax, nrhs - c0);
     double* Gc = G + (int64_t)c0 * ldg;
     ColSeg x = hodlr::seg_single(Gc, ldg);
-    for (int t = 0; t < p->L; ++t) {
-      const int nchild = 1 << (t + 1), nnode = 1 << t;
-      const int64_t n_child = N >> (t + 1);
-      const double* Yt = p->Y + (int64_t)t * N * r;
-      int cpc = 0;
-      // z_c = u_c^T g_c per child, stacked unswapped per node
-      HIP_TRY(hodlr::launch_reduce(Yt, N, r, x, nc, n_child, nchild, reduce_rc(n_child), p->part,
-                                   &cpc, st));
-      CoreBufs c = core_level(p, t);
-      HIP_TRY(hodlr::launch_assemble(p->part, cpc, r, nc, 0, 0, nc, nnode, nullptr, c.R, st,
-                                     /*swap=*/false));
-      ColSeg rx = hodlr::seg_single(c.R, 2 * r);
-      HIP_TRY(hodlr::launch_getrs_t(2 * r, c.S, 2 * r, (int64_t)4 * r * r, nnode, c.perm, c.dinv,
-                                    rx, (int64_t)2 * r * nc, nc, st));
-      // g_c0 -= V_c0 y_1, g_c1 -= V_c1 y_0
-      HIP_TRY(hodlr::launch_update(p->V, p->ldv, r, c.R, 2 * r, (int64_t)2 * r * nc, 1, n_child,
-                                   nchild, -1.0, x, nc, st, /*half_xor=*/1));
+    if (L > 0) {
+      int rc = leaf_projections(p, p->Y, p->N, x, nc, p->Pb[0], st);
+      if (rc) return rc;
+      const int64_t ps = (int64_t)r * nc;
+      for (int t = 0; t < L; ++t) {
+        const int nnode = 1 << t, shift = L - 1 - t;
+        CoreBufs c = core_level(p, t);
+        const double* u = p->Pb[0];
+        if (t > 0) {
+          hodlr::BGemmArgs h;
+          h.batch = (int)P; h.M = r; h.K = r; h.Nc = nc; h.csplit = nc; h.transA = 1;
+          h.A = p->Q; h.lda = r; h.sa = (int64_t)r * r;
+          h.B = p->G; h.ldb = r; h.bs1 = ps; h.bs2 = 2 * ps;
+          h.alpha1 = -1.0; h.beta1 = 1.0;
+          h.D1 = p->Pb[0]; h.ldd1 = r; h.sd1 = ps;
+          h.C1 = p->Pb[1]; h.ldc1 = r; h.sc1 = ps;
+          HIP_TRY(hodlr::launch_bgemm(h, st));
+          u = p->Pb[1];
+        }
+        hodlr::BGemmArgs z;
+        z.batch = (int)P; z.M = r; z.K = r; z.Nc = nc; z.csplit = nc; z.transA = 1;
+        z.A = W_at(p, t); z.lda = r; z.sa = (int64_t)r * r;
+        z.B = u; z.ldb = r; z.bs1 = ps; z.bs2 = 2 * ps;
+        z.alpha1 = 1.0; z.beta1 = 0.0;
+        z.C1 = p->Pb[2]; z.ldc1 = r; z.sc1 = ps;
+        HIP_TRY(hodlr::launch_bgemm(z, st));
+        HIP_TRY(hodlr::launch_assemble(p->Pb[2], 1 << shift, r, nc, 0, 0, nc, nnode, nullptr, c.R,
+                                       st, /*swap=*/false));
+        ColSeg rx = hodlr::seg_single(c.R, 2 * r);
+        HIP_TRY(hodlr::launch_getrs_t(2 * r, c.S, 2 * r, (int64_t)4 * r * r, nnode, c.perm,
+                                      c.dinv, rx, (int64_t)2 * r * nc, nc, st));
+        // F_i (+)= y of the other child's half
+        hodlr::BGemmArgs f;
+        f.batch = (int)P; f.M = r; f.K = r; f.Nc = nc; f.csplit = nc;
+        f.A = nullptr;
+        f.B = c.R; f.ldb = 2 * r; f.bshift = shift; f.bs2 = (int64_t)2 * r * nc; f.bs1 = r;
+        f.bflip = 1;
+        f.alpha1 = 1.0; f.beta1 = (t == 0) ? 0.0 : 1.0;
+        f.D1 = p->G; f.ldd1 = r; f.sd1 = ps;
+        f.C1 = p->G; f.ldc1 = r; f.sc1 = ps;
+        HIP_TRY(hodlr::launch_bgemm(f, st));
+      }
+      rc = final_update(p, p->V, p->ldv, p->G, nc, Gc, ldg, st);
+      if (rc) return rc;
     }
-    HIP_TRY(hodlr::launch_getrs_t(m, p->K, p->ldk, m, (int)p->P, p->leaf_perm, p->leaf_dinv, x, m,
+    HIP_TRY(hodlr::launch_getrs_t(m, p->K, p->ldk, m, (int)P, p->leaf_perm, p->leaf_dinv, x, m,
                                   nc, st));
   }
   return HODLR_OK;

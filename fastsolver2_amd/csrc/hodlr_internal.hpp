@@ -121,6 +121,25 @@ hipError_t launch_dense_blocks(int64_t n, int leaf, int rank, const double* U, i
                                double* K, int64_t ldk, hipStream_t st);
 hipError_t launch_fill_scale(int64_t rows, int cols, double* A, int64_t ld, int mode, double v,
                              hipStream_t st);
+// Batched small GEMM over leaves, see k_bgemm.
+struct BGemmArgs {
+  int batch = 0, M = 0, K = 0, Nc = 0, csplit = 0, transA = 0, bshift = 0, bflip = 0;
+  const double* A = nullptr;
+  int64_t lda = 0, sa = 0;
+  const double* B = nullptr;
+  int64_t ldb = 0, bs2 = 0, bs1 = 0;
+  double alpha1 = 1.0, beta1 = 0.0, alpha2 = 1.0, beta2 = 0.0;
+  const double* D1 = nullptr;
+  int64_t ldd1 = 0, sd1 = 0;
+  double* C1 = nullptr;
+  int64_t ldc1 = 0, sc1 = 0;
+  const double* D2 = nullptr;
+  int64_t ldd2 = 0, sd2 = 0;
+  double* C2 = nullptr;
+  int64_t ldc2 = 0, sc2 = 0;
+};
+hipError_t launch_bgemm(const BGemmArgs& g, hipStream_t st);
+hipError_t launch_eye(int batch, int r, double* W, hipStream_t st);
 hipError_t launch_selftest_mfma(double* out, hipStream_t st);
 hipError_t launch_copy_cols(int64_t rows, int cols, const double* src, int64_t lds,
                             double* dst, int64_t ldd, hipStream_t st);

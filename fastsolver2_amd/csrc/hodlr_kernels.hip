@@ -1921,6 +1921,89 @@ __global__ void k_copy_cols(int64_t rows, int cols, const double* __restrict__ s
   }
 }
 
+// =====================================================================
+// Batched small GEMM over leaves (one workgroup per item i):
+//   C_i(:, j) = alpha * (op(A_i) B_i)(:, j) + beta * D_i(:, j)
+// written to (C1, D1, alpha1, beta1) for j < csplit and to (C2, D2,
+// alpha2, beta2) at column j - csplit otherwise; a null C skips its
+// columns.  A_i = A + i*sa (M x K, or K x M read transposed; null A is the
+// identity); B_i = B + (c>>1)*bs2 + ((c&1)^bflip)*bs1 with c = i >> bshift,
+// which selects the half of a node's core solution that belongs to the
+// leaf's child at a given level.  16 x 16 output tiles are dealt to the
+// four waves.  D may alias C; A and B must not.
+// =====================================================================
+__global__ void __launch_bounds__(256) k_bgemm(BGemmArgs g) {
+  const int i = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const double* A = g.A ? g.A + (int64_t)i * g.sa : nullptr;
+  const int c = i >> g.bshift;
+  const double* Bp = g.B + (int64_t)(c >> 1) * g.bs2 + (int64_t)((c & 1) ^ g.bflip) * g.bs1;
+  const int mt = (g.M + 15) >> 4, nt = (g.Nc + 15) >> 4;
+  for (int tile = wid; tile < mt * nt; tile += 4) {
+    const int tm = tile % mt, tn = tile / mt;
+    const int col0 = tn * 16;
+    if (col0 + 16 <= g.csplit && !g.C1) continue;
+    if (col0 >= g.csplit && !g.C2) continue;
+    const int arow = tm * 16 + lr;
+    const int bcol = col0 + lr;
+    d4 acc = d4zero();
+#pragma unroll 4
+    for (int k0 = 0; k0 < g.K; k0 += 4) {
+      const int k = k0 + lk;
+      double af = 0.0, bf = 0.0;
+      if (k < g.K && arow < g.M)
+        af = !A ? (arow == k ? 1.0 : 0.0)
+                : (g.transA ? A[k + (int64_t)arow * g.lda] : A[arow + (int64_t)k * g.lda]);
+      if (k < g.K && bcol < g.Nc) bf = Bp[k + (int64_t)bcol * g.ldb];
+      acc = mfma(af, bf, acc);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int orow = tm * 16 + crow(lane, v);
+      const int ocol = col0 + lr;
+      if (orow >= g.M || ocol >= g.Nc) continue;
+      if (ocol < g.csplit) {
+        if (g.C1) {
+          double o = g.alpha1 * acc[v];
+          if (g.beta1 != 0.0) o += g.beta1 * g.D1[(int64_t)i * g.sd1 + orow + (int64_t)ocol * g.ldd1];
+          g.C1[(int64_t)i * g.sc1 + orow + (int64_t)ocol * g.ldc1] = o;
+        }
+      } else if (g.C2) {
+        const int oc = ocol - g.csplit;
+        double o = g.alpha2 * acc[v];
+        if (g.beta2 != 0.0) o += g.beta2 * g.D2[(int64_t)i * g.sd2 + orow + (int64_t)oc * g.ldd2];
+        g.C2[(int64_t)i * g.sc2 + orow + (int64_t)oc * g.ldc2] = o;
+      }
+    }
+  }
+}
+
+hipError_t launch_bgemm(const BGemmArgs& g, hipStream_t st) {
+  if (g.batch <= 0 || g.M <= 0 || g.Nc <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bgemm, dim3((unsigned)g.batch), dim3(256), 0, st, g);
+  return hipGetLastError();
+}
+
+// W_i = I (r x r, ld r) for i < batch.
+__global__ void k_eye(int batch, int r, double* __restrict__ W) {
+  const int64_t total = (int64_t)batch * r * r;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * 256) {
+    const int64_t q = e % ((int64_t)r * r);
+    W[e] = (q % r == q / r) ? 1.0 : 0.0;
+  }
+}
+
+hipError_t launch_eye(int batch, int r, double* W, hipStream_t st) {
+  const int64_t total = (int64_t)batch * r * r;
+  if (total <= 0) return hipSuccess;
+  int64_t g = (total + 255) / 256;
+  if (g > 16384) g = 16384;
+  hipLaunchKernelGGL(k_eye, dim3((unsigned)g), dim3(256), 0, st, batch, r, W);
+  return hipGetLastError();
+}
+
 // Lane-layout self-test of v_mfma_f64_16x16x4_f64 (one wave, in bounds):
 // A[i][k] = i + 100 k, B[k][j] = (k == j % 4), so D[i][j] = i + 100 (j % 4);
 // out[4*lane + v] receives accumulator element v of each lane.

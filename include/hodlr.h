@@ -210,7 +210,8 @@ enum hodlr_prof_kind {
   HODLR_PROF_GETRF_CORE = 5, /* batched LU of the 2r x 2r cores            */
   HODLR_PROF_GETRS_CORE = 6, /* core solves                                */
   HODLR_PROF_UPDATE = 7,     /* level update [Y | B] -= Y eta (gemmBro)    */
-  HODLR_PROF_NKINDS = 8
+  HODLR_PROF_LEAF_LEVEL = 8, /* per-leaf r x r level step (W, G, projections) */
+  HODLR_PROF_NKINDS = 9
 };
 /* No reference counterpart (the reference profiles with Legion Prof,
  * tests/Makefile:119-212).  While enabled, the plan entry points record HIP
