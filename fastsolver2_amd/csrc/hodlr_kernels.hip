@@ -307,16 +307,18 @@ k_getrf(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
 }
 
 // =====================================================================
-// Batched LU for n <= 512: one workgroup of 512 threads (8 waves) per
-// block, thread t owning row j0 + t of the current NB-column panel in
-// registers.  Per pivot column: a wave argmax (first row of largest |a|,
-// as idamax) + one cross-wave step, the pivot and displaced rows broadcast
-// through LDS, and the rank-1 update in registers -- two barriers per
-// column.  U12 = L11^-1 A12 is a per-column register substitution (one
-// thread per column), and the trailing update A22 -= L21 U12 runs on MFMA
-// with both operands read from L2.
+// Batched LU for n <= 512: one workgroup (NT threads) per block, thread t
+// owning row j0 + t of the current NB-column panel in registers.  Pivoting
+// is implicit: rows stay in their threads and carry their LAPACK position;
+// per pivot column one wave argmax (DPP, first row of largest |a|, as
+// idamax), one LDS exchange of the waves' candidates and one barrier.
+// Panels are grouped in outer blocks of S: U = L11^-1 A12 and the rank-NB
+// update run on MFMA only inside the outer block; once its S panels are
+// factored, a block forward substitution forms U12 for the columns to the
+// right and the trailing matrix takes ONE update of inner dimension S*NB,
+// so the trailing matrix crosses HBM once per S panels.
 // =====================================================================
-template <int NB, int NT>
+template <int NB, int NT, int S>
 __global__ void __launch_bounds__(NT)
 k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
             int* __restrict__ ipiv0, int* __restrict__ perm0, int* __restrict__ info0) {
@@ -328,8 +330,8 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
   __shared__ int cidx[2][NW];
   __shared__ double crowv[2][NW][NB];
   __shared__ int pivs[NB];
-  __shared__ double L11[NB * (NB + 1)];   // L11[i*(NB+1) + k]
-  __shared__ double Xi[NB * (NB + 1)];    // L11^-1, same layout
+  __shared__ double L11s[S][NB * (NB + 1)];  // L11 of each panel of the outer block, [i*(NB+1) + k]
+  __shared__ double Xis[S][NB * (NB + 1)];   // their inverses, same layout
   __shared__ double Tt[NB * (NB + 1)];    // doubling scratch
   __shared__ int perm[NT];
   const int b = blockIdx.x;
@@ -341,6 +343,12 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
   for (int j0 = 0; j0 < n; j0 += NB) {
     const int nb = min(NB, n - j0);
     const int rows = n - j0;
+    // outer block [J0, cend) of S panels; this is its panel sp
+    const int sp = (j0 / NB) % S;
+    const int J0 = j0 - sp * NB;
+    const int cend = min(J0 + S * NB, n);
+    double* L11 = L11s[sp];
+    double* Xi = Xis[sp];
     const bool own = t < rows;
     double p[NB];
     if (own) {
@@ -486,7 +494,6 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
       }
       __syncthreads();
       for (int h = 1; h < NB; h <<= 1) {
-        // Tt[i][j] = sum_k C[i][k] A^-1[k][j], i in lower half, j in upper half
         for (int e = t; e < NB * h / 2; e += NT) {
           const int q = e / (h * h), r = e % (h * h);
           const int s0 = q * 2 * h, i = s0 + h + r / h, j = s0 + r % h;
@@ -504,10 +511,72 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
         }
         __syncthreads();
       }
-      // U12 = Xi * A12 on MFMA; each wave owns whole 16-column tiles, so it
-      // reads a tile's NB source rows before overwriting them.
-      const int o = j0 + nb;
-      const int nct = (m2 + 15) >> 4;
+    }
+
+    // A(rb.., cb..) -= A(rb.., kb..kb+kc) * A(kb..kb+kc, cb..) over nr x ncl,
+    // 32 x 32 blocks per wave, the inner dimension in 16-wide chunks.
+    auto update = [&](int rb, int nr, int cb, int ncl, int kb, int kc) {
+      const int tr = (nr + 31) >> 5, tc = (ncl + 31) >> 5;
+      for (int blk = wid; blk < tr * tc; blk += NW) {
+        const int r0 = (blk % tr) * 32, c0 = (blk / tr) * 32;
+        const bool full = (r0 + 32 <= nr) && (c0 + 32 <= ncl);
+        d4 c[2][2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int bb = 0; bb < 2; ++bb) {
+            const int col = c0 + bb * 16 + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int row = r0 + a * 16 + crow(lane, v);
+              c[a][bb][v] = (full || (row < nr && col < ncl))
+                                ? A[(int64_t)(rb + row) + (int64_t)(cb + col) * lda] : 0.0;
+            }
+          }
+        for (int k0 = 0; k0 < kc; k0 += 16) {
+          double af[4][2], bf[4][2];
+#pragma unroll
+          for (int kk = 0; kk < 4; ++kk) {
+            const int k = k0 + kk * 4 + (lane >> 4);
+#pragma unroll
+            for (int a = 0; a < 2; ++a) {
+              const int row = r0 + a * 16 + (lane & 15);
+              af[kk][a] = (k < kc && (full || row < nr)) ? A[(int64_t)(rb + row) + (int64_t)(kb + k) * lda] : 0.0;
+            }
+#pragma unroll
+            for (int bb = 0; bb < 2; ++bb) {
+              const int col = c0 + bb * 16 + (lane & 15);
+              bf[kk][bb] = (k < kc && (full || col < ncl)) ? A[(int64_t)(kb + k) + (int64_t)(cb + col) * lda] : 0.0;
+            }
+          }
+#pragma unroll
+          for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+              for (int bb = 0; bb < 2; ++bb) c[a][bb] = mfma(-af[kk][a], bf[kk][bb], c[a][bb]);
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int bb = 0; bb < 2; ++bb) {
+            const int col = c0 + bb * 16 + (lane & 15);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int row = r0 + a * 16 + crow(lane, v);
+              if (full || (row < nr && col < ncl))
+                A[(int64_t)(rb + row) + (int64_t)(cb + col) * lda] = c[a][bb][v];
+            }
+          }
+      }
+    };
+
+    const int o = j0 + nb;
+    // inside the outer block: U = L11^-1 A(j0.., o..cend), then the rows
+    // below take the rank-nb update on those columns only
+    if (o < cend) {
+      const int m2b = cend - o;
+      const int nct = (m2b + 15) >> 4;
       for (int ct = wid; ct < nct; ct += NW) {
         const int col = ct * 16 + (lane & 15);
         d4 u[NB / 16];
@@ -516,7 +585,7 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
 #pragma unroll
         for (int kk = 0; kk < NB / 4; ++kk) {
           const int k = kk * 4 + (lane >> 4);
-          const double bf = (k < nb && col < m2) ? A[(int64_t)(j0 + k) + (int64_t)(o + col) * lda] : 0.0;
+          const double bf = (k < nb && col < m2b) ? A[(int64_t)(j0 + k) + (int64_t)(o + col) * lda] : 0.0;
 #pragma unroll
           for (int q = 0; q < NB / 16; ++q)
             u[q] = mfma(Xi[(q * 16 + (lane & 15)) * (NB + 1) + k], bf, u[q]);
@@ -526,127 +595,67 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             const int r = q * 16 + crow(lane, v);
-            if (col < m2 && r < nb) A[(int64_t)(j0 + r) + (int64_t)(o + col) * lda] = u[q][v];
+            if (col < m2b && r < nb) A[(int64_t)(j0 + r) + (int64_t)(o + col) * lda] = u[q][v];
           }
       }
-    }
-    __syncthreads();
-
-    // trailing update A22 -= L21 * U12, 32x32 blocks per wave
-    if (m2 > 0) {
-      const int o = j0 + nb;
-      const int t32 = (m2 + 31) >> 5;
-      // software-pipelined over this wave's blocks: the next block's C,
-      // L21 and U12 operands are in flight while the current block's
-      // MFMAs run; partial blocks (only at the matrix edge) take the masked
-      // loads.
-      const int nblk = t32 * t32;
-      auto load_blk = [&](int blk, d4 (&c)[2][2], double (&af)[NB / 4][2], double (&bf)[NB / 4][2]) {
-        const int r0 = (blk % t32) * 32, c0 = (blk / t32) * 32;
-        const bool full = (nb == NB) && (r0 + 32 <= m2) && (c0 + 32 <= m2);
-        if (full) {
-          const double* Cp = A + (int64_t)(o + r0) + (int64_t)(o + c0) * lda;
-          const double* Lp = A + (int64_t)(o + r0) + (int64_t)j0 * lda;
-          const double* Up = A + (int64_t)j0 + (int64_t)(o + c0) * lda;
+      __syncthreads();
+      update(o, n - o, o, m2b, j0, nb);
+    } else if (cend < n) {
+      // outer block complete (all its panels full): block forward
+      // substitution U_s = L_ss^-1 (A_s - sum_{q<s} L_sq U_q) on the columns
+      // right of it, then the trailing update with inner dimension cend - J0
+      const int nsp = (cend - J0) / NB;
+      const int ntr = n - cend;
+      const int nct = (ntr + 15) >> 4;
+      for (int ct = wid; ct < nct; ct += NW) {
+        const int col = ct * 16 + (lane & 15);
+        const bool cok = col < ntr;
+        d4 u[S][NB / 16];
 #pragma unroll
-          for (int a = 0; a < 2; ++a)
+        for (int s2 = 0; s2 < S; ++s2) {
+          if (s2 < nsp) {
+            const int r0 = J0 + s2 * NB;
+            d4 acc[NB / 16];
 #pragma unroll
-            for (int bb = 0; bb < 2; ++bb)
+            for (int a = 0; a < NB / 16; ++a)
 #pragma unroll
               for (int v = 0; v < 4; ++v)
-                c[a][bb][v] = Cp[(a * 16 + crow(lane, v)) + (int64_t)(bb * 16 + (lane & 15)) * lda];
+                acc[a][v] = cok ? A[(int64_t)(r0 + a * 16 + crow(lane, v)) + (int64_t)(cend + col) * lda] : 0.0;
 #pragma unroll
-          for (int kk = 0; kk < NB / 4; ++kk) {
-            const int k = kk * 4 + (lane >> 4);
+            for (int q = 0; q < S; ++q) {
+              if (q < s2) {
 #pragma unroll
-            for (int a = 0; a < 2; ++a) af[kk][a] = Lp[(a * 16 + (lane & 15)) + (int64_t)k * lda];
+                for (int kk = 0; kk < NB / 4; ++kk) {
+                  const int k = kk * 4 + (lane >> 4);
+                  const double bf = u[q][kk >> 2][kk & 3];
 #pragma unroll
-            for (int bb = 0; bb < 2; ++bb) bf[kk][bb] = Up[k + (int64_t)(bb * 16 + (lane & 15)) * lda];
-          }
-        } else {
-#pragma unroll
-          for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int bb = 0; bb < 2; ++bb) {
-              const int col = c0 + bb * 16 + (lane & 15);
-#pragma unroll
-              for (int v = 0; v < 4; ++v) {
-                const int row = r0 + a * 16 + crow(lane, v);
-                c[a][bb][v] = (row < m2 && col < m2) ? A[(int64_t)(o + row) + (int64_t)(o + col) * lda] : 0.0;
+                  for (int a = 0; a < NB / 16; ++a) {
+                    const double af = A[(int64_t)(r0 + a * 16 + (lane & 15)) + (int64_t)(J0 + q * NB + k) * lda];
+                    acc[a] = mfma(-af, bf, acc[a]);
+                  }
+                }
               }
             }
 #pragma unroll
-          for (int kk = 0; kk < NB / 4; ++kk) {
-            const int k = kk * 4 + (lane >> 4);
+            for (int a = 0; a < NB / 16; ++a) u[s2][a] = d4zero();
 #pragma unroll
-            for (int a = 0; a < 2; ++a) {
-              const int row = r0 + a * 16 + (lane & 15);
-              af[kk][a] = (k < nb && row < m2) ? A[(int64_t)(o + row) + (int64_t)(j0 + k) * lda] : 0.0;
+            for (int kk = 0; kk < NB / 4; ++kk) {
+              const int k = kk * 4 + (lane >> 4);
+              const double bf = acc[kk >> 2][kk & 3];
+#pragma unroll
+              for (int a = 0; a < NB / 16; ++a)
+                u[s2][a] = mfma(Xis[s2][(a * 16 + (lane & 15)) * (NB + 1) + k], bf, u[s2][a]);
             }
 #pragma unroll
-            for (int bb = 0; bb < 2; ++bb) {
-              const int col = c0 + bb * 16 + (lane & 15);
-              bf[kk][bb] = (k < nb && col < m2) ? A[(int64_t)(j0 + k) + (int64_t)(o + col) * lda] : 0.0;
-            }
-          }
-        }
-      };
-      auto store_blk = [&](int blk, const d4 (&c)[2][2]) {
-        const int r0 = (blk % t32) * 32, c0 = (blk / t32) * 32;
+            for (int a = 0; a < NB / 16; ++a)
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-          for (int bb = 0; bb < 2; ++bb) {
-            const int col = c0 + bb * 16 + (lane & 15);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-              const int row = r0 + a * 16 + crow(lane, v);
-              if (row < m2 && col < m2) A[(int64_t)(o + row) + (int64_t)(o + col) * lda] = c[a][bb][v];
-            }
-          }
-      };
-      if (NT > 256) {
-        // 2 waves per SIMD: no register room for a second operand set
-        for (int blk = wid; blk < nblk; blk += NW) {
-          d4 c[2][2];
-          double af[NB / 4][2], bf[NB / 4][2];
-          load_blk(blk, c, af, bf);
-#pragma unroll
-          for (int kk = 0; kk < NB / 4; ++kk)
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-              for (int bb = 0; bb < 2; ++bb) c[a][bb] = mfma(-af[kk][a], bf[kk][bb], c[a][bb]);
-          store_blk(blk, c);
-        }
-      } else if (wid < nblk) {
-        d4 c[2][2];
-        double af[NB / 4][2], bf[NB / 4][2];
-        load_blk(wid, c, af, bf);
-        for (int blk = wid; blk < nblk; blk += NW) {
-          d4 c2[2][2];
-          double af2[NB / 4][2], bf2[NB / 4][2];
-          const bool more = blk + NW < nblk;
-          if (more) load_blk(blk + NW, c2, af2, bf2);
-#pragma unroll
-          for (int kk = 0; kk < NB / 4; ++kk)
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-              for (int bb = 0; bb < 2; ++bb) c[a][bb] = mfma(-af[kk][a], bf[kk][bb], c[a][bb]);
-          store_blk(blk, c);
-          if (more) {
-#pragma unroll
-            for (int a = 0; a < 2; ++a)
-#pragma unroll
-              for (int bb = 0; bb < 2; ++bb) c[a][bb] = c2[a][bb];
-#pragma unroll
-            for (int kk = 0; kk < NB / 4; ++kk)
-#pragma unroll
-              for (int a = 0; a < 2; ++a) { af[kk][a] = af2[kk][a]; bf[kk][a] = bf2[kk][a]; }
+              for (int v = 0; v < 4; ++v)
+                if (cok) A[(int64_t)(r0 + a * 16 + crow(lane, v)) + (int64_t)(cend + col) * lda] = u[s2][a][v];
           }
         }
       }
+      __syncthreads();
+      update(cend, ntr, cend, ntr, J0, cend - J0);
     }
     __syncthreads();
   }
@@ -929,12 +938,12 @@ hipError_t launch_getrf(int n, double* A, int64_t lda, int64_t bstride, int batc
     return hipGetLastError();
   }
   if (n <= 256) {
-    hipLaunchKernelGGL((k_getrf_reg<GETRF_NB, 256>), dim3(batch), dim3(256), 0, st, A, lda, bstride,
+    hipLaunchKernelGGL((k_getrf_reg<GETRF_NB, 256, 2>), dim3(batch), dim3(256), 0, st, A, lda, bstride,
                        n, ipiv, perm, info);
     return hipGetLastError();
   }
   if (n <= 512) {
-    hipLaunchKernelGGL((k_getrf_reg<GETRF_NB, 512>), dim3(batch), dim3(512), 0, st, A, lda, bstride,
+    hipLaunchKernelGGL((k_getrf_reg<GETRF_NB, 512, 2>), dim3(batch), dim3(512), 0, st, A, lda, bstride,
                        n, ipiv, perm, info);
     return hipGetLastError();
   }
