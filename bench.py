@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0       # MI355X HBM3E
+LD_PAD = 8                  # leading dimension N + 8 for every N-row operand
 
 
 def log(*a):
@@ -113,9 +114,11 @@ def main():
     W, K_ = args.warmup, args.steps
 
     log(f"workload: N={N} leaf={m} levels={L} rank={r} nrhs={s} FP64; steps={K_} warmup={W}")
-    K0, U, V, B0 = F.workload_gaussian(N, m, r, s, seed=0)
-    Ks = [K0.clone() for _ in range(W + K_)]
-    Bs = [B0.clone() for _ in range(W + K_)]
+    # operands with leading dimension N + 8: a power-of-two column stride
+    # makes a leaf's columns alias in the HBM channel interleave
+    K0, U, V, B0 = F.workload_gaussian(N, m, r, s, seed=0, pad=LD_PAD)
+    Ks = [F.fortran_clone(K0) for _ in range(W + K_)]
+    Bs = [F.fortran_clone(B0) for _ in range(W + K_)]
     plan = F.HodlrPlan(N, m, L, r, nrhs_max=s)
     log(f"plan device memory: {plan.device_bytes / 2**30:.2f} GiB")
     torch.cuda.synchronize()
@@ -151,7 +154,7 @@ def main():
     log(f"relative residual max|b - A x| / max|b| = {rel_res:.3e}")
 
     # factor-reuse solve rate (same factorization, fresh right-hand sides)
-    Bsol = [B0.clone() for _ in range(K_)]
+    Bsol = [F.fortran_clone(B0) for _ in range(K_)]
     torch.cuda.synchronize()
     ev0.record()
     for i in range(K_):
@@ -215,10 +218,10 @@ def main():
     if not args.no_adjoint:
         aN, am, ar, as_ = 65536, 256, 32, 16
         aL = int(round(math.log2(aN // am)))
-        aK0, aU, aV, aB0 = F.workload_gaussian(aN, am, ar, as_, seed=0)
+        aK0, aU, aV, aB0 = F.workload_gaussian(aN, am, ar, as_, seed=0, pad=LD_PAD)
         aplan = F.HodlrPlan(aN, am, aL, ar, nrhs_max=as_)
-        aK = aK0.clone()
-        aX = aB0.clone()
+        aK = F.fortran_clone(aK0)
+        aX = F.fortran_clone(aB0)
         Xstar = F.fortran(torch.randn(as_, aN, dtype=torch.float64, device="cuda").t())
         fw = []
         for i in range(W + K_):

@@ -21,7 +21,8 @@ import torch
 __all__ = [
     "lib", "lib_path", "available", "fortran_empty", "fortran", "HodlrPlan", "hsolve",
     "gemm_red", "gemm_bro", "node_solve", "matvec", "generate", "workload_gaussian",
-    "dgesv_batched", "HodlrError", "grad", "HodlrSolveFunction", "hodlr_solve",
+    "dgesv_batched", "HodlrError", "grad", "HodlrSolveFunction", "hodlr_solve", "fortran_clone",
+    "FP64", "MIXED_IR", "IRStats",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -39,6 +40,21 @@ class HodlrError(RuntimeError):
         msg = lib().hodlr_strerror(status).decode()
         detail = lib().hodlr_last_error_detail().decode()
         super().__init__(f"{where}: status {status} ({msg}){' - ' + detail if detail else ''}")
+
+
+class IRStats(C.Structure):
+    """hodlr_ir_stats: refinement outcome (include/hodlr.h)."""
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("residual", C.c_double),
+                ("history", C.c_double * 33)]
+
+    def as_dict(self):
+        return {"iterations": self.iterations, "converged": bool(self.converged),
+                "residual": self.residual,
+                "history": [self.history[i] for i in range(self.iterations + 1)]}
+
+
+FP64 = 0
+MIXED_IR = 1
 
 
 class _Desc(C.Structure):
@@ -68,6 +84,8 @@ def lib():
         "hodlr_factor_solve": ([vp, vp, i64, vp, i64, vp, i64, vp, i64, ci, vp], ci),
         "hodlr_plan_status": ([vp, vp, C.POINTER(ci)], ci),
         "hodlr_solve_adjoint": ([vp, vp, i64, ci, vp], ci),
+        "hodlr_solve_ir": ([vp, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64, ci, dbl, ci,
+                            C.POINTER(IRStats), vp], ci),
         "hodlr_grad": ([i64, ci, ci, ci, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64,
                         vp, i64, vp], ci),
         "hodlr_matvec": ([i64, ci, ci, ci, vp, i64, vp, i64, vp, i64, vp, i64, dbl, dbl, vp, i64,
@@ -82,6 +100,7 @@ def lib():
                            ci),
         "hodlr_shift_leaf_diagonal": ([i64, ci, dbl, vp, i64, vp], ci),
         "hodlr_selftest_mfma": ([vp, vp], ci),
+        "hodlr_selftest_mfma_f32": ([vp, vp], ci),
         "hodlr_dense_blocks": ([i64, ci, ci, vp, i64, vp, i64, vp, dbl, vp, i64, vp], ci),
         "hodlr_profile_enable": ([ci], ci),
         "hodlr_profile_reset": ([], ci),
@@ -102,9 +121,23 @@ def available() -> bool:
 
 
 # ---------------------------------------------------------------- tensors
-def fortran_empty(rows, cols, device="cuda", dtype=torch.float64):
-    """Column-major rows x cols tensor (strides (1, rows))."""
-    return torch.empty((cols, rows), dtype=dtype, device=device).t()
+def fortran_empty(rows, cols, device="cuda", dtype=torch.float64, pad=0):
+    """Column-major rows x cols tensor with leading dimension rows + pad
+    (strides (1, rows + pad)).  A small pad keeps the column stride off a
+    power of two, so that a leaf's columns do not alias in the HBM channel
+    interleave."""
+    if pad <= 0:
+        return torch.empty((cols, rows), dtype=dtype, device=device).t()
+    return torch.empty((cols, rows + pad), dtype=dtype, device=device).t()[:rows]
+
+
+def fortran_clone(t: torch.Tensor) -> torch.Tensor:
+    """Copy of a column-major operand with the same leading dimension."""
+    ld = _ld(t)
+    out = fortran_empty(t.shape[0], t.shape[1], device=t.device, dtype=t.dtype,
+                        pad=ld - t.shape[0])
+    out.copy_(t)
+    return out
 
 
 def fortran(t: torch.Tensor) -> torch.Tensor:
@@ -151,13 +184,14 @@ class HodlrPlan:
     """Device factorization of one HODLR matrix (the working form of the
     reference's HMatrix, include/hmatrix.hpp:8-34)."""
 
-    def __init__(self, n, leaf, levels, rank, nrhs_max=1, device=None):
+    def __init__(self, n, leaf, levels, rank, nrhs_max=1, device=None, precision=FP64):
         if not torch.cuda.is_available():
             raise RuntimeError("HodlrPlan needs a GPU")
         dev = torch.cuda.current_device() if device is None else int(device)
         self.n, self.leaf, self.levels, self.rank, self.nrhs_max = n, leaf, levels, rank, nrhs_max
         self.device = dev
-        d = _Desc(n, leaf, levels, rank, nrhs_max, 0, dev, 0)
+        self.precision = precision
+        d = _Desc(n, leaf, levels, rank, nrhs_max, precision, dev, 0)
         h = C.c_void_p()
         _call(lib().hodlr_plan_create, C.byref(h), C.byref(d), where="hodlr_plan_create")
         self._h = h
@@ -208,6 +242,20 @@ class HodlrPlan:
         _call(lib().hodlr_solve, self._h, _ptr(B2), _ld(B2), B2.shape[1], _stream(),
               where="hodlr_solve")
         return B
+
+    def solve_ir(self, K, U, V, B, X=None, tol=1e-12, maxit=10):
+        """Iteratively refined solve X = A^-1 B against the ORIGINAL K, U, V
+        (FP64 HODLR-matvec residuals); returns (X, stats dict)."""
+        for t, nm in ((K, "K"), (U, "U"), (V, "V"), (B, "B")):
+            _check(t, nm)
+        B2 = B if B.dim() == 2 else B.view(-1, 1)
+        if X is None:
+            X = fortran_empty(B2.shape[0], B2.shape[1], device=B.device)
+        st = IRStats()
+        _call(lib().hodlr_solve_ir, self._h, _ptr(K), _ld(K), _ptr(U), _ld(U), _ptr(V), _ld(V),
+              _ptr(B2), _ld(B2), _ptr(X), _ld(X), B2.shape[1], float(tol), int(maxit),
+              C.byref(st), _stream(), where="hodlr_solve_ir")
+        return X, st.as_dict()
 
     def solve_adjoint(self, G):
         """G <- A^-T G with the stored factorization."""
@@ -358,28 +406,31 @@ def dgesv_batched(A, B, n, batch):
     return st, ipiv
 
 
-def generate(seed, stream_id, system, kind, rows, cols, scale=1.0, out=None, device="cuda"):
+def generate(seed, stream_id, system, kind, rows, cols, scale=1.0, out=None, device="cuda",
+             pad=0):
     """Philox4x64-10 generator on the device (kind 0 uniform, 1 normal)."""
     if out is None:
-        out = fortran_empty(rows, cols, device=device)
+        out = fortran_empty(rows, cols, device=device, pad=pad)
     _call(lib().hodlr_generate, seed, stream_id, system, kind, rows, cols, scale, _ptr(out),
           _ld(out), _stream(), where="hodlr_generate")
     return out
 
 
 def workload_gaussian(N, leaf, r, s, seed=0, system=0, shift=None, rhs="uniform",
-                      device="cuda"):
+                      device="cuda", pad=0):
     """Seeded Gaussian HODLR workload generated on the device:
     K_i = G_i + shift*I (G iid N(0,1)), U, V iid N(0,1)/sqrt(r), B iid U[0,1)
-    (rhs='uniform') or N(0,1) (rhs='normal').  Streams 0..3 = K, U, V, B."""
+    (rhs='uniform') or N(0,1) (rhs='normal').  Streams 0..3 = K, U, V, B.
+    The values do not depend on `pad` (the leading dimension is N + pad)."""
     shift = float(leaf) if shift is None else float(shift)
-    K = generate(seed, 0, system, 1, N, leaf, 1.0, device=device)
+    K = generate(seed, 0, system, 1, N, leaf, 1.0, device=device, pad=pad)
     _call(lib().hodlr_shift_leaf_diagonal, N, leaf, shift, _ptr(K), _ld(K), _stream(),
           where="hodlr_shift_leaf_diagonal")
     sc = 1.0 / math.sqrt(float(r))
-    U = generate(seed, 1, system, 1, N, r, sc, device=device)
-    V = generate(seed, 2, system, 1, N, r, sc, device=device)
-    B = generate(seed, 3, system, 0 if rhs == "uniform" else 1, N, s, 1.0, device=device)
+    U = generate(seed, 1, system, 1, N, r, sc, device=device, pad=pad)
+    V = generate(seed, 2, system, 1, N, r, sc, device=device, pad=pad)
+    B = generate(seed, 3, system, 0 if rhs == "uniform" else 1, N, s, 1.0, device=device,
+                 pad=pad)
     return K, U, V, B
 
 

@@ -213,6 +213,16 @@ struct hodlr_plan_s {
   double* G = nullptr;
   double* R = nullptr;
   int ncmax = 0;
+  // mixed precision: FP32 copy of the leaf blocks (factored in place) and
+  // its diagonal-block inverses; K itself is left untouched
+  bool mixed = false;
+  float* K32 = nullptr;
+  int64_t ldk32 = 0;
+  float* leaf_dinv32 = nullptr;
+  // refinement buffers (allocated on first use): residual block, matvec
+  // workspace, column norms
+  double* ir_buf = nullptr;
+  int64_t ir_elems = 0;
   // factored operands
   const double* K = nullptr;
   const double* U = nullptr;
@@ -244,6 +254,25 @@ int palloc(hodlr_plan p, T** out, int64_t count) {
 void plan_free(hodlr_plan p) {
   for (void* a : p->allocs) (void)hipFree(a);
   p->allocs.clear();
+  if (p->ir_buf) (void)hipFree(p->ir_buf);
+  p->ir_buf = nullptr;
+  p->ir_elems = 0;
+}
+
+// Leaf solve with the stored leaf factorization (FP64 or the FP32 copy).
+int leaf_solve(hodlr_plan p, const ColSeg& x, int ncols, bool trans, hipStream_t st) {
+  const int m = p->m;
+  if (p->mixed) {
+    HIP_TRY(hodlr::launch_getrs_f32(m, p->K32, p->ldk32, m, (int)p->P, p->leaf_perm,
+                                    p->leaf_dinv32, x, m, ncols, trans, st));
+  } else if (trans) {
+    HIP_TRY(hodlr::launch_getrs_t(m, p->K, p->ldk, m, (int)p->P, p->leaf_perm, p->leaf_dinv, x, m,
+                                  ncols, st));
+  } else {
+    HIP_TRY(hodlr::launch_getrs(m, p->K, p->ldk, m, (int)p->P, p->leaf_perm, p->leaf_dinv, x, m,
+                                ncols, st));
+  }
+  return HODLR_OK;
 }
 
 CoreBufs core_level(hodlr_plan p, int t) {
@@ -264,13 +293,24 @@ int check_ld(int64_t ld, int64_t n) { return ld >= n ? HODLR_OK : HODLR_ERR_ARG;
 int leaf_phase(hodlr_plan p, double* K, int64_t ldk, const double* U, int64_t ldu, double* B,
                int64_t ldb, int nrhs, hipStream_t st) {
   const int m = p->m;
-  const double kb = 8.0 * (double)p->N * m;
-  {
-    ProfScope ps(HODLR_PROF_GETRF_LEAF, st, p->P * lu_flops(m), 2.0 * kb);
-    HIP_TRY(hodlr::launch_getrf(m, K, ldk, m, (int)p->P, p->leaf_ipiv, p->leaf_perm, p->leaf_info,
-                                st));
-  }
-  {
+  const double kb = (p->mixed ? 4.0 : 8.0) * (double)p->N * m;
+  if (p->mixed) {
+    // FP32 copy of the leaf blocks, factored in place; K stays FP64 for the
+    // refinement residual
+    HIP_TRY(hodlr::launch_convert_f32(p->N, m, K, ldk, p->K32, p->ldk32, st));
+    {
+      ProfScope ps(HODLR_PROF_GETRF_LEAF, st, p->P * lu_flops(m), 2.0 * kb);
+      HIP_TRY(hodlr::launch_getrf_f32(m, p->K32, p->ldk32, m, (int)p->P, p->leaf_ipiv,
+                                      p->leaf_perm, p->leaf_info, st));
+    }
+    ProfScope ps(HODLR_PROF_TRTRI, st, 0.0, kb);
+    HIP_TRY(hodlr::launch_trtri_diag_f32(m, p->K32, p->ldk32, m, (int)p->P, p->leaf_dinv32, st));
+  } else {
+    {
+      ProfScope ps(HODLR_PROF_GETRF_LEAF, st, p->P * lu_flops(m), 2.0 * kb);
+      HIP_TRY(hodlr::launch_getrf(m, K, ldk, m, (int)p->P, p->leaf_ipiv, p->leaf_perm,
+                                  p->leaf_info, st));
+    }
     ProfScope ps(HODLR_PROF_TRTRI, st, 0.0, kb);
     HIP_TRY(hodlr::launch_trtri_diag(m, K, ldk, m, (int)p->P, p->leaf_dinv, st));
   }
@@ -287,8 +327,7 @@ int leaf_phase(hodlr_plan p, double* K, int64_t ldk, const double* U, int64_t ld
   if (ncols > 0) {
     ProfScope ps(HODLR_PROF_GETRS_LEAF, st, 2.0 * (double)p->P * m * m * ncols,
                  kb + 16.0 * (double)p->N * ncols);
-    HIP_TRY(hodlr::launch_getrs(m, K, ldk, m, (int)p->P, p->leaf_perm, p->leaf_dinv, x, m, ncols,
-                                st));
+    return leaf_solve(p, x, ncols, false, st);
   }
   return HODLR_OK;
 }
@@ -455,7 +494,8 @@ int hodlr_plan_create(hodlr_plan* plan, const hodlr_desc* desc) {
   if (d.n != ((int64_t)d.leaf << d.levels)) return HODLR_ERR_SHAPE;
   if (d.levels > 0 && d.leaf <= d.rank) return HODLR_ERR_SHAPE;  // matrix.cc:225
   if (d.leaf > hodlr::kMaxBlock || 2 * d.rank > hodlr::kMaxBlock) return HODLR_ERR_UNSUPPORTED;
-  if (d.precision != HODLR_FP64) return HODLR_ERR_UNSUPPORTED;
+  if (d.precision != HODLR_FP64 && d.precision != HODLR_MIXED_IR) return HODLR_ERR_ARG;
+  if (d.precision == HODLR_MIXED_IR && d.leaf > 512) return HODLR_ERR_UNSUPPORTED;
   DeviceGuard g(d.device);
   if (!g.ok) return HODLR_ERR_ARG;
   hodlr_plan p = new hodlr_plan_s();
@@ -467,7 +507,14 @@ int hodlr_plan_create(hodlr_plan* plan, const hodlr_desc* desc) {
   rc = rc ? rc : palloc(p, &p->leaf_ipiv, p->P * p->m);
   rc = rc ? rc : palloc(p, &p->leaf_perm, p->P * p->m);
   rc = rc ? rc : palloc(p, &p->leaf_info, p->P);
-  rc = rc ? rc : palloc(p, &p->leaf_dinv, p->P * hodlr::dinv_elems(p->m));
+  p->mixed = d.precision == HODLR_MIXED_IR;
+  if (p->mixed) {
+    p->ldk32 = p->N + 32;  // keeps the column stride off a power of two
+    rc = rc ? rc : palloc(p, &p->K32, p->ldk32 * p->m);
+    rc = rc ? rc : palloc(p, &p->leaf_dinv32, p->P * hodlr::dinv_elems(p->m));
+  } else {
+    rc = rc ? rc : palloc(p, &p->leaf_dinv, p->P * hodlr::dinv_elems(p->m));
+  }
   rc = rc ? rc : palloc(p, &p->Y, p->L > 0 ? p->N * r : 0);
   rc = rc ? rc : palloc(p, &p->W, (int64_t)p->L * p->P * r * r);
   rc = rc ? rc : palloc(p, &p->Q, p->L > 0 ? p->P * r * r : 0);
@@ -544,14 +591,14 @@ int hodlr_solve(hodlr_plan p, double* B, int64_t ldb, int nrhs, hipStream_t st) 
   if (!p->factored) return HODLR_ERR_STATE;
   if (check_ld(ldb, p->N)) return HODLR_ERR_ARG;
   DeviceGuard g(p->d.device);
-  const int r = p->r, m = p->m, L = p->L;
+  const int r = p->r, L = p->L;
   const int64_t P = p->P;
   for (int c0 = 0; c0 < nrhs; c0 += p->ncmax) {
     const int nc = std::min(p->ncmax, nrhs - c0);
     double* Bc = B + (int64_t)c0 * ldb;
     ColSeg x = hodlr::seg_single(Bc, ldb);
-    HIP_TRY(hodlr::launch_getrs(m, p->K, p->ldk, m, (int)P, p->leaf_perm, p->leaf_dinv, x, m,
-                                nc, st));
+    int rc0 = leaf_solve(p, x, nc, false, st);
+    if (rc0) return rc0;
     if (L == 0) continue;
     // p_i = V_i^T B_i; per level the current projection is p_i - Q_i G_i
     int rc = leaf_projections(p, p->V, p->ldv, x, nc, p->Pb[0], st);
@@ -600,7 +647,7 @@ int hodlr_solve_adjoint(hodlr_plan p, double* G, int64_t ldg, int nrhs, hipStrea
   if (!p->factored) return HODLR_ERR_STATE;
   if (check_ld(ldg, p->N)) return HODLR_ERR_ARG;
   DeviceGuard g(p->d.device);
-  const int r = p->r, m = p->m, L = p->L;
+  const int r = p->r, L = p->L;
   const int64_t P = p->P;
   for (int c0 = 0; c0 < nrhs; c0 += p->ncmax) {
     const int nc = std::min(p->ncmax, nrhs - c0);
@@ -651,8 +698,80 @@ int hodlr_solve_adjoint(hodlr_plan p, double* G, int64_t ldg, int nrhs, hipStrea
       rc = final_update(p, p->V, p->ldv, p->G, nc, Gc, ldg, st);
       if (rc) return rc;
     }
-    HIP_TRY(hodlr::launch_getrs_t(m, p->K, p->ldk, m, (int)P, p->leaf_perm, p->leaf_dinv, x, m,
-                                  nc, st));
+    int rc1 = leaf_solve(p, x, nc, true, st);
+    if (rc1) return rc1;
+  }
+  return HODLR_OK;
+}
+
+int hodlr_solve_ir(hodlr_plan p, const double* K, int64_t ldk, const double* U, int64_t ldu,
+                   const double* V, int64_t ldv, const double* B, int64_t ldb, double* X,
+                   int64_t ldx, int nrhs, double tol, int maxit, hodlr_ir_stats* stats,
+                   hipStream_t st) {
+  if (!p || !K || !U || !V || !B || !X || nrhs < 0 || maxit < 0 || maxit > 32 || !(tol >= 0.0))
+    return HODLR_ERR_ARG;
+  if (!p->factored) return HODLR_ERR_STATE;
+  const int64_t N = p->N;
+  if (check_ld(ldk, N) || check_ld(ldu, N) || check_ld(ldv, N) || check_ld(ldb, N) ||
+      check_ld(ldx, N))
+    return HODLR_ERR_ARG;
+  DeviceGuard g(p->d.device);
+  if (stats) memset(stats, 0, sizeof(*stats));
+  if (nrhs == 0) {
+    if (stats) stats->converged = 1;
+    return HODLR_OK;
+  }
+  // residual block (N x nrhs), matvec workspace, norms of B and R
+  const int64_t mvw = hodlr::matvec_work_elems(N, p->m, p->r, nrhs);
+  const int64_t need = N * nrhs + mvw + 2 * (int64_t)nrhs;
+  if (p->ir_elems < need) {
+    HIP_TRY(hipStreamSynchronize(st));
+    if (p->ir_buf) HIP_TRY(hipFree(p->ir_buf));
+    p->ir_buf = nullptr;
+    p->ir_elems = 0;
+    if (hipMalloc(&p->ir_buf, sizeof(double) * (size_t)need) != hipSuccess) {
+      snprintf(g_detail, sizeof g_detail, "refinement buffer of %lld doubles", (long long)need);
+      return HODLR_ERR_NOMEM;
+    }
+    p->ir_elems = need;
+  }
+  double* R = p->ir_buf;
+  double* work = R + N * nrhs;
+  double* nb = work + mvw;
+  double* nr = nb + nrhs;
+  std::vector<double> hb(nrhs), hr(nrhs);
+  HIP_TRY(hodlr::launch_colmax(N, nrhs, B, ldb, nb, st));
+  HIP_TRY(hipMemcpyAsync(hb.data(), nb, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+  // X = solve(B)
+  HIP_TRY(hodlr::launch_copy_cols(N, nrhs, B, ldb, X, ldx, st));
+  int rc = hodlr_solve(p, X, ldx, nrhs, st);
+  if (rc) return rc;
+  for (int it = 0;; ++it) {
+    // R = B - A X
+    HIP_TRY(hodlr::launch_copy_cols(N, nrhs, B, ldb, R, N, st));
+    HIP_TRY(hodlr::launch_matvec(N, p->m, p->r, nrhs, K, ldk, U, ldu, V, ldv, X, ldx, -1.0, 1.0, R,
+                                 N, work, st));
+    HIP_TRY(hodlr::launch_colmax(N, nrhs, R, N, nr, st));
+    HIP_TRY(hipMemcpyAsync(hr.data(), nr, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    double res = 0.0;
+    for (int j = 0; j < nrhs; ++j) {
+      const double q = hb[j] > 0.0 ? hr[j] / hb[j] : hr[j];
+      res = (q > res || q != q) ? q : res;
+    }
+    if (stats) {
+      stats->history[it] = res;
+      stats->residual = res;
+      stats->iterations = it;
+    }
+    if (res <= tol) {
+      if (stats) stats->converged = 1;
+      break;
+    }
+    if (it == maxit || res != res) break;
+    rc = hodlr_solve(p, R, N, nrhs, st);
+    if (rc) return rc;
+    HIP_TRY(hodlr::launch_add_cols(N, nrhs, R, N, X, ldx, st));
   }
   return HODLR_OK;
 }
@@ -948,6 +1067,12 @@ int hodlr_generate(uint64_t seed, uint32_t stream_id, uint32_t system, int kind,
 int hodlr_selftest_mfma(double* out, hipStream_t st) {
   if (!out) return HODLR_ERR_ARG;
   HIP_TRY(hodlr::launch_selftest_mfma(out, st));
+  return HODLR_OK;
+}
+
+int hodlr_selftest_mfma_f32(double* out, hipStream_t st) {
+  if (!out) return HODLR_ERR_ARG;
+  HIP_TRY(hodlr::launch_selftest_mfma_f32(out, st));
   return HODLR_OK;
 }
 

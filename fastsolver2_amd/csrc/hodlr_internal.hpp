@@ -44,6 +44,21 @@ __host__ __device__ inline int64_t dinv_elems(int n) { return (int64_t)ntri(n) *
 hipError_t launch_getrf(int n, double* A, int64_t lda, int64_t bstride, int batch,
                         int* ipiv, int* perm, int* info, hipStream_t st);
 // Inverts the 32x32 diagonal blocks of L (unit) and U of each LU.
+// FP32 leaf factorization / solve for the mixed-precision path (n <= 512);
+// the solve reads and writes FP64 right-hand sides.
+hipError_t launch_getrf_f32(int n, float* A, int64_t lda, int64_t bstride, int batch, int* ipiv,
+                            int* perm, int* info, hipStream_t st);
+hipError_t launch_trtri_diag_f32(int n, const float* LU, int64_t lda, int64_t bstride, int batch,
+                                 float* dinv, hipStream_t st);
+hipError_t launch_getrs_f32(int n, const float* LU, int64_t lda, int64_t bstride, int batch,
+                            const int* perm, const float* dinv, const ColSeg& X, int64_t xbstride,
+                            int ncols, bool trans, hipStream_t st);
+hipError_t launch_convert_f32(int64_t rows, int cols, const double* src, int64_t lds, float* dst,
+                              int64_t ldd, hipStream_t st);
+hipError_t launch_colmax(int64_t rows, int cols, const double* A, int64_t lda, double* out,
+                         hipStream_t st);
+hipError_t launch_add_cols(int64_t rows, int cols, const double* X, int64_t ldx, double* Y,
+                           int64_t ldy, hipStream_t st);
 hipError_t launch_trtri_diag(int n, const double* LU, int64_t lda, int64_t bstride,
                              int batch, double* dinv, hipStream_t st);
 // X_b <- U^-1 L^-1 P X_b for batch item b, columns [0, ncols) of X, where
@@ -141,6 +156,7 @@ struct BGemmArgs {
 hipError_t launch_bgemm(const BGemmArgs& g, hipStream_t st);
 hipError_t launch_eye(int batch, int r, double* W, hipStream_t st);
 hipError_t launch_selftest_mfma(double* out, hipStream_t st);
+hipError_t launch_selftest_mfma_f32(double* out, hipStream_t st);
 hipError_t launch_copy_cols(int64_t rows, int cols, const double* src, int64_t lds,
                             double* dst, int64_t ldd, hipStream_t st);
 

@@ -68,6 +68,38 @@ __device__ __forceinline__ d4 d4zero() {
   return z;
 }
 
+// Element-type traits of the LU kernels: FP64 (the reference's precision)
+// and FP32 (the mixed-precision leaf factorization).  The FP32 matrix core
+// (v_mfma_f32_16x16x4f32) lays its accumulator out as row 4*(lane/16) + v,
+// the FP64 one as row lane/16 + 4v.
+typedef float f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ f4 mfma(float a, float b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+template <typename T> struct Elem;
+template <> struct Elem<double> {
+  typedef d4 V4;
+  __device__ static int crow(int lane, int v) { return (lane >> 4) + 4 * v; }
+  __device__ static double sfmin() { return DBL_MIN; }
+  // bits(|a|) + 1 orders non-negative values; NaN -> 1 (counts as zero)
+  __device__ static unsigned long long key(double a) {
+    return (a == a) ? (unsigned long long)__double_as_longlong(fabs(a)) + 1ULL : 1ULL;
+  }
+};
+template <> struct Elem<float> {
+  typedef f4 V4;
+  __device__ static int crow(int lane, int v) { return 4 * (lane >> 4) + v; }
+  __device__ static float sfmin() { return FLT_MIN; }
+  __device__ static unsigned long long key(float a) {
+    return (a == a) ? (unsigned long long)__float_as_uint(fabsf(a)) + 1ULL : 1ULL;
+  }
+};
+template <typename T>
+__device__ __forceinline__ typename Elem<T>::V4 zero4() {
+  typename Elem<T>::V4 z = {T(0), T(0), T(0), T(0)};
+  return z;
+}
+
 __device__ __forceinline__ const double* seg_in(const ColSeg& s, int j) {
   return j < s.nA ? s.inA + (int64_t)j * s.ldiA : s.inB + (int64_t)(j - s.nA) * s.ldiB;
 }
@@ -318,25 +350,26 @@ k_getrf(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
 // right and the trailing matrix takes ONE update of inner dimension S*NB,
 // so the trailing matrix crosses HBM once per S panels.
 // =====================================================================
-template <int NB, int NT, int S>
+template <int NB, int NT, int S, typename T>
 __global__ void __launch_bounds__(NT)
-k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
+k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
             int* __restrict__ ipiv0, int* __restrict__ perm0, int* __restrict__ info0) {
-  // per-column candidates, double-buffered by column parity: each wave
+  // per-column candidates, T-buffered by column parity: each wave
   // publishes its best row (key, index and the whole panel row) so the
   // winner's row is readable right after the single barrier of the column
   constexpr int NW = NT / 64;
+  typedef typename Elem<T>::V4 V4;
   __shared__ unsigned long long ckey[2][NW];
   __shared__ int cidx[2][NW];
-  __shared__ double crowv[2][NW][NB];
+  __shared__ T crowv[2][NW][NB];
   __shared__ int pivs[NB];
-  __shared__ double L11s[S][NB * (NB + 1)];  // L11 of each panel of the outer block, [i*(NB+1) + k]
-  __shared__ double Xis[S][NB * (NB + 1)];   // their inverses, same layout
-  __shared__ double Tt[NB * (NB + 1)];    // doubling scratch
+  __shared__ T L11s[S][NB * (NB + 1)];  // L11 of each panel of the outer block, [i*(NB+1) + k]
+  __shared__ T Xis[S][NB * (NB + 1)];   // their inverses, same layout
+  __shared__ T Tt[NB * (NB + 1)];    // doubling scratch
   __shared__ int perm[NT];
   const int b = blockIdx.x;
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
-  double* A = A0 + (int64_t)b * bstride;
+  T* A = A0 + (int64_t)b * bstride;
   int info = 0;  // thread 0
   if (t < n) perm[t] = t;
 
@@ -347,12 +380,12 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
     const int sp = (j0 / NB) % S;
     const int J0 = j0 - sp * NB;
     const int cend = min(J0 + S * NB, n);
-    double* L11 = L11s[sp];
-    double* Xi = Xis[sp];
+    T* L11 = L11s[sp];
+    T* Xi = Xis[sp];
     const bool own = t < rows;
-    double p[NB];
+    T p[NB];
     if (own) {
-      const double* Arow = A + (int64_t)(j0 + t) + (int64_t)j0 * lda;
+      const T* Arow = A + (int64_t)(j0 + t) + (int64_t)j0 * lda;
 #pragma unroll
       for (int c = 0; c < NB; ++c) p[c] = (c < nb) ? Arow[(int64_t)c * lda] : 0.0;
     } else {
@@ -371,10 +404,8 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
         const int par = jj & 1;
         // key = bits(|a|) + 1 orders non-negative doubles; a NaN counts as
         // zero; 0 = not a candidate (retired rows, rows past n)
-        const double aj = p[jj];
-        const unsigned long long mykey =
-            live ? ((aj == aj) ? (unsigned long long)__double_as_longlong(fabs(aj)) + 1ULL : 1ULL)
-                 : 0ULL;
+        const T aj = p[jj];
+        const unsigned long long mykey = live ? Elem<T>::key(aj) : 0ULL;
         const int cpos = mykey ? pos : INT_MAX;
         // this wave's candidate: largest key, then smallest position
         const unsigned long long wmax = wave_max_u64(mykey);
@@ -415,7 +446,7 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
         }
         const int bpos = __builtin_amdgcn_readlane(qw, bw);
         const bool winner = (lane == wl) && (wmax != 0ULL);
-        const double* prow = crowv[par][bw];
+        const T* prow = crowv[par][bw];
         if (t == 0) pivs[jj] = bpos;
         // the global winner retires to position jj; the row at jj takes the
         // winner's old position
@@ -425,10 +456,10 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
         } else if (pos == jj) {
           pos = bpos;
         }
-        const double pv = prow[jj];
+        const T pv = prow[jj];
         if (pv != 0.0) {
           if (live) {
-            const double l = (fabs(pv) >= DBL_MIN) ? p[jj] * (1.0 / pv) : p[jj] / pv;
+            const T l = (fabs(pv) >= Elem<T>::sfmin()) ? p[jj] * (T(1) / pv) : p[jj] / pv;
             p[jj] = l;
 #pragma unroll
             for (int c = 0; c < NB; ++c)
@@ -443,7 +474,7 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
 
     // write the panel rows to their final positions; L11 to LDS
     if (own) {
-      double* Arow = A + (int64_t)(j0 + pos) + (int64_t)j0 * lda;
+      T* Arow = A + (int64_t)(j0 + pos) + (int64_t)j0 * lda;
 #pragma unroll
       for (int c = 0; c < NB; ++c)
         if (c < nb) Arow[(int64_t)c * lda] = p[c];
@@ -474,11 +505,11 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
     const int m2 = n - j0 - nb;
     if (t < n - nb) {
       const int col = t < j0 ? t : t + nb;
-      double* Ac = A + (int64_t)col * lda;
+      T* Ac = A + (int64_t)col * lda;
       for (int jj = 0; jj < nb; ++jj) {
         const int pp = pivs[jj];
         if (pp != jj) {
-          const double tmp = Ac[j0 + jj];
+          const T tmp = Ac[j0 + jj];
           Ac[j0 + jj] = Ac[j0 + pp];
           Ac[j0 + pp] = tmp;
         }
@@ -497,7 +528,7 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
         for (int e = t; e < NB * h / 2; e += NT) {
           const int q = e / (h * h), r = e % (h * h);
           const int s0 = q * 2 * h, i = s0 + h + r / h, j = s0 + r % h;
-          double acc = 0.0;
+          T acc = 0.0;
           for (int k = j; k < s0 + h; ++k) acc += L11[i * (NB + 1) + k] * Xi[k * (NB + 1) + j];
           Tt[i * (NB + 1) + j] = acc;
         }
@@ -505,7 +536,7 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
         for (int e = t; e < NB * h / 2; e += NT) {
           const int q = e / (h * h), r = e % (h * h);
           const int s0 = q * 2 * h, i = s0 + h + r / h, j = s0 + r % h;
-          double acc = 0.0;
+          T acc = 0.0;
           for (int k = s0 + h; k <= i; ++k) acc += Xi[i * (NB + 1) + k] * Tt[k * (NB + 1) + j];
           Xi[i * (NB + 1) + j] = -acc;
         }
@@ -520,7 +551,7 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
       for (int blk = wid; blk < tr * tc; blk += NW) {
         const int r0 = (blk % tr) * 32, c0 = (blk / tr) * 32;
         const bool full = (r0 + 32 <= nr) && (c0 + 32 <= ncl);
-        d4 c[2][2];
+        V4 c[2][2];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -528,13 +559,13 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
             const int col = c0 + bb * 16 + (lane & 15);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-              const int row = r0 + a * 16 + crow(lane, v);
+              const int row = r0 + a * 16 + Elem<T>::crow(lane, v);
               c[a][bb][v] = (full || (row < nr && col < ncl))
                                 ? A[(int64_t)(rb + row) + (int64_t)(cb + col) * lda] : 0.0;
             }
           }
         for (int k0 = 0; k0 < kc; k0 += 16) {
-          double af[4][2], bf[4][2];
+          T af[4][2], bf[4][2];
 #pragma unroll
           for (int kk = 0; kk < 4; ++kk) {
             const int k = k0 + kk * 4 + (lane >> 4);
@@ -563,7 +594,7 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
             const int col = c0 + bb * 16 + (lane & 15);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-              const int row = r0 + a * 16 + crow(lane, v);
+              const int row = r0 + a * 16 + Elem<T>::crow(lane, v);
               if (full || (row < nr && col < ncl))
                 A[(int64_t)(rb + row) + (int64_t)(cb + col) * lda] = c[a][bb][v];
             }
@@ -579,13 +610,13 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
       const int nct = (m2b + 15) >> 4;
       for (int ct = wid; ct < nct; ct += NW) {
         const int col = ct * 16 + (lane & 15);
-        d4 u[NB / 16];
+        V4 u[NB / 16];
 #pragma unroll
-        for (int q = 0; q < NB / 16; ++q) u[q] = d4zero();
+        for (int q = 0; q < NB / 16; ++q) u[q] = zero4<T>();
 #pragma unroll
         for (int kk = 0; kk < NB / 4; ++kk) {
           const int k = kk * 4 + (lane >> 4);
-          const double bf = (k < nb && col < m2b) ? A[(int64_t)(j0 + k) + (int64_t)(o + col) * lda] : 0.0;
+          const T bf = (k < nb && col < m2b) ? A[(int64_t)(j0 + k) + (int64_t)(o + col) * lda] : 0.0;
 #pragma unroll
           for (int q = 0; q < NB / 16; ++q)
             u[q] = mfma(Xi[(q * 16 + (lane & 15)) * (NB + 1) + k], bf, u[q]);
@@ -594,7 +625,7 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
         for (int q = 0; q < NB / 16; ++q)
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            const int r = q * 16 + crow(lane, v);
+            const int r = q * 16 + Elem<T>::crow(lane, v);
             if (col < m2b && r < nb) A[(int64_t)(j0 + r) + (int64_t)(o + col) * lda] = u[q][v];
           }
       }
@@ -610,47 +641,53 @@ k_getrf_reg(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
       for (int ct = wid; ct < nct; ct += NW) {
         const int col = ct * 16 + (lane & 15);
         const bool cok = col < ntr;
-        d4 u[S][NB / 16];
+        V4 u[S][NB / 16];
 #pragma unroll
         for (int s2 = 0; s2 < S; ++s2) {
           if (s2 < nsp) {
             const int r0 = J0 + s2 * NB;
-            d4 acc[NB / 16];
+            V4 acc[NB / 16];
 #pragma unroll
             for (int a = 0; a < NB / 16; ++a)
 #pragma unroll
               for (int v = 0; v < 4; ++v)
-                acc[a][v] = cok ? A[(int64_t)(r0 + a * 16 + crow(lane, v)) + (int64_t)(cend + col) * lda] : 0.0;
+                acc[a][v] = cok ? A[(int64_t)(r0 + a * 16 + Elem<T>::crow(lane, v)) + (int64_t)(cend + col) * lda] : 0.0;
 #pragma unroll
             for (int q = 0; q < S; ++q) {
               if (q < s2) {
+                // k-step (a2, v): rows a2*16 + crow(lane, v) of U_q are
+                // exactly the accumulator element v of tile a2
 #pragma unroll
-                for (int kk = 0; kk < NB / 4; ++kk) {
-                  const int k = kk * 4 + (lane >> 4);
-                  const double bf = u[q][kk >> 2][kk & 3];
+                for (int a2 = 0; a2 < NB / 16; ++a2)
 #pragma unroll
-                  for (int a = 0; a < NB / 16; ++a) {
-                    const double af = A[(int64_t)(r0 + a * 16 + (lane & 15)) + (int64_t)(J0 + q * NB + k) * lda];
-                    acc[a] = mfma(-af, bf, acc[a]);
+                  for (int v = 0; v < 4; ++v) {
+                    const int k = a2 * 16 + Elem<T>::crow(lane, v);
+                    const T bf = u[q][a2][v];
+#pragma unroll
+                    for (int a = 0; a < NB / 16; ++a) {
+                      const T af = A[(int64_t)(r0 + a * 16 + (lane & 15)) + (int64_t)(J0 + q * NB + k) * lda];
+                      acc[a] = mfma(-af, bf, acc[a]);
+                    }
                   }
-                }
               }
             }
 #pragma unroll
-            for (int a = 0; a < NB / 16; ++a) u[s2][a] = d4zero();
+            for (int a = 0; a < NB / 16; ++a) u[s2][a] = zero4<T>();
 #pragma unroll
-            for (int kk = 0; kk < NB / 4; ++kk) {
-              const int k = kk * 4 + (lane >> 4);
-              const double bf = acc[kk >> 2][kk & 3];
+            for (int a2 = 0; a2 < NB / 16; ++a2)
 #pragma unroll
-              for (int a = 0; a < NB / 16; ++a)
-                u[s2][a] = mfma(Xis[s2][(a * 16 + (lane & 15)) * (NB + 1) + k], bf, u[s2][a]);
-            }
+              for (int v = 0; v < 4; ++v) {
+                const int k = a2 * 16 + Elem<T>::crow(lane, v);
+                const T bf = acc[a2][v];
+#pragma unroll
+                for (int a = 0; a < NB / 16; ++a)
+                  u[s2][a] = mfma(Xis[s2][(a * 16 + (lane & 15)) * (NB + 1) + k], bf, u[s2][a]);
+              }
 #pragma unroll
             for (int a = 0; a < NB / 16; ++a)
 #pragma unroll
               for (int v = 0; v < 4; ++v)
-                if (cok) A[(int64_t)(r0 + a * 16 + crow(lane, v)) + (int64_t)(cend + col) * lda] = u[s2][a][v];
+                if (cok) A[(int64_t)(r0 + a * 16 + Elem<T>::crow(lane, v)) + (int64_t)(cend + col) * lda] = u[s2][a][v];
           }
         }
       }
@@ -938,12 +975,12 @@ hipError_t launch_getrf(int n, double* A, int64_t lda, int64_t bstride, int batc
     return hipGetLastError();
   }
   if (n <= 256) {
-    hipLaunchKernelGGL((k_getrf_reg<GETRF_NB, 256, 2>), dim3(batch), dim3(256), 0, st, A, lda, bstride,
+    hipLaunchKernelGGL((k_getrf_reg<GETRF_NB, 256, 2, double>), dim3(batch), dim3(256), 0, st, A, lda, bstride,
                        n, ipiv, perm, info);
     return hipGetLastError();
   }
   if (n <= 512) {
-    hipLaunchKernelGGL((k_getrf_reg<GETRF_NB, 512, 2>), dim3(batch), dim3(512), 0, st, A, lda, bstride,
+    hipLaunchKernelGGL((k_getrf_reg<GETRF_NB, 512, 2, double>), dim3(batch), dim3(512), 0, st, A, lda, bstride,
                        n, ipiv, perm, info);
     return hipGetLastError();
   }
@@ -979,34 +1016,35 @@ hipError_t launch_getrf(int n, double* A, int64_t lda, int64_t bstride, int batc
 // last block is padded with the identity.  Lane c < 32 computes column c
 // of Linv, lane 32 + c column c of Uinv (substitution on e_c).
 // =====================================================================
+template <typename T>
 __global__ void __launch_bounds__(64)
-k_trtri_diag(const double* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
-             double* __restrict__ dinv0) {
-  __shared__ double T[kTri][kTri + 1];  // T[col][row]
+k_trtri_diag(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
+             T* __restrict__ dinv0) {
+  __shared__ T Ts[kTri][kTri + 1];  // Ts[col][row]
   const int nblk = ntri(n);
   const int b = blockIdx.x / nblk, ib = blockIdx.x % nblk;
   const int i0 = ib * kTri;
   const int bs = min(kTri, n - i0);
-  const double* LU = LU0 + (int64_t)b * bstride;
+  const T* LU = LU0 + (int64_t)b * bstride;
   const int tid = threadIdx.x;
   for (int idx = tid; idx < kTri * kTri; idx += 64) {
     const int c = idx / kTri, r = idx % kTri;
-    double v;
+    T v;
     if (r < bs && c < bs) v = LU[(int64_t)(i0 + r) + (int64_t)(i0 + c) * lda];
     else v = (r == c) ? 1.0 : 0.0;
-    T[c][r] = v;
+    Ts[c][r] = v;
   }
   __syncthreads();
-  double* out = dinv0 + (int64_t)b * dinv_elems(n) + (int64_t)ib * 2 * kTri * kTri;
+  T* out = dinv0 + (int64_t)b * dinv_elems(n) + (int64_t)ib * 2 * kTri * kTri;
   const int c = tid & 31;
-  double x[kTri];
+  T x[kTri];
   if (tid < 32) {
     // L x = e_c, unit lower
 #pragma unroll
     for (int i = 0; i < kTri; ++i) {
-      double acc = (i == c) ? 1.0 : 0.0;
+      T acc = (i == c) ? 1.0 : 0.0;
 #pragma unroll
-      for (int k = 0; k < i; ++k) acc -= T[k][i] * x[k];
+      for (int k = 0; k < i; ++k) acc -= Ts[k][i] * x[k];
       x[i] = (i < c) ? 0.0 : acc;
     }
 #pragma unroll
@@ -1016,10 +1054,10 @@ k_trtri_diag(const double* __restrict__ LU0, int64_t lda, int64_t bstride, int n
 #pragma unroll
     for (int ii = 0; ii < kTri; ++ii) {
       const int i = kTri - 1 - ii;
-      double acc = (i == c) ? 1.0 : 0.0;
+      T acc = (i == c) ? 1.0 : 0.0;
 #pragma unroll
-      for (int k = i + 1; k < kTri; ++k) acc -= T[k][i] * x[k];
-      x[i] = (i > c) ? 0.0 : acc / T[i][i];
+      for (int k = i + 1; k < kTri; ++k) acc -= Ts[k][i] * x[k];
+      x[i] = (i > c) ? 0.0 : acc / Ts[i][i];
     }
 #pragma unroll
     for (int i = 0; i < kTri; ++i) out[kTri * kTri + i + c * kTri] = x[i];
@@ -1029,8 +1067,8 @@ k_trtri_diag(const double* __restrict__ LU0, int64_t lda, int64_t bstride, int n
 hipError_t launch_trtri_diag(int n, const double* LU, int64_t lda, int64_t bstride, int batch,
                              double* dinv, hipStream_t st) {
   if (batch <= 0 || n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_trtri_diag, dim3(batch * ntri(n)), dim3(64), 0, st, LU, lda, bstride, n,
-                     dinv);
+  hipLaunchKernelGGL(k_trtri_diag<double>, dim3(batch * ntri(n)), dim3(64), 0, st, LU, lda,
+                     bstride, n, dinv);
   return hipGetLastError();
 }
 
@@ -1167,30 +1205,31 @@ k_getrs(const double* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
 // L2 -- each factor element is read once per chunk, two barriers per block.
 //   TRANS = false: X <- U^-1 L^-1 P^T X;  TRANS = true: X <- P L^-T U^-T X.
 // =====================================================================
-template <int NCT, bool TRANS>
+template <int NCT, bool TRANS, typename T>
 __global__ void __launch_bounds__(512)
-k_getrs_reg(const double* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
-            const int* __restrict__ perm0, const double* __restrict__ dinv0, ColSeg X,
+k_getrs_reg(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
+            const int* __restrict__ perm0, const T* __restrict__ dinv0, ColSeg X,
             int64_t xbstride, int ncols, int nchunk) {
   constexpr int W = NCT * 16;
-  __shared__ double xa[kTri * (W + 1)];  // raw block (owners -> all)
-  __shared__ double xs[kTri * (W + 1)];  // solved block
+  typedef typename Elem<T>::V4 V4;
+  __shared__ T xa[kTri * (W + 1)];  // raw block (owners -> all)
+  __shared__ T xs[kTri * (W + 1)];  // solved block
   const int G = gridDim.x;
   const int gq = G >> 3, grem = G & 7;
   const int xcd = blockIdx.x & 7;
   const int logical = xcd * gq + min(xcd, grem) + (blockIdx.x >> 3);
   const int b = logical / nchunk, chunk = logical % nchunk;
   const int c0 = chunk * W;
-  const double* LU = LU0 + (int64_t)b * bstride;
+  const T* LU = LU0 + (int64_t)b * bstride;
   const int* perm = perm0 + (int64_t)b * n;
-  const double* dinv = dinv0 + (int64_t)b * dinv_elems(n);
+  const T* dinv = dinv0 + (int64_t)b * dinv_elems(n);
   const int64_t xoff = (int64_t)b * xbstride;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int nblk = ntri(n);
   const int nrt = nblk * 2;  // 16-row tiles
 
-  d4 acc[4][NCT];
+  V4 acc[4][NCT];
   // gather (rows by perm for the forward solve)
 #pragma unroll
   for (int q = 0; q < 4; ++q)
@@ -1199,8 +1238,8 @@ k_getrs_reg(const double* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
       const int col = c0 + ct * 16 + lr;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int row = (wid + 8 * q) * 16 + crow(lane, v);
-        double val = 0.0;
+        const int row = (wid + 8 * q) * 16 + Elem<T>::crow(lane, v);
+        T val = 0.0;
         if (row < n && col < ncols) val = seg_in(X, col)[xoff + (TRANS ? row : perm[row])];
         acc[q][ct][v] = val;
       }
@@ -1212,7 +1251,7 @@ k_getrs_reg(const double* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
     for (int step = 0; step < nblk; ++step) {
       const int ib = fwd ? step : nblk - 1 - step;
       const int i0 = ib * kTri;
-      const double* Di = dinv + (int64_t)ib * 2 * kTri * kTri + islot * kTri * kTri;
+      const T* Di = dinv + (int64_t)ib * 2 * kTri * kTri + islot * kTri * kTri;
       // (a1) owners of row tiles 2ib, 2ib+1 stage the raw block
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -1222,23 +1261,23 @@ k_getrs_reg(const double* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
           for (int ct = 0; ct < NCT; ++ct)
 #pragma unroll
             for (int v = 0; v < 4; ++v)
-              xa[((rt & 1) * 16 + crow(lane, v)) * (W + 1) + ct * 16 + lr] = acc[q][ct][v];
+              xa[((rt & 1) * 16 + Elem<T>::crow(lane, v)) * (W + 1) + ct * 16 + lr] = acc[q][ct][v];
         }
       }
       __syncthreads();
       // (a2) xs = D * xa, 2*NCT tiles spread over the waves
       for (int tt = wid; tt < 2 * NCT; tt += 8) {
         const int rtl = tt & 1, ct = tt >> 1;
-        d4 a4 = d4zero();
+        V4 a4 = zero4<T>();
 #pragma unroll
         for (int kk = 0; kk < kTri / 4; ++kk) {
           const int k = kk * 4 + lk;
           const int row = rtl * 16 + lr;
-          const double af = TRANS ? Di[k + row * kTri] : Di[row + k * kTri];
+          const T af = TRANS ? Di[k + row * kTri] : Di[row + k * kTri];
           a4 = mfma(af, xa[k * (W + 1) + ct * 16 + lr], a4);
         }
 #pragma unroll
-        for (int v = 0; v < 4; ++v) xs[(rtl * 16 + crow(lane, v)) * (W + 1) + ct * 16 + lr] = a4[v];
+        for (int v = 0; v < 4; ++v) xs[(rtl * 16 + Elem<T>::crow(lane, v)) * (W + 1) + ct * 16 + lr] = a4[v];
       }
       __syncthreads();
       // (b) owners take the solved block back; other tiles on the far side
@@ -1252,14 +1291,14 @@ k_getrs_reg(const double* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
             for (int ct = 0; ct < NCT; ++ct)
 #pragma unroll
               for (int v = 0; v < 4; ++v)
-                acc[q][ct][v] = xs[((rt & 1) * 16 + crow(lane, v)) * (W + 1) + ct * 16 + lr];
+                acc[q][ct][v] = xs[((rt & 1) * 16 + Elem<T>::crow(lane, v)) * (W + 1) + ct * 16 + lr];
           } else if (fwd ? (rt >> 1) > ib : (rt >> 1) < ib) {
             const int row = rt * 16 + lr;
 #pragma unroll
             for (int kk = 0; kk < kTri / 4; ++kk) {
               const int k = kk * 4 + lk;
               const int kcol = i0 + k;
-              double af = 0.0;
+              T af = 0.0;
               if (row < n && kcol < n)
                 af = TRANS ? LU[(int64_t)kcol + (int64_t)row * lda] : LU[(int64_t)row + (int64_t)kcol * lda];
 #pragma unroll
@@ -1279,18 +1318,18 @@ k_getrs_reg(const double* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
       const int col = c0 + ct * 16 + lr;
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int row = (wid + 8 * q) * 16 + crow(lane, v);
+        const int row = (wid + 8 * q) * 16 + Elem<T>::crow(lane, v);
         if (row < n && col < ncols) seg_out(X, col)[xoff + (TRANS ? perm[row] : row)] = acc[q][ct][v];
       }
     }
 }
 
-template <int NCT, bool TRANS>
-static void launch_getrs_reg(int n, const double* LU, int64_t lda, int64_t bstride, int batch,
-                             const int* perm, const double* dinv, const ColSeg& X,
+template <int NCT, bool TRANS, typename T>
+static void launch_getrs_reg(int n, const T* LU, int64_t lda, int64_t bstride, int batch,
+                             const int* perm, const T* dinv, const ColSeg& X,
                              int64_t xbstride, int ncols, hipStream_t st) {
   const int nchunk = (ncols + NCT * 16 - 1) / (NCT * 16);
-  hipLaunchKernelGGL((k_getrs_reg<NCT, TRANS>), dim3(batch * nchunk), dim3(512), 0, st, LU, lda,
+  hipLaunchKernelGGL((k_getrs_reg<NCT, TRANS, T>), dim3(batch * nchunk), dim3(512), 0, st, LU, lda,
                      bstride, n, perm, dinv, X, xbstride, ncols, nchunk);
 }
 
@@ -1303,11 +1342,11 @@ static hipError_t launch_getrs_impl(int n, const double* LU, int64_t lda, int64_
   if (n > kMaxBlock) return hipErrorInvalidValue;
   if (n <= 512) {
     if (ncols <= 16)
-      launch_getrs_reg<1, TRANS>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
+      launch_getrs_reg<1, TRANS, double>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
     else if (ncols <= 32)
-      launch_getrs_reg<2, TRANS>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
+      launch_getrs_reg<2, TRANS, double>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
     else
-      launch_getrs_reg<5, TRANS>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
+      launch_getrs_reg<5, TRANS, double>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
     return hipGetLastError();
   }
   const int npad = ntri(n) * kTri;
@@ -1349,6 +1388,118 @@ hipError_t launch_getrs_t(int n, const double* LU, int64_t lda, int64_t bstride,
                           const int* perm, const double* dinv, const ColSeg& X, int64_t xbstride,
                           int ncols, hipStream_t st) {
   return launch_getrs_impl<true>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
+}
+
+// FP32 leaf kernels for the mixed-precision path (blocks up to 512).  The
+// right-hand sides stay FP64 in memory; the solve converts on load/store.
+hipError_t launch_getrf_f32(int n, float* A, int64_t lda, int64_t bstride, int batch, int* ipiv,
+                            int* perm, int* info, hipStream_t st) {
+  if (batch <= 0 || n <= 0) return hipSuccess;
+  if (n > 512) return hipErrorInvalidValue;
+  if (n <= 256)
+    hipLaunchKernelGGL((k_getrf_reg<GETRF_NB, 256, 2, float>), dim3(batch), dim3(256), 0, st, A,
+                       lda, bstride, n, ipiv, perm, info);
+  else
+    hipLaunchKernelGGL((k_getrf_reg<GETRF_NB, 512, 2, float>), dim3(batch), dim3(512), 0, st, A,
+                       lda, bstride, n, ipiv, perm, info);
+  return hipGetLastError();
+}
+
+hipError_t launch_trtri_diag_f32(int n, const float* LU, int64_t lda, int64_t bstride, int batch,
+                                 float* dinv, hipStream_t st) {
+  if (batch <= 0 || n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_trtri_diag<float>, dim3(batch * ntri(n)), dim3(64), 0, st, LU, lda,
+                     bstride, n, dinv);
+  return hipGetLastError();
+}
+
+hipError_t launch_getrs_f32(int n, const float* LU, int64_t lda, int64_t bstride, int batch,
+                            const int* perm, const float* dinv, const ColSeg& X, int64_t xbstride,
+                            int ncols, bool trans, hipStream_t st) {
+  if (batch <= 0 || n <= 0 || ncols <= 0) return hipSuccess;
+  if (n > 512) return hipErrorInvalidValue;
+  if (trans) {
+    if (ncols <= 16) launch_getrs_reg<1, true, float>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
+    else if (ncols <= 32) launch_getrs_reg<2, true, float>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
+    else launch_getrs_reg<5, true, float>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
+  } else {
+    if (ncols <= 16) launch_getrs_reg<1, false, float>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
+    else if (ncols <= 32) launch_getrs_reg<2, false, float>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
+    else launch_getrs_reg<5, false, float>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
+  }
+  return hipGetLastError();
+}
+
+__global__ void k_convert_f32(int64_t rows, int cols, const double* __restrict__ src, int64_t lds,
+                              float* __restrict__ dst, int64_t ldd) {
+  const int64_t total = rows * (int64_t)cols;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * 256) {
+    const int64_t i = e % rows, j = e / rows;
+    dst[i + j * ldd] = (float)src[i + j * lds];
+  }
+}
+
+hipError_t launch_convert_f32(int64_t rows, int cols, const double* src, int64_t lds, float* dst,
+                              int64_t ldd, hipStream_t st) {
+  const int64_t total = rows * (int64_t)cols;
+  if (total <= 0) return hipSuccess;
+  int64_t g = (total + 255) / 256;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(k_convert_f32, dim3((unsigned)g), dim3(256), 0, st, rows, cols, src, lds, dst,
+                     ldd);
+  return hipGetLastError();
+}
+
+// Column max-norms: out[j] = max_i |A(i, j)| (one workgroup per column).
+__global__ void __launch_bounds__(256) k_colmax(int64_t rows, const double* __restrict__ A,
+                                                int64_t lda, double* __restrict__ out) {
+  __shared__ double red[256];
+  const int j = blockIdx.x;
+  double m = 0.0;
+  for (int64_t i = threadIdx.x; i < rows; i += 256) {
+    const double a = fabs(A[i + (int64_t)j * lda]);
+    m = (a > m || a != a) ? a : m;  // a NaN propagates
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      const double o = red[threadIdx.x + w];
+      const double c = red[threadIdx.x];
+      red[threadIdx.x] = (o > c || o != o) ? o : c;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = red[0];
+}
+
+hipError_t launch_colmax(int64_t rows, int cols, const double* A, int64_t lda, double* out,
+                         hipStream_t st) {
+  if (cols <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_colmax, dim3(cols), dim3(256), 0, st, rows, A, lda, out);
+  return hipGetLastError();
+}
+
+// Y += X (rows x cols).
+__global__ void k_axpy_cols(int64_t rows, int cols, const double* __restrict__ X, int64_t ldx,
+                            double* __restrict__ Y, int64_t ldy) {
+  const int64_t total = rows * (int64_t)cols;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * 256) {
+    const int64_t i = e % rows, j = e / rows;
+    Y[i + j * ldy] += X[i + j * ldx];
+  }
+}
+
+hipError_t launch_add_cols(int64_t rows, int cols, const double* X, int64_t ldx, double* Y,
+                           int64_t ldy, hipStream_t st) {
+  const int64_t total = rows * (int64_t)cols;
+  if (total <= 0) return hipSuccess;
+  int64_t g = (total + 255) / 256;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(k_axpy_cols, dim3((unsigned)g), dim3(256), 0, st, rows, cols, X, ldx, Y, ldy);
+  return hipGetLastError();
 }
 
 // =====================================================================
@@ -1957,7 +2108,6 @@ __global__ void __launch_bounds__(256) k_bgemm(BGemmArgs g) {
     const int arow = tm * 16 + lr;
     const int bcol = col0 + lr;
     d4 acc = d4zero();
-#pragma unroll 4
     for (int k0 = 0; k0 < g.K; k0 += 4) {
       const int k = k0 + lk;
       double af = 0.0, bf = 0.0;
@@ -2027,6 +2177,23 @@ __global__ void k_selftest_mfma(double* __restrict__ out) {
 
 hipError_t launch_selftest_mfma(double* out, hipStream_t st) {
   hipLaunchKernelGGL(k_selftest_mfma, dim3(1), dim3(64), 0, st, out);
+  return hipGetLastError();
+}
+
+// Same probe for v_mfma_f32_16x16x4_f32; out[4*lane + v] (as double).
+__global__ void k_selftest_mfma_f32(double* __restrict__ out) {
+  typedef float f4_ __attribute__((ext_vector_type(4)));
+  const int l = threadIdx.x;
+  const float a = (float)(l & 15) + 100.0f * (float)(l >> 4);
+  const float b = ((l >> 4) == ((l & 15) & 3)) ? 1.0f : 0.0f;
+  f4_ z = {0.0f, 0.0f, 0.0f, 0.0f};
+  f4_ c = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, z, 0, 0, 0);
+#pragma unroll
+  for (int v = 0; v < 4; ++v) out[4 * l + v] = (double)c[v];
+}
+
+hipError_t launch_selftest_mfma_f32(double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_selftest_mfma_f32, dim3(1), dim3(64), 0, st, out);
   return hipGetLastError();
 }
 

@@ -75,7 +75,8 @@ int hodlr_plan_destroy(hodlr_plan plan);
 int64_t hodlr_plan_device_bytes(hodlr_plan plan);
 
 /* Factorization only (leaf LUs, the per-level Y_t blocks, core LUs).
- * K (N x leaf, ld ldk) is overwritten by the leaf LU factors. */
+ * K (N x leaf, ld ldk) is overwritten by the leaf LU factors -- except for a
+ * HODLR_MIXED_IR plan, which factors an FP32 copy and leaves K untouched. */
 int hodlr_factor(hodlr_plan plan, double *K, int64_t ldk,
                  const double *U, int64_t ldu, const double *V, int64_t ldv,
                  hipStream_t stream);
@@ -92,6 +93,26 @@ int hodlr_solve(hodlr_plan plan, double *B, int64_t ldb, int nrhs, hipStream_t s
 int hodlr_factor_solve(hodlr_plan plan, double *K, int64_t ldk,
                        const double *U, int64_t ldu, const double *V, int64_t ldv,
                        double *B, int64_t ldb, int nrhs, hipStream_t stream);
+
+/* Iterative refinement in FP64 (north-star mixed-precision mode; also valid
+ * for an FP64 plan).  X (N x nrhs) <- A^-1 B refined:
+ *   X = solve(B); repeat { R = B - A X (FP64 HODLR matvec with the ORIGINAL
+ *   K, U, V); stop if max_j |R_j|_inf / |B_j|_inf <= tol; X += solve(R) }
+ * with solve() the plan's stored factorization (FP32 leaf LU for a
+ * HODLR_MIXED_IR plan, whose hodlr_factor leaves K untouched).  At most
+ * maxit corrections.  Synchronizes `stream` once per iteration.  Returns
+ * HODLR_OK whether or not tol was reached (see stats->converged).  No
+ * reference counterpart (the reference solves in FP64 only). */
+typedef struct hodlr_ir_stats {
+  int    iterations;    /* corrections applied                              */
+  int    converged;     /* 1 if the residual reached tol                    */
+  double residual;      /* final max_j |b_j - A x_j|_inf / |b_j|_inf        */
+  double history[33];   /* residual after 0, 1, ... corrections            */
+} hodlr_ir_stats;
+int hodlr_solve_ir(hodlr_plan plan, const double *K, int64_t ldk, const double *U, int64_t ldu,
+                   const double *V, int64_t ldv, const double *B, int64_t ldb, double *X,
+                   int64_t ldx, int nrhs, double tol, int maxit, hodlr_ir_stats *stats,
+                   hipStream_t stream);
 
 /* Adjoint (transposed) solve with the stored factorization, in place:
  * G <- A^-T G, nrhs columns (any number; processed in chunks).  The
@@ -228,6 +249,11 @@ int hodlr_profile_read(int kind, int64_t *count, double *total_ms, double *flops
  * element 4*l + v must equal row + 100*(col % 4) with row = l/16 + 4v,
  * col = l%16.  No reference counterpart. */
 int hodlr_selftest_mfma(double *out, hipStream_t stream);
+/* Same probe for the FP32 matrix core (v_mfma_f32_16x16x4f32) used by the
+ * mixed-precision leaf kernels: 256 values (as doubles), element 4*l + v
+ * equal to row + 100*(col % 4) with row = 4*(l/16) + v, col = l%16.  No
+ * reference counterpart. */
+int hodlr_selftest_mfma_f32(double *out, hipStream_t stream);
 
 /* Library version string. */
 const char *hodlr_version(void);

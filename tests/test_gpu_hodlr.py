@@ -164,6 +164,28 @@ def test_factor_solve_matches_oracle(gpu, orc, N, m, r, s):
     plan.close()
 
 
+
+def test_padded_leading_dimension_is_bitwise_identical(gpu):
+    """Operands with leading dimension N + 8 (the benchmark layout) give
+    bit-identical results to ld = N: addressing only, same arithmetic."""
+    F = gpu
+    N, m, r, s = 8192, 256, 32, 4
+    L = 5
+    say(f"case: padded ld N={N} m={m} r={r} s={s}")
+    outs = []
+    for pad in (0, 8):
+        K, U, V, B = F.workload_gaussian(N, m, r, s, seed=4, pad=pad)
+        assert F._ld(K) == N + pad
+        plan = F.HodlrPlan(N, m, L, r, nrhs_max=s)
+        plan.factor_solve(K, U, V, B)
+        assert plan.status()[0] == 0
+        X2 = F.fortran_clone(B)
+        plan.solve(X2)  # factor reuse on the solution (solves A y = x)
+        outs.append((to_host(B), to_host(X2)))
+        plan.close()
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+
 def test_factor_then_solve_many_rhs(gpu, orc):
     """hodlr_factor + hodlr_solve with more columns than nrhs_max (column
     batches), against the oracle."""
