@@ -80,6 +80,56 @@ def cpu_baseline(K, U, V, B, m, nthreads, runs=3):
     return statistics.median(times), name, tl
 
 
+def bench_config4(F, torch, W, log):
+    """N = 2^20, 2048 leaves of 512 (11 levels), rank 64, 4 RHS iid N(0,1),
+    leaf blocks G + s I with s calibrated to cond ~ 1e4: mixed + refinement
+    to 1e-12 next to the plain FP64 factor + solve, one timed run each."""
+    from fastsolver2_amd.workloads import conditioned_shift_global
+    N, m, r, s = 1 << 20, 512, 64, 4
+    L = 11
+    shift, cnd = conditioned_shift_global(N, m, r, 1e4, seed=0)
+    K, U, V, B = F.workload_gaussian(N, m, r, s, seed=0, shift=shift, rhs="normal", pad=LD_PAD)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    mplan = F.HodlrPlan(N, m, L, r, nrhs_max=s, precision=F.MIXED_IR)
+    X = F.fortran_empty(N, s, pad=LD_PAD)
+    tm = []
+    for i in range(W + 1):
+        torch.cuda.synchronize()
+        ev0.record()
+        X.copy_(B)
+        mplan.factor_solve(K, U, V, X)
+        X, st = mplan.solve_ir(K, U, V, B, X=X, tol=1e-12, maxit=30, x_given=True)
+        ev1.record()
+        torch.cuda.synchronize()
+        tm.append(ev0.elapsed_time(ev1))
+    mplan.close()
+    plan = F.HodlrPlan(N, m, L, r, nrhs_max=s)
+    Kf = F.fortran_clone(K)
+    Xf = F.fortran_clone(B)
+    tf = []
+    for i in range(W + 1):
+        Kf.copy_(K)
+        Xf.copy_(B)
+        torch.cuda.synchronize()
+        ev0.record()
+        plan.factor_solve(Kf, U, V, Xf)
+        ev1.record()
+        torch.cuda.synchronize()
+        tf.append(ev0.elapsed_time(ev1))
+    plan.close()
+    err = (X - Xf).abs().max().item() / Xf.abs().max().item()
+    out = {"workload": f"N={N} leaf={m} levels={L} rank={r} nrhs={s}, leaf shift {shift:.4f} "
+                       f"(estimated cond_2(A) {cnd:.3g}), B iid N(0,1)",
+           "mixed_ir_ms": round(tm[-1], 3), "fp64_ms": round(tf[-1], 3),
+           "speedup_vs_fp64": round(tf[-1] / tm[-1], 3), "iterations": st["iterations"],
+           "residual": st["residual"], "max_rel_diff_vs_fp64": err}
+    log(f"config N=2^20 cond~1e4: mixed+IR {tm[-1]:.2f} ms ({st['iterations']} corrections, "
+        f"residual {st['residual']:.2e}), FP64 {tf[-1]:.2f} ms")
+    del K, U, V, B, X, Kf, Xf
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,6 +137,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--config4", action=argparse.BooleanOptionalAction, default=True,
+                    help="mixed precision at N=2^20, cond~1e4 (default on)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--n", type=int, default=262144)
     ap.add_argument("--leaf", type=int, default=512)
@@ -212,6 +265,40 @@ def main():
             traffic = None
     roof["traffic"] = traffic
 
+    # mixed precision + FP64 refinement to 1e-12 on the same system: FP32
+    # leaf LU, FP64 levels, residuals with the FP64 HODLR matvec
+    mixed = None
+    if not args.no_mixed:
+        mplan = F.HodlrPlan(N, m, L, r, nrhs_max=s, precision=F.MIXED_IR)
+        Xm = F.fortran_empty(N, s, pad=LD_PAD)
+        mts, sts = [], None
+        for i in range(W + K_):
+            torch.cuda.synchronize()
+            ev0.record()
+            Xm.copy_(B0)
+            mplan.factor_solve(K0, U, V, Xm)  # K0 untouched: the FP32 copy is factored
+            Xm, sts = mplan.solve_ir(K0, U, V, B0, X=Xm, tol=1e-12, maxit=10, x_given=True)
+            ev1.record()
+            torch.cuda.synchronize()
+            if i >= W:
+                mts.append(ev0.elapsed_time(ev1))
+        mms = statistics.median(mts)
+        err = (Xm - Bs[W + K_ - 1]).abs().max().item() / Bs[W + K_ - 1].abs().max().item()
+        mixed = {"workload": f"same system as value (N={N} leaf={m} rank={r} nrhs={s}), "
+                             "FP32 leaf LU + FP64 levels, FP64 refinement to 1e-12",
+                 "value": round(1000.0 / mms, 3), "unit": "solves/s", "ms": round(mms, 4),
+                 "iterations": sts["iterations"], "residual": sts["residual"],
+                 "history": sts["history"], "max_rel_diff_vs_fp64": err,
+                 "speedup_vs_fp64": round(ms / mms, 3),
+                 "gflops_alg": round(fa / mms / 1e6, 2)}
+        log(f"mixed + IR: {mms:.3f} ms/solve ({1000.0 / mms:.1f} solves/s, {ms / mms:.2f}x FP64), "
+            f"{sts['iterations']} corrections, residual {sts['residual']:.2e}, "
+            f"max rel diff vs FP64 {err:.2e}")
+        mplan.close()
+        del Xm
+        if args.config4:
+            mixed["config4"] = bench_config4(F, torch, W, log)
+
     # differentiable solve (N=65536, leaf 256, rank 32, 16 RHS): forward
     # factor+solve, then the adjoint solve and the K/U/V gradients
     adj = None
@@ -295,7 +382,7 @@ def main():
         "gflops_exec": round(fe / ms / 1e6, 2),
         "solve_only_ms": round(solve_ms, 4),
         "relative_residual": rel_res,
-        "mixed_ir": None,
+        "mixed_ir": mixed,
         "adjoint_ms": adj["adjoint_ms"] if adj else None,
         "adjoint": adj,
         "roofline": roof,

@@ -84,8 +84,10 @@ def lib():
         "hodlr_factor_solve": ([vp, vp, i64, vp, i64, vp, i64, vp, i64, ci, vp], ci),
         "hodlr_plan_status": ([vp, vp, C.POINTER(ci)], ci),
         "hodlr_solve_adjoint": ([vp, vp, i64, ci, vp], ci),
-        "hodlr_solve_ir": ([vp, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64, ci, dbl, ci,
+        "hodlr_solve_ir": ([vp, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64, ci, dbl, ci, ci,
                             C.POINTER(IRStats), vp], ci),
+        "hodlr_matvec_t": ([i64, ci, ci, ci, vp, i64, vp, i64, vp, i64, vp, i64, dbl, dbl, vp, i64,
+                            vp], ci),
         "hodlr_grad": ([i64, ci, ci, ci, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64,
                         vp, i64, vp], ci),
         "hodlr_matvec": ([i64, ci, ci, ci, vp, i64, vp, i64, vp, i64, vp, i64, dbl, dbl, vp, i64,
@@ -243,18 +245,21 @@ class HodlrPlan:
               where="hodlr_solve")
         return B
 
-    def solve_ir(self, K, U, V, B, X=None, tol=1e-12, maxit=10):
+    def solve_ir(self, K, U, V, B, X=None, tol=1e-12, maxit=10, x_given=False):
         """Iteratively refined solve X = A^-1 B against the ORIGINAL K, U, V
-        (FP64 HODLR-matvec residuals); returns (X, stats dict)."""
+        (FP64 HODLR-matvec residuals); returns (X, stats dict).  With
+        x_given, X already holds a first solution (e.g. from factor_solve)."""
         for t, nm in ((K, "K"), (U, "U"), (V, "V"), (B, "B")):
             _check(t, nm)
         B2 = B if B.dim() == 2 else B.view(-1, 1)
         if X is None:
+            if x_given:
+                raise ValueError("x_given needs X")
             X = fortran_empty(B2.shape[0], B2.shape[1], device=B.device)
         st = IRStats()
         _call(lib().hodlr_solve_ir, self._h, _ptr(K), _ld(K), _ptr(U), _ld(U), _ptr(V), _ld(V),
               _ptr(B2), _ld(B2), _ptr(X), _ld(X), B2.shape[1], float(tol), int(maxit),
-              C.byref(st), _stream(), where="hodlr_solve_ir")
+              1 if x_given else 0, C.byref(st), _stream(), where="hodlr_solve_ir")
         return X, st.as_dict()
 
     def solve_adjoint(self, G):
@@ -313,8 +318,8 @@ def node_solve(VTu, VTd, r):
                                   VTd.shape[1], _stream())
 
 
-def matvec(K, U, V, X, leaf, alpha=1.0, beta=0.0, Y=None):
-    """Y = alpha * A X + beta * Y for the HODLR matrix (K, U, V)."""
+def matvec(K, U, V, X, leaf, alpha=1.0, beta=0.0, Y=None, trans=False):
+    """Y = alpha * A X + beta * Y (A^T with trans) for the HODLR matrix (K, U, V)."""
     for t, nm in ((K, "K"), (U, "U"), (V, "V"), (X, "X")):
         _check(t, nm)
     X2 = X if X.dim() == 2 else X.view(-1, 1)
@@ -322,7 +327,8 @@ def matvec(K, U, V, X, leaf, alpha=1.0, beta=0.0, Y=None):
         Y = fortran_empty(X2.shape[0], X2.shape[1], device=X.device)
         beta = 0.0
     Y2 = Y if Y.dim() == 2 else Y.view(-1, 1)
-    _call(lib().hodlr_matvec, K.shape[0], leaf, U.shape[1], X2.shape[1], _ptr(K), _ld(K),
+    fn = lib().hodlr_matvec_t if trans else lib().hodlr_matvec
+    _call(fn, K.shape[0], leaf, U.shape[1], X2.shape[1], _ptr(K), _ld(K),
           _ptr(U), _ld(U), _ptr(V), _ld(V), _ptr(X2), _ld(X2), alpha, beta, _ptr(Y2), _ld(Y2),
           _stream(), where="hodlr_matvec")
     return Y

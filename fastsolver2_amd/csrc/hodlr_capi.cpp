@@ -706,10 +706,11 @@ int hodlr_solve_adjoint(hodlr_plan p, double* G, int64_t ldg, int nrhs, hipStrea
 
 int hodlr_solve_ir(hodlr_plan p, const double* K, int64_t ldk, const double* U, int64_t ldu,
                    const double* V, int64_t ldv, const double* B, int64_t ldb, double* X,
-                   int64_t ldx, int nrhs, double tol, int maxit, hodlr_ir_stats* stats,
-                   hipStream_t st) {
+                   int64_t ldx, int nrhs, double tol, int maxit, int flags,
+                   hodlr_ir_stats* stats, hipStream_t st) {
   if (!p || !K || !U || !V || !B || !X || nrhs < 0 || maxit < 0 || maxit > 32 || !(tol >= 0.0))
     return HODLR_ERR_ARG;
+  if (flags & ~HODLR_IR_X_GIVEN) return HODLR_ERR_ARG;
   if (!p->factored) return HODLR_ERR_STATE;
   const int64_t N = p->N;
   if (check_ld(ldk, N) || check_ld(ldu, N) || check_ld(ldv, N) || check_ld(ldb, N) ||
@@ -742,10 +743,12 @@ int hodlr_solve_ir(hodlr_plan p, const double* K, int64_t ldk, const double* U, 
   std::vector<double> hb(nrhs), hr(nrhs);
   HIP_TRY(hodlr::launch_colmax(N, nrhs, B, ldb, nb, st));
   HIP_TRY(hipMemcpyAsync(hb.data(), nb, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
-  // X = solve(B)
-  HIP_TRY(hodlr::launch_copy_cols(N, nrhs, B, ldb, X, ldx, st));
-  int rc = hodlr_solve(p, X, ldx, nrhs, st);
-  if (rc) return rc;
+  int rc = HODLR_OK;
+  if (!(flags & HODLR_IR_X_GIVEN)) {  // X = solve(B)
+    HIP_TRY(hodlr::launch_copy_cols(N, nrhs, B, ldb, X, ldx, st));
+    rc = hodlr_solve(p, X, ldx, nrhs, st);
+    if (rc) return rc;
+  }
   for (int it = 0;; ++it) {
     // R = B - A X
     HIP_TRY(hodlr::launch_copy_cols(N, nrhs, B, ldb, R, N, st));
@@ -844,9 +847,10 @@ int hodlr_plan_status(hodlr_plan p, hipStream_t st, int* where) {
   return HODLR_OK;
 }
 
-int hodlr_matvec(int64_t n, int leaf, int rank, int nrhs, const double* K, int64_t ldk,
-                 const double* U, int64_t ldu, const double* V, int64_t ldv, const double* X,
-                 int64_t ldx, double alpha, double beta, double* Y, int64_t ldy, hipStream_t st) {
+static int matvec_impl(bool trans, int64_t n, int leaf, int rank, int nrhs, const double* K,
+                       int64_t ldk, const double* U, int64_t ldu, const double* V, int64_t ldv,
+                       const double* X, int64_t ldx, double alpha, double beta, double* Y,
+                       int64_t ldy, hipStream_t st) {
   if (n <= 0 || leaf <= 0 || rank <= 0 || nrhs < 0 || n % leaf) return HODLR_ERR_ARG;
   if (!K || !U || !V || !X || !Y) return HODLR_ERR_ARG;
   if (ldk < n || ldu < n || ldv < n || ldx < n || ldy < n) return HODLR_ERR_ARG;
@@ -855,8 +859,23 @@ int hodlr_matvec(int64_t n, int leaf, int rank, int nrhs, const double* K, int64
   int rc = workspace(sizeof(double) * (size_t)hodlr::matvec_work_elems(n, leaf, rank, nrhs), &w);
   if (rc) return rc;
   HIP_TRY(hodlr::launch_matvec(n, leaf, rank, nrhs, K, ldk, U, ldu, V, ldv, X, ldx, alpha, beta, Y,
-                               ldy, static_cast<double*>(w), st));
+                               ldy, static_cast<double*>(w), st, trans));
   return HODLR_OK;
+}
+
+int hodlr_matvec(int64_t n, int leaf, int rank, int nrhs, const double* K, int64_t ldk,
+                 const double* U, int64_t ldu, const double* V, int64_t ldv, const double* X,
+                 int64_t ldx, double alpha, double beta, double* Y, int64_t ldy, hipStream_t st) {
+  return matvec_impl(false, n, leaf, rank, nrhs, K, ldk, U, ldu, V, ldv, X, ldx, alpha, beta, Y,
+                     ldy, st);
+}
+
+int hodlr_matvec_t(int64_t n, int leaf, int rank, int nrhs, const double* K, int64_t ldk,
+                   const double* U, int64_t ldu, const double* V, int64_t ldv, const double* X,
+                   int64_t ldx, double alpha, double beta, double* Y, int64_t ldy,
+                   hipStream_t st) {
+  return matvec_impl(true, n, leaf, rank, nrhs, K, ldk, U, ldu, V, ldv, X, ldx, alpha, beta, Y,
+                     ldy, st);
 }
 
 // ---------------------------------------------------------------------

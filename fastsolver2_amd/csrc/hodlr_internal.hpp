@@ -109,7 +109,8 @@ hipError_t launch_sum_slots(const double* part, int nchunk, int nslot, int ka, i
 hipError_t launch_matvec(int64_t n, int leaf, int rank, int nrhs, const double* K,
                          int64_t ldk, const double* U, int64_t ldu, const double* V,
                          int64_t ldv, const double* X, int64_t ldx, double alpha, double beta,
-                         double* Y, int64_t ldy, double* work, hipStream_t st);
+                         double* Y, int64_t ldy, double* work, hipStream_t st,
+                         bool trans = false);
 int64_t matvec_work_elems(int64_t n, int leaf, int rank, int nrhs);
 // part (nleaf x r x nrhs) = V_i^T X_i per leaf, tot (r x nrhs) = their sum
 // in leaf order.

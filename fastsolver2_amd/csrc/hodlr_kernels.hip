@@ -15,6 +15,7 @@
 // -> gemmBro (spmd_driver/solver_irregular_part.cc:117-140).  The kernels
 // here are the batched, level-synchronous building blocks of that loop.
 #include <hip/hip_runtime.h>
+#include <utility>
 
 #include <cfloat>
 #include <climits>
@@ -1785,7 +1786,7 @@ __global__ void k_mv_total(const double* __restrict__ part, int nleaf, int64_t e
   tot[e] = s;
 }
 
-template <int NCT>
+template <int NCT, bool TRANS>
 __global__ void __launch_bounds__(256)
 k_mv_apply(int leaf, int r, int nrhs, const double* __restrict__ K, int64_t ldk,
            const double* __restrict__ U, int64_t ldu, const double* __restrict__ X, int64_t ldx,
@@ -1804,7 +1805,9 @@ k_mv_apply(int leaf, int r, int nrhs, const double* __restrict__ K, int64_t ldk,
   for (int k0 = 0; k0 < leaf; k0 += 4) {
     const int k = k0 + lk;
     const int arow = rloc + lr;
-    const double af = (k < leaf && arow < leaf) ? K[l0 + arow + (int64_t)k * ldk] : 0.0;
+    double af = 0.0;
+    if (k < leaf && arow < leaf)
+      af = TRANS ? K[l0 + k + (int64_t)arow * ldk] : K[l0 + arow + (int64_t)k * ldk];
 #pragma unroll
     for (int ct = 0; ct < NCT; ++ct) {
       if (j0 + ct * 16 < nrhs) {
@@ -1924,7 +1927,13 @@ hipError_t launch_leaf_outer(int64_t n, int leaf, int nc, int k, double alpha, c
 hipError_t launch_matvec(int64_t n, int leaf, int rank, int nrhs, const double* K, int64_t ldk,
                          const double* U, int64_t ldu, const double* V, int64_t ldv,
                          const double* X, int64_t ldx, double alpha, double beta, double* Y,
-                         int64_t ldy, double* work, hipStream_t st) {
+                         int64_t ldy, double* work, hipStream_t st, bool trans) {
+  if (trans) {
+    // A^T = blockdiag(K_i^T) + (V U^T off the leaf blocks): the same product
+    // with U and V exchanged and the leaf blocks read transposed
+    std::swap(U, V);
+    std::swap(ldu, ldv);
+  }
   const int64_t nleaf = n / leaf;
   double* part = work;
   double* tot = work + nleaf * (int64_t)rank * nrhs;
@@ -1937,8 +1946,12 @@ hipError_t launch_matvec(int64_t n, int leaf, int rank, int nrhs, const double* 
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const int gpl = (leaf + 63) / 64;
   dim3 grid((unsigned)(nleaf * gpl), (unsigned)((nrhs + 63) / 64));
-  hipLaunchKernelGGL(k_mv_apply<4>, grid, dim3(256), 0, st, leaf, rank, nrhs, K, ldk, U, ldu, X,
-                     ldx, part, tot, alpha, beta, Y, ldy, gpl);
+  if (trans)
+    hipLaunchKernelGGL((k_mv_apply<4, true>), grid, dim3(256), 0, st, leaf, rank, nrhs, K, ldk, U,
+                       ldu, X, ldx, part, tot, alpha, beta, Y, ldy, gpl);
+  else
+    hipLaunchKernelGGL((k_mv_apply<4, false>), grid, dim3(256), 0, st, leaf, rank, nrhs, K, ldk, U,
+                       ldu, X, ldx, part, tot, alpha, beta, Y, ldy, gpl);
   return hipGetLastError();
 }
 

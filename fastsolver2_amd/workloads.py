@@ -1,20 +1,26 @@
 """Synthetic HODLR workloads beyond the plain Gaussian one.
 
-``conditioned_shift`` calibrates the diagonal shift s of the leaf blocks
-K_i = G_i + s*I (G iid N(0,1), m x m) so that their 2-norm condition number
-is about a target (the mixed-precision workload asks for cond ~ 1e4).  The
-blocks are the device generator's (same seed, stream 0 = K), so the
-calibration sees exactly the matrices the solver will factor; the condition
-number is taken as the maximum over the first few leaves.
+The mixed-precision workload asks for leaf blocks K_i = G_i + s*I (G iid
+N(0,1), m x m) with s set so that cond(A) ~ 1e4.
+  - ``condition_estimate`` estimates cond_2 of the whole HODLR matrix:
+    sigma_max by power iteration on A^T A (FP64 HODLR matvecs), sigma_min
+    by inverse iteration on (A^T A)^-1 = A^-1 A^-T (an FP64 factorization,
+    solve and adjoint solve);
+  - ``conditioned_shift_global`` bisects s on that estimate;
+  - ``conditioned_shift`` is the cheaper per-leaf version (max cond_2 over
+    the first few leaf blocks), used to bracket the global search.
+The blocks are the device generator's (same seed, stream 0 = K), so the
+calibration sees exactly the matrices the solver will factor.
 """
 from __future__ import annotations
 
 import numpy as np
 import torch
 
-from . import generate
+from . import (HodlrPlan, fortran_clone, fortran_empty, generate, lib, matvec, workload_gaussian,
+               _call, _ld, _ptr, _stream)
 
-__all__ = ["conditioned_shift"]
+__all__ = ["conditioned_shift", "condition_estimate", "conditioned_shift_global"]
 
 
 def conditioned_shift(N, leaf, target_cond=1e4, seed=0, system=0, samples=2, iters=30):
@@ -52,3 +58,77 @@ def conditioned_shift(N, leaf, target_cond=1e4, seed=0, system=0, samples=2, ite
             break
     s = 0.5 * (lo + hi)
     return float(s), float(cond(s))
+
+
+def _normalize(x):
+    n = x.norm().item()
+    x.div_(n)
+    return n
+
+
+def condition_estimate(K, U, V, leaf, iters=30, seed=0):
+    """(sigma_max, sigma_min, cond_2) estimates of the HODLR matrix A(K, U, V)."""
+    N, r = U.shape
+    L = int(round(np.log2(N // leaf)))
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    x = fortran_empty(N, 1)
+    x.copy_(torch.randn(N, 1, generator=g, dtype=torch.float64))
+    _normalize(x)
+    lam = 0.0
+    for _ in range(iters):
+        y = matvec(K, U, V, x, leaf)
+        x = matvec(K, U, V, y, leaf, trans=True)
+        lam = _normalize(x)
+    smax = float(np.sqrt(lam))
+    plan = HodlrPlan(N, leaf, L, r, nrhs_max=1)
+    Kf = fortran_clone(K)
+    plan.factor(Kf, U, V)
+    st, where = plan.status()
+    if st != 0:
+        plan.close()
+        return smax, 0.0, float("inf")
+    x = fortran_empty(N, 1)
+    x.copy_(torch.randn(N, 1, generator=g, dtype=torch.float64))
+    _normalize(x)
+    mu = 0.0
+    for _ in range(iters):
+        plan.solve_adjoint(x)
+        plan.solve(x)
+        mu = _normalize(x)
+    plan.close()
+    del Kf
+    smin = float(1.0 / np.sqrt(mu))
+    return smax, smin, smax / smin
+
+
+def conditioned_shift_global(N, leaf, r, target_cond=1e4, seed=0, iters=12, power_iters=30):
+    """Shift s with cond_2(A) ~ target_cond for the Gaussian workload
+    (K_i = G_i + s I, U, V iid N(0,1)/sqrt(r), streams 0..2 of `seed`)."""
+    K, U, V, _ = workload_gaussian(N, leaf, r, 1, seed=seed, shift=0.0)
+    G = fortran_clone(K)
+
+    def cond_at(s):
+        K.copy_(G)
+        _call(lib().hodlr_shift_leaf_diagonal, N, leaf, float(s), _ptr(K), _ld(K), _stream(),
+              where="hodlr_shift_leaf_diagonal")
+        return condition_estimate(K, U, V, leaf, iters=power_iters, seed=seed)[2]
+
+    lo = conditioned_shift(N, leaf, target_cond, seed=seed)[0]
+    c_lo = cond_at(lo)
+    hi = lo
+    c_hi = c_lo
+    while c_hi > target_cond:
+        lo, c_lo = hi, c_hi
+        hi = hi * 1.5 + 1.0
+        c_hi = cond_at(hi)
+    if c_lo <= target_cond:  # the per-leaf shift already meets the target
+        return float(lo), float(c_lo)
+    for _ in range(iters):
+        mid = 0.5 * (lo + hi)
+        c = cond_at(mid)
+        if c > target_cond:
+            lo = mid
+        else:
+            hi, c_hi = mid, c
+    del K, U, V, G
+    return float(hi), float(c_hi)

@@ -111,8 +111,11 @@ typedef struct hodlr_ir_stats {
 } hodlr_ir_stats;
 int hodlr_solve_ir(hodlr_plan plan, const double *K, int64_t ldk, const double *U, int64_t ldu,
                    const double *V, int64_t ldv, const double *B, int64_t ldb, double *X,
-                   int64_t ldx, int nrhs, double tol, int maxit, hodlr_ir_stats *stats,
-                   hipStream_t stream);
+                   int64_t ldx, int nrhs, double tol, int maxit, int flags,
+                   hodlr_ir_stats *stats, hipStream_t stream);
+/* flags for hodlr_solve_ir: X already holds solve(B) (for instance from
+ * hodlr_factor_solve on a copy of B), so the first solve is skipped. */
+#define HODLR_IR_X_GIVEN 1
 
 /* Adjoint (transposed) solve with the stored factorization, in place:
  * G <- A^-T G, nrhs columns (any number; processed in chunks).  The
@@ -150,6 +153,12 @@ int hodlr_matvec(int64_t n, int leaf, int rank, int nrhs,
                  const double *V, int64_t ldv, const double *X, int64_t ldx,
                  double alpha, double beta, double *Y, int64_t ldy,
                  hipStream_t stream);
+/* The same with A^T: y_i = K_i^T x_i + V_i (U^T x - U_i^T x_i). */
+int hodlr_matvec_t(int64_t n, int leaf, int rank, int nrhs,
+                   const double *K, int64_t ldk, const double *U, int64_t ldu,
+                   const double *V, int64_t ldv, const double *X, int64_t ldx,
+                   double alpha, double beta, double *Y, int64_t ldy,
+                   hipStream_t stream);
 
 /* ------------------------------------------- reference-layout operators */
 /* GPU form of hsolve (src/tasks/leaf_solve.cc:8-10, :73-154), same argument

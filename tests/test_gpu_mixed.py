@@ -100,23 +100,44 @@ def test_mixed_ir_reaches_1e12(gpu, orc, N, m, r, s):
 
 
 def test_mixed_ir_conditioned_1e4(gpu):
-    """Leaf blocks G + s I with s calibrated to cond ~ 1e4 (the mixed-precision
-    workload's conditioning): refinement still reaches 1e-12."""
-    from fastsolver2_amd.workloads import conditioned_shift
+    """Leaf shift s calibrated so that cond_2(A) ~ 1e4 for the whole HODLR
+    matrix (the mixed-precision workload's conditioning): refinement from
+    the fused factor+solve first pass reaches 1e-12."""
+    from fastsolver2_amd.workloads import conditioned_shift_global, condition_estimate
     F = gpu
     N, m, r, s = 32768, 512, 64, 4
-    shift, c = conditioned_shift(N, m, 1e4, seed=5)
-    say(f"case: cond~1e4 N={N} m={m} shift={shift:.4f} sampled cond={c:.3g}")
+    shift, c = conditioned_shift_global(N, m, r, 1e4, seed=5)
+    say(f"case: cond~1e4 N={N} m={m} shift={shift:.4f} estimated cond={c:.3g}")
     assert 5e3 < c < 2e4
     K, U, V, B = F.workload_gaussian(N, m, r, s, seed=5, shift=shift, rhs="normal")
+    assert abs(np.log10(condition_estimate(K, U, V, m)[2] / c)) < 0.05
     L = int(round(np.log2(N // m)))
     plan = F.HodlrPlan(N, m, L, r, nrhs_max=s, precision=F.MIXED_IR)
-    plan.factor(K, U, V)
+    X = F.fortran_clone(B)
+    plan.factor_solve(K, U, V, X)
     assert plan.status()[0] == 0
-    X, st = plan.solve_ir(K, U, V, B, tol=1e-12, maxit=20)
+    X, st = plan.solve_ir(K, U, V, B, X=X, tol=1e-12, maxit=20, x_given=True)
     say(f"  iterations {st['iterations']} residual {st['residual']:.2e}")
     assert st["converged"] and st["residual"] <= 1e-12
     plan.close()
+
+
+def test_transposed_matvec_and_condition_estimate(gpu, orc):
+    """A^T x against the dense assembly, and the cond_2 estimate against
+    numpy's SVD of the dense matrix (N = 2048)."""
+    from fastsolver2_amd.workloads import condition_estimate
+    F = gpu
+    N, m, r = 2048, 128, 16
+    say(f"case: A^T x and cond estimate N={N}")
+    K, U, V, B = F.workload_gaussian(N, m, r, 3, seed=12, shift=14.0)
+    Kh, Uh, Vh, Bh = (to_host(t) for t in (K, U, V, B))
+    A = orc.dense_assemble(Kh, Uh, Vh, m)
+    y = to_host(F.matvec(K, U, V, B, m, trans=True))
+    assert relerr(y, A.T @ Bh) <= 1e-13
+    smax, smin, c = condition_estimate(K, U, V, m, iters=60)
+    sv = np.linalg.svd(A, compute_uv=False)
+    say(f"  cond estimate {c:.4g} vs SVD {sv[0] / sv[-1]:.4g}")
+    assert abs(smax / sv[0] - 1) < 1e-3 and abs(smin / sv[-1] - 1) < 5e-2
 
 
 def test_ir_with_fp64_plan(gpu):
