@@ -161,6 +161,10 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
     torch.cuda.set_device(0)
+    # all work on one non-default stream (default priority): the library
+    # captures its refinement solve as a HIP graph on such a stream
+    side = torch.cuda.Stream()
+    torch.cuda.set_stream(side)
     N, m, r, s = args.n, args.leaf, args.rank, args.nrhs
     L = int(round(math.log2(N // m)))
     assert m << L == N

@@ -223,6 +223,12 @@ struct hodlr_plan_s {
   // workspace, column norms
   double* ir_buf = nullptr;
   int64_t ir_elems = 0;
+  // the correction solve on the residual block, captured as a HIP graph
+  // (stream-capture of hodlr_solve; rebuilt when the block or nrhs change,
+  // dropped by every factorization)
+  hipGraphExec_t ir_exec = nullptr;
+  const double* ir_exec_R = nullptr;
+  int ir_exec_nrhs = 0;
   // factored operands
   const double* K = nullptr;
   const double* U = nullptr;
@@ -251,12 +257,20 @@ int palloc(hodlr_plan p, T** out, int64_t count) {
   return HODLR_OK;
 }
 
+void drop_graph(hodlr_plan p) {
+  if (p->ir_exec) (void)hipGraphExecDestroy(p->ir_exec);
+  p->ir_exec = nullptr;
+  p->ir_exec_R = nullptr;
+  p->ir_exec_nrhs = 0;
+}
+
 void plan_free(hodlr_plan p) {
   for (void* a : p->allocs) (void)hipFree(a);
   p->allocs.clear();
   if (p->ir_buf) (void)hipFree(p->ir_buf);
   p->ir_buf = nullptr;
   p->ir_elems = 0;
+  drop_graph(p);
 }
 
 // Leaf solve with the stored leaf factorization (FP64 or the FP32 copy).
@@ -564,6 +578,7 @@ int hodlr_factor(hodlr_plan p, double* K, int64_t ldk, const double* U, int64_t 
   DeviceGuard g(p->d.device);
   p->K = K; p->U = U; p->V = V; p->ldk = ldk; p->ldu = ldu; p->ldv = ldv;
   p->factored = false;
+  drop_graph(p);
   int rc = leaf_phase(p, K, ldk, U, ldu, nullptr, 0, 0, st);
   if (!rc) rc = level_phase(p, nullptr, 0, 0, st);
   if (!rc) p->factored = true;
@@ -580,6 +595,7 @@ int hodlr_factor_solve(hodlr_plan p, double* K, int64_t ldk, const double* U, in
   DeviceGuard g(p->d.device);
   p->K = K; p->U = U; p->V = V; p->ldk = ldk; p->ldu = ldu; p->ldv = ldv;
   p->factored = false;
+  drop_graph(p);
   int rc = leaf_phase(p, K, ldk, U, ldu, B, ldb, nrhs, st);
   if (!rc) rc = level_phase(p, B, ldb, nrhs, st);
   if (!rc) p->factored = true;
@@ -704,6 +720,37 @@ int hodlr_solve_adjoint(hodlr_plan p, double* G, int64_t ldg, int nrhs, hipStrea
   return HODLR_OK;
 }
 
+// R <- A^-1 R for the refinement's residual block: replays a captured HIP
+// graph of hodlr_solve when the stream can be captured (not the legacy null
+// stream, profiling off), else runs it eagerly.
+static int correction_solve(hodlr_plan p, double* R, int64_t ldr, int nrhs, hipStream_t st) {
+  if (p->ir_exec && (p->ir_exec_R != R || p->ir_exec_nrhs != nrhs)) drop_graph(p);
+  if (!p->ir_exec && st != nullptr && !g_prof.on) {
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+      const int rc = hodlr_solve(p, R, ldr, nrhs, st);
+      hipGraph_t graph = nullptr;
+      const hipError_t e = hipStreamEndCapture(st, &graph);
+      if (rc == HODLR_OK && e == hipSuccess && graph) {
+        if (hipGraphInstantiate(&p->ir_exec, graph, nullptr, nullptr, 0) != hipSuccess)
+          p->ir_exec = nullptr;
+      }
+      if (graph) (void)hipGraphDestroy(graph);
+      (void)hipGetLastError();
+      if (p->ir_exec) {
+        p->ir_exec_R = R;
+        p->ir_exec_nrhs = nrhs;
+      }
+    } else {
+      (void)hipGetLastError();
+    }
+  }
+  if (p->ir_exec) {
+    HIP_TRY(hipGraphLaunch(p->ir_exec, st));
+    return HODLR_OK;
+  }
+  return hodlr_solve(p, R, ldr, nrhs, st);
+}
+
 int hodlr_solve_ir(hodlr_plan p, const double* K, int64_t ldk, const double* U, int64_t ldu,
                    const double* V, int64_t ldv, const double* B, int64_t ldb, double* X,
                    int64_t ldx, int nrhs, double tol, int maxit, int flags,
@@ -772,7 +819,7 @@ int hodlr_solve_ir(hodlr_plan p, const double* K, int64_t ldk, const double* U, 
       break;
     }
     if (it == maxit || res != res) break;
-    rc = hodlr_solve(p, R, N, nrhs, st);
+    rc = correction_solve(p, R, N, nrhs, st);
     if (rc) return rc;
     HIP_TRY(hodlr::launch_add_cols(N, nrhs, R, N, X, ldx, st));
   }
