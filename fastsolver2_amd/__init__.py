@@ -22,7 +22,7 @@ __all__ = [
     "lib", "lib_path", "available", "fortran_empty", "fortran", "HodlrPlan", "hsolve",
     "gemm_red", "gemm_bro", "node_solve", "matvec", "generate", "workload_gaussian",
     "dgesv_batched", "HodlrError", "grad", "HodlrSolveFunction", "hodlr_solve", "fortran_clone",
-    "FP64", "MIXED_IR", "IRStats",
+    "FP64", "MIXED_IR", "IRStats", "HodlrBatch",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -84,6 +84,15 @@ def lib():
         "hodlr_factor_solve": ([vp, vp, i64, vp, i64, vp, i64, vp, i64, ci, vp], ci),
         "hodlr_plan_status": ([vp, vp, C.POINTER(ci)], ci),
         "hodlr_solve_adjoint": ([vp, vp, i64, ci, vp], ci),
+        "hodlr_batch_create": ([C.POINTER(vp), ci, vp], ci),
+        "hodlr_batch_destroy": ([vp], ci),
+        "hodlr_batch_size": ([vp], ci),
+        "hodlr_batch_plan": ([vp, ci], vp),
+        "hodlr_batch_factor_solve": ([vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp], ci),
+        "hodlr_batch_status": ([vp, vp, C.POINTER(ci), C.POINTER(ci)], ci),
+        "hodlr_batch_set_tuning": ([vp, ci, ci], ci),
+        "hodlr_batch_get_tuning": ([vp, C.POINTER(ci), C.POINTER(ci)], ci),
+        "hodlr_batch_tune": ([vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(dbl)], ci),
         "hodlr_solve_ir": ([vp, vp, i64, vp, i64, vp, i64, vp, i64, vp, i64, ci, dbl, ci, ci,
                             C.POINTER(IRStats), vp], ci),
         "hodlr_matvec_t": ([i64, ci, ci, ci, vp, i64, vp, i64, vp, i64, vp, i64, dbl, dbl, vp, i64,
@@ -274,6 +283,87 @@ class HodlrPlan:
         w = C.c_int(-1)
         st = lib().hodlr_plan_status(self._h, _stream(), C.byref(w))
         return st, w.value
+
+
+class HodlrBatch:
+    """Independent HODLR systems solved together (include/hodlr.h, batches):
+    the batch is captured once as one HIP graph over concurrent lanes and
+    replayed while the operand pointers stay the same.  `systems` is a list
+    of dicts with keys n, leaf, levels, rank and optionally nrhs_max."""
+
+    def __init__(self, systems, device=None, precision=FP64):
+        if not torch.cuda.is_available():
+            raise RuntimeError("HodlrBatch needs a GPU")
+        dev = torch.cuda.current_device() if device is None else int(device)
+        n = len(systems)
+        descs = (_Desc * n)()
+        for i, sy in enumerate(systems):
+            descs[i] = _Desc(sy["n"], sy["leaf"], sy["levels"], sy["rank"], sy.get("nrhs_max", 1),
+                             precision, dev, 0)
+        h = C.c_void_p()
+        _call(lib().hodlr_batch_create, C.byref(h), n, C.cast(descs, C.c_void_p),
+              where="hodlr_batch_create")
+        self._h = h
+        self.n = n
+        self.systems = [dict(sy) for sy in systems]
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().hodlr_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _arrays(self, Ks, Us, Vs, Bs):
+        n = self.n
+        if not (len(Ks) == len(Us) == len(Vs) == len(Bs) == n):
+            raise ValueError("one K, U, V, B per system")
+        P = C.c_void_p * n
+        L = C.c_int64 * n
+        I = C.c_int * n
+        for t in list(Ks) + list(Us) + list(Vs) + list(Bs):
+            _check(t, "operand")
+        B2 = [b if b.dim() == 2 else b.view(-1, 1) for b in Bs]
+        arr = (P(*[t.data_ptr() for t in Ks]), L(*[_ld(t) for t in Ks]),
+               P(*[t.data_ptr() for t in Us]), L(*[_ld(t) for t in Us]),
+               P(*[t.data_ptr() for t in Vs]), L(*[_ld(t) for t in Vs]),
+               P(*[t.data_ptr() for t in B2]), L(*[_ld(t) for t in B2]),
+               I(*[t.shape[1] for t in B2]))
+        return [C.cast(a, C.c_void_p) for a in arr], arr
+
+    def factor_solve(self, Ks, Us, Vs, Bs):
+        """Every system: K_i overwritten by its leaf LU, B_i by the solution."""
+        ptrs, keep = self._arrays(Ks, Us, Vs, Bs)
+        _call(lib().hodlr_batch_factor_solve, self._h, *ptrs, _stream(),
+              where="hodlr_batch_factor_solve")
+        return Bs
+
+    def tune(self, Ks, Us, Vs, Bs):
+        """Autotune (lanes, tile) on scratch operands; returns (lanes, tile, ms)."""
+        ptrs, keep = self._arrays(Ks, Us, Vs, Bs)
+        best = C.c_double()
+        _call(lib().hodlr_batch_tune, self._h, *ptrs, _stream(), C.byref(best),
+              where="hodlr_batch_tune")
+        lanes, tile = self.get_tuning()
+        return lanes, tile, best.value
+
+    def set_tuning(self, lanes, tile=0):
+        _call(lib().hodlr_batch_set_tuning, self._h, lanes, tile, where="hodlr_batch_set_tuning")
+
+    def get_tuning(self):
+        l, t = C.c_int(), C.c_int()
+        _call(lib().hodlr_batch_get_tuning, self._h, C.byref(l), C.byref(t),
+              where="hodlr_batch_get_tuning")
+        return l.value, t.value
+
+    def status(self):
+        sy, w = C.c_int(-1), C.c_int(-1)
+        st = lib().hodlr_batch_status(self._h, _stream(), C.byref(sy), C.byref(w))
+        return st, sy.value, w.value
 
 
 # ---------------------------------------------------------------- operators

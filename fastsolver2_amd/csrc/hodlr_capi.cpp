@@ -183,6 +183,8 @@ int core_factor_solve(int nnode, int r, const CoreBufs& c, int nh, hipStream_t s
 
 }  // namespace
 
+bool hodlr::profiling_enabled() { return g_prof.on; }
+
 // =====================================================================
 // plans
 // =====================================================================

@@ -67,6 +67,13 @@ hipError_t launch_getrs(int n, const double* LU, int64_t lda, int64_t bstride, i
                         const int* perm, const double* dinv, const ColSeg& X,
                         int64_t xbstride, int ncols, hipStream_t st);
 // Transposed solve X_b <- P^T L^-T U^-T X_b (dgetrs TRANS='T').
+// Column tile of the register-resident solve for blocks up to 512: 1, 2 or
+// 5 MFMA tiles (16, 32, 80 right-hand sides) per workgroup; 0 = chosen by
+// the number of columns.  Process-wide (set by the batch autotuner).
+void set_getrs_tile(int nct);
+int getrs_tile();
+// true while per-launch profiling is on (no stream capture then)
+bool profiling_enabled();
 hipError_t launch_getrs_t(int n, const double* LU, int64_t lda, int64_t bstride, int batch,
                           const int* perm, const double* dinv, const ColSeg& X,
                           int64_t xbstride, int ncols, hipStream_t st);

@@ -1334,6 +1334,10 @@ static void launch_getrs_reg(int n, const T* LU, int64_t lda, int64_t bstride, i
                      bstride, n, perm, dinv, X, xbstride, ncols, nchunk);
 }
 
+int g_getrs_tile = 0;
+void set_getrs_tile(int nct) { g_getrs_tile = nct; }
+int getrs_tile() { return g_getrs_tile; }
+
 template <bool TRANS>
 static hipError_t launch_getrs_impl(int n, const double* LU, int64_t lda, int64_t bstride,
                                     int batch, const int* perm, const double* dinv,
@@ -1342,9 +1346,13 @@ static hipError_t launch_getrs_impl(int n, const double* LU, int64_t lda, int64_
   if (batch <= 0 || n <= 0 || ncols <= 0) return hipSuccess;
   if (n > kMaxBlock) return hipErrorInvalidValue;
   if (n <= 512) {
-    if (ncols <= 16)
+    // column tile: 16, 32 or 80 right-hand sides per workgroup, by width
+    // unless a tuner fixed it
+    int nct = ncols <= 16 ? 1 : (ncols <= 32 ? 2 : 5);
+    if (g_getrs_tile == 1 || g_getrs_tile == 2 || g_getrs_tile == 5) nct = g_getrs_tile;
+    if (nct == 1)
       launch_getrs_reg<1, TRANS, double>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
-    else if (ncols <= 32)
+    else if (nct == 2)
       launch_getrs_reg<2, TRANS, double>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
     else
       launch_getrs_reg<5, TRANS, double>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);

@@ -20,7 +20,8 @@ import torch
 from . import (HodlrPlan, fortran_clone, fortran_empty, generate, lib, matvec, workload_gaussian,
                _call, _ld, _ptr, _stream)
 
-__all__ = ["conditioned_shift", "condition_estimate", "conditioned_shift_global"]
+__all__ = ["conditioned_shift", "condition_estimate", "conditioned_shift_global", "batch_systems",
+           "batch_operands"]
 
 
 def conditioned_shift(N, leaf, target_cond=1e4, seed=0, system=0, samples=2, iters=30):
@@ -132,3 +133,43 @@ def conditioned_shift_global(N, leaf, r, target_cond=1e4, seed=0, iters=12, powe
             hi, c_hi = mid, c
     del K, U, V, G
     return float(hi), float(c_hi)
+
+
+def batch_systems(nsys=256, seed=0, n_range=(4096, 65536), leaves=(128, 256, 512),
+                  ranks=(16, 100)):
+    """The uneven batch: per system N log-uniform in n_range rounded to a
+    multiple of the leaf size, leaf size drawn from `leaves`, rank uniform in
+    `ranks` (inclusive); numpy's Philox stream of `seed` draws them.  A
+    system of P leaves is padded to 2^ceil(log2 P) leaves (identity leaf
+    blocks, zero U, V and B rows) so that its tree is balanced; `n` is the
+    padded size, `n_real` the system's own."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    out = []
+    for _ in range(nsys):
+        nraw = float(np.exp(rng.uniform(np.log(n_range[0]), np.log(n_range[1]))))
+        m = int(leaves[int(rng.integers(len(leaves)))])
+        r = int(rng.integers(ranks[0], ranks[1] + 1))
+        P = max(2, int(round(nraw / m)))
+        L = int(np.ceil(np.log2(P)))
+        out.append(dict(n=m << L, n_real=P * m, leaf=m, levels=L, rank=r, nrhs_max=1))
+    return out
+
+
+def batch_operands(systems, seed=0, pad=0):
+    """Device operands of every system (generators of the Gaussian workload
+    with the system index in the counter: K_i = G_i + m I, U, V iid
+    N(0,1)/sqrt(r), one right-hand side iid U[0,1)); padded rows carry
+    identity leaves and zero U, V, B."""
+    Ks, Us, Vs, Bs = [], [], [], []
+    for k, sy in enumerate(systems):
+        n, m, r, nr = sy["n"], sy["leaf"], sy["rank"], sy["n_real"]
+        K, U, V, B = workload_gaussian(n, m, r, 1, seed=seed, system=k, pad=pad)
+        if nr < n:
+            K[nr:, :].zero_()
+            rows = torch.arange(nr, n, device=K.device)
+            K[rows, rows % m] = 1.0
+            U[nr:, :].zero_()
+            V[nr:, :].zero_()
+            B[nr:, :].zero_()
+        Ks.append(K); Us.append(U); Vs.append(V); Bs.append(B)
+    return Ks, Us, Vs, Bs

@@ -138,6 +138,43 @@ int hodlr_grad(int64_t n, int leaf, int rank, int nrhs, const double *U, int64_t
                const double *Lambda, int64_t ldl, double *dK, int64_t lddk,
                double *dU, int64_t lddu, double *dV, int64_t lddv, hipStream_t stream);
 
+/* ------------------------------------------------------------- batches */
+/* Many independent HODLR systems (uneven N, leaf size, rank, nrhs), one
+ * plan each.  hodlr_batch_factor_solve enqueues every system's fused factor
+ * + solve (hodlr_factor_solve semantics, per system i: K[i] overwritten by
+ * its leaf LU, B[i] by the solution).  On a capturable stream (not the
+ * legacy null stream) the whole batch is captured once as ONE HIP graph --
+ * the stream forks into `lanes` internal streams that run systems
+ * concurrently and joins back -- and calls with the same operand pointers
+ * replay it.  No reference counterpart (the reference solves one system
+ * per run); it extends the driver of spmd_driver/solver_irregular_part.cc
+ * to independent systems. */
+typedef struct hodlr_batch_s *hodlr_batch;
+int hodlr_batch_create(hodlr_batch *batch, int nsys, const hodlr_desc *descs);
+int hodlr_batch_destroy(hodlr_batch batch);
+int hodlr_batch_size(hodlr_batch batch);
+/* The plan of system i (owned by the batch; for hodlr_solve and friends). */
+hodlr_plan hodlr_batch_plan(hodlr_batch batch, int i);
+int hodlr_batch_factor_solve(hodlr_batch batch, double *const *K, const int64_t *ldk,
+                             const double *const *U, const int64_t *ldu,
+                             const double *const *V, const int64_t *ldv,
+                             double *const *B, const int64_t *ldb, const int *nrhs,
+                             hipStream_t stream);
+/* Synchronizes; first failing system in *sys, its hodlr_plan_status where. */
+int hodlr_batch_status(hodlr_batch batch, hipStream_t stream, int *sys, int *where);
+/* Launch shape knobs: lanes (1..16 concurrent streams) and the leaf-solve
+ * column tile (0 = by width, else 1, 2 or 5 MFMA tiles per workgroup). */
+int hodlr_batch_set_tuning(hodlr_batch batch, int lanes, int tile);
+int hodlr_batch_get_tuning(hodlr_batch batch, int *lanes, int *tile);
+/* Autotuner: times a graph replay of the batch for every (lanes, tile)
+ * pair on the given operands and keeps the fastest (*best_ms).  The
+ * operands are overwritten many times: pass scratch copies. */
+int hodlr_batch_tune(hodlr_batch batch, double *const *K, const int64_t *ldk,
+                     const double *const *U, const int64_t *ldu,
+                     const double *const *V, const int64_t *ldv,
+                     double *const *B, const int64_t *ldb, const int *nrhs,
+                     hipStream_t stream, double *best_ms);
+
 /* Synchronizes `stream` and reports the numerical status of the last factor:
  * HODLR_OK, HODLR_SINGULAR_LEAF (*where = leaf index) or HODLR_SINGULAR_CORE
  * (*where = level * 2^20 + node).  Replaces the reference's
