@@ -130,6 +130,59 @@ def bench_config4(F, torch, W, log):
     return out
 
 
+def bench_batch(F, torch, W, K_, log, nsys=256):
+    """256 independent systems (N log-uniform in [4096, 65536] rounded to the
+    leaf, leaf in {128, 256, 512}, rank uniform in [16, 100], 1 RHS U[0,1),
+    seed 0), padded to balanced trees; autotuned, one graph per batch."""
+    from fastsolver2_amd.workloads import batch_operands, batch_systems
+    systems = batch_systems(nsys, seed=0)
+    Ks, Us, Vs, Bs = batch_operands(systems, seed=0, pad=LD_PAD)
+    K0 = [F.fortran_clone(k) for k in Ks]
+    B0 = [F.fortran_clone(x) for x in Bs]
+    b = F.HodlrBatch(systems)
+    t0 = time.perf_counter()
+    lanes, tile, tune_ms = b.tune(Ks, Us, Vs, Bs)
+    tune_s = time.perf_counter() - t0
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ts = []
+    for i in range(W + K_):
+        for k, k0 in zip(Ks, K0):
+            k.copy_(k0)
+        for x, x0 in zip(Bs, B0):
+            x.copy_(x0)
+        torch.cuda.synchronize()
+        ev0.record()
+        b.factor_solve(Ks, Us, Vs, Bs)
+        ev1.record()
+        torch.cuda.synchronize()
+        if i >= W:
+            ts.append(ev0.elapsed_time(ev1))
+    st = b.status()
+    if st[0] != 0:
+        log(f"batch status {st}")
+    worst = 0.0
+    for sy, K, U, V, X, B in zip(systems, K0, Us, Vs, Bs, B0):
+        R = F.matvec(K, U, V, X, sy["leaf"], alpha=-1.0)
+        R.add_(B)
+        worst = max(worst, (R.abs().max() / B.abs().max()).item())
+    ms = statistics.median(ts)
+    fl = sum(flops_alg(sy["n"], sy["leaf"], sy["rank"], 1) for sy in systems)
+    out = {"workload": f"{nsys} systems, N log-uniform [4096, 65536] (sum {sum(s['n_real'] for s in systems)}"
+                       f" rows, {sum(s['n'] for s in systems)} padded), leaf {{128,256,512}}, "
+                       "rank U[16,100], 1 RHS, seed 0, one HIP graph",
+           "ms": round(ms, 3), "systems_per_s": round(nsys * 1000.0 / ms, 1),
+           "batches_per_s": round(1000.0 / ms, 3), "gflops_alg": round(fl / ms / 1e6, 1),
+           "tuned": {"lanes": lanes, "tile": tile, "tune_replay_ms": round(tune_ms, 3),
+                     "tune_seconds": round(tune_s, 2)},
+           "max_relative_residual": worst}
+    log(f"batch of {nsys}: {ms:.2f} ms ({nsys * 1000.0 / ms:.0f} systems/s, {fl / ms / 1e6:.0f} GFLOP/s), "
+        f"tuned lanes={lanes} tile={tile}, worst residual {worst:.2e}")
+    b.close()
+    del Ks, Us, Vs, Bs, K0, B0
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +191,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the uneven-batch leg")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only copy BYTES device-to-device 4 times (calibrates HBM counters)")
     ap.add_argument("--config4", action=argparse.BooleanOptionalAction, default=True,
                     help="mixed precision at N=2^20, cond~1e4 (default on)")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -161,6 +217,15 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
     torch.cuda.set_device(0)
+    if args.copy_probe:
+        n = args.copy_probe // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: 4 device copies of {n * 8} bytes")
+        return
     # all work on one non-default stream (default priority): the library
     # captures its refinement solve as a HIP graph on such a stream
     side = torch.cuda.Stream()
@@ -351,6 +416,10 @@ def main():
         aplan.close()
         del aK, aK0, aU, aV, aB0, aX, Lam, gK, gU, gV
 
+    batch = None
+    if not args.no_batch:
+        batch = bench_batch(F, torch, W, K_, log)
+
     cpu = None
     if not args.no_cpu:
         nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -389,6 +458,7 @@ def main():
         "mixed_ir": mixed,
         "adjoint_ms": adj["adjoint_ms"] if adj else None,
         "adjoint": adj,
+        "batch": batch,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
