@@ -23,6 +23,24 @@ FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared",
          "-Wall", "-Wno-unused-result"]
 
 
+def build_facade_test(verbose: bool = False) -> str:
+    """tests/cpp/test_facade (the C++ facade tests), linked against the library."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "test_facade.cpp")
+    out = os.path.join(root, "tests", "cpp", "test_facade")
+    if not os.path.exists(src):
+        return ""
+    cmd = [HIPCC, "-O2", "-std=c++17", "-I", os.path.join(root, "include"), src, "-L", HERE,
+           "-lhodlr_amd", "-Wl,-rpath,$ORIGIN/../../fastsolver2_amd", "-o", out + ".tmp"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("facade test build failed:\n" + res.stdout + res.stderr)
+    os.replace(out + ".tmp", out)
+    return out
+
+
 def build(verbose: bool = False) -> str:
     srcs = [os.path.join(CSRC, s) for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
     tmp = LIB + ".tmp"
@@ -42,3 +60,4 @@ def build(verbose: bool = False) -> str:
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_facade_test(verbose="-v" in sys.argv))
