@@ -154,6 +154,7 @@ struct CoreBufs {
   int* perm;
   int* info;
   double* dinv;
+  int* general = nullptr;  // per-core flags of the structured LU (null: general LU)
 };
 
 // Core step shared by every level: S (already assembled) is factored in
@@ -165,7 +166,12 @@ int core_factor_solve(int nnode, int r, const CoreBufs& c, int nh, hipStream_t s
   const double blk = 8.0 * n2 * n2 * nnode;
   {
     ProfScope ps(HODLR_PROF_GETRF_CORE, st, nnode * lu_flops(n2), 2.0 * blk);
-    HIP_TRY(hodlr::launch_getrf(n2, c.S, n2, (int64_t)n2 * n2, nnode, c.ipiv, c.perm, c.info, st));
+    if (c.general)
+      HIP_TRY(hodlr::launch_getrf_core(n2, c.S, n2, (int64_t)n2 * n2, nnode, c.ipiv, c.perm, c.info,
+                                       c.general, st));
+    else
+      HIP_TRY(hodlr::launch_getrf(n2, c.S, n2, (int64_t)n2 * n2, nnode, c.ipiv, c.perm, c.info,
+                                  st));
   }
   {
     ProfScope ps(HODLR_PROF_TRTRI, st, 0.0, blk);
@@ -209,6 +215,7 @@ struct hodlr_plan_s {
   int* core_perm = nullptr;
   int* core_info = nullptr;
   double* core_dinv = nullptr;
+  int* core_general = nullptr;  // flags of the structured core LU (largest level)
   std::vector<int64_t> s_off, p_off, d_off, i_off;
   // workspace: per-leaf projections (two, ping-pong), accumulators, core RHS
   double* Pb[3] = {nullptr, nullptr, nullptr};
@@ -299,6 +306,7 @@ CoreBufs core_level(hodlr_plan p, int t) {
   c.perm = p->core_perm + p->p_off[t];
   c.info = p->core_info + p->i_off[t];
   c.dinv = p->core_dinv + p->d_off[t];
+  c.general = p->core_general;
   return c;
 }
 
@@ -545,6 +553,7 @@ int hodlr_plan_create(hodlr_plan* plan, const hodlr_desc* desc) {
   rc = rc ? rc : palloc(p, &p->core_perm, po);
   rc = rc ? rc : palloc(p, &p->core_info, io > 0 ? io : 1);
   rc = rc ? rc : palloc(p, &p->core_dinv, dof);
+  rc = rc ? rc : palloc(p, &p->core_general, p->L > 0 ? ((int64_t)1 << (p->L - 1)) : 0);
   // per-leaf projections / accumulators and core RHS for the widest
   // operand: [Y | B] with nrhs_max columns, or a solve chunk of that width
   p->ncmax = r + (p->smax > 0 ? p->smax : 1);

@@ -59,6 +59,11 @@ hipError_t launch_colmax(int64_t rows, int cols, const double* A, int64_t lda, d
                          hipStream_t st);
 hipError_t launch_add_cols(int64_t rows, int cols, const double* X, int64_t ldx, double* Y,
                            int64_t ldy, hipStream_t st);
+// LU of HODLR cores S = [I, A; B, I] (n2 = 2r): the block structure when
+// max|B| <= 1, else the general kernel (per core; `general` is batch ints of
+// workspace).  Same factors, pivots and INFO as launch_getrf.
+hipError_t launch_getrf_core(int n2, double* S, int64_t lda, int64_t bstride, int batch, int* ipiv,
+                             int* perm, int* info, int* general, hipStream_t st);
 hipError_t launch_trtri_diag(int n, const double* LU, int64_t lda, int64_t bstride,
                              int batch, double* dinv, hipStream_t st);
 // X_b <- U^-1 L^-1 P X_b for batch item b, columns [0, ncols) of X, where

@@ -712,7 +712,10 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
 template <int NB>
 __global__ void __launch_bounds__(64)
 k_getrf_w64(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
-            int* __restrict__ ipiv0, int* __restrict__ perm0, int* __restrict__ info0) {
+            int* __restrict__ ipiv0, int* __restrict__ perm0, int* __restrict__ info0,
+            const int* __restrict__ only) {
+  // `only` (optional): factor block b only where only[b] != 0
+  if (only && !only[blockIdx.x]) return;
   constexpr int R = 2;  // rows per lane
   __shared__ double prow[NB];
   __shared__ int pivs[NB];
@@ -972,7 +975,7 @@ hipError_t launch_getrf(int n, double* A, int64_t lda, int64_t bstride, int batc
   if (n > kMaxBlock) return hipErrorInvalidValue;
   if (n <= 128) {
     hipLaunchKernelGGL(k_getrf_w64<32>, dim3(batch), dim3(64), 0, st, A, lda, bstride, n, ipiv,
-                       perm, info);
+                       perm, info, (const int*)nullptr);
     return hipGetLastError();
   }
   if (n <= 256) {
@@ -1010,6 +1013,153 @@ hipError_t launch_getrf(int n, double* A, int64_t lda, int64_t bstride, int batc
   }
   return hipGetLastError();
 }
+
+// =====================================================================
+// LU of the HODLR cores S = [I, A; B, I] (2r x 2r, r <= 64), one wave each.
+// With max|B| <= 1 partial pivoting takes the diagonal in each of the
+// first r columns (ties go to the first index, as idamax), so
+//   L = [I, 0; P2 B, L2],  U = [I, A; 0, U2],  L2 U2 = P2 (I - B A)
+// -- the same factors dgetrf produces (its first 64-column panel is the
+// identity block, its trailing update the GEMM I - B A).  The kernel forms
+// C = I - B A on MFMA, factors C with implicit pivoting (row i of C in lane
+// i, 64 columns fully in registers) and writes S's LU in LAPACK layout.
+// Cores with max|B| > 1 or a NaN in B are flagged in `general` and left to
+// k_getrf_w64.
+// =====================================================================
+__global__ void __launch_bounds__(64)
+k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
+             int* __restrict__ ipiv0, int* __restrict__ perm0, int* __restrict__ info0,
+             int* __restrict__ general) {
+  __shared__ double Cs[64 * 65];
+  __shared__ double prow[64];
+  const int b = blockIdx.x, lane = threadIdx.x;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int r = n2 >> 1;
+  double* S = A0 + (int64_t)b * bstride;
+  // (1) max |B| (B = S(r:2r, 0:r)); a NaN makes the key maximal
+  unsigned long long mk = 0ULL;
+  for (int e = lane; e < r * r; e += 64) {
+    const double v = S[(int64_t)(r + e % r) + (int64_t)(e / r) * lda];
+    const unsigned long long k = (v == v) ? (unsigned long long)__double_as_longlong(fabs(v))
+                                          : ~0ULL;
+    mk = k > mk ? k : mk;
+  }
+  mk = wave_max_u64(mk);
+  const bool fallback = mk > (unsigned long long)__double_as_longlong(1.0);
+  if (lane == 0) general[b] = fallback ? 1 : 0;
+  if (fallback) return;
+  // (2) C = I - B A on MFMA, 16 x 16 tiles into LDS (Cs[row*65 + col])
+  const int nt = (r + 15) >> 4;
+  for (int t = 0; t < nt * nt; ++t) {
+    const int tm = t % nt, tn = t / nt;
+    d4 acc = d4zero();
+    for (int k0 = 0; k0 < r; k0 += 4) {
+      const int k = k0 + lk;
+      const int ar = tm * 16 + lr, bc = tn * 16 + lr;
+      const double af = (k < r && ar < r) ? S[(int64_t)(r + ar) + (int64_t)k * lda] : 0.0;
+      const double bf = (k < r && bc < r) ? S[(int64_t)k + (int64_t)(r + bc) * lda] : 0.0;
+      acc = mfma(af, bf, acc);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = tm * 16 + crow(lane, v), col = tn * 16 + lr;
+      if (row < r && col < r) Cs[row * 65 + col] = (row == col ? 1.0 : 0.0) - acc[v];
+    }
+  }
+  __syncthreads();
+  // (3) LU of C with implicit pivoting: lane i keeps row i
+  double p[64];
+#pragma unroll
+  for (int c = 0; c < 64; ++c) p[c] = (lane < r && c < r) ? Cs[lane * 65 + c] : 0.0;
+  int pos = lane;
+  bool live = lane < r;
+  int info = 0;  // lane 0
+  int mypiv = 0;  // lane jj: pivot position chosen at step jj
+#pragma unroll
+  for (int jj = 0; jj < 64; ++jj) {
+    if (jj < r) {
+      const unsigned long long key = live ? Elem<double>::key(p[jj]) : 0ULL;
+      const int cpos = key ? pos : INT_MAX;
+      const unsigned long long wmax = wave_max_u64(key);
+      const unsigned long long bal = __ballot(key == wmax);
+      int wl;
+      if (__popcll(bal) == 1) {
+        wl = __ffsll((long long)bal) - 1;
+      } else {
+        int qq = (key == wmax) ? cpos : INT_MAX;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) qq = min(qq, __shfl_xor(qq, off, 64));
+        wl = __ffsll((long long)__ballot(key == wmax && cpos == qq)) - 1;
+      }
+      const int bpos = __builtin_amdgcn_readlane(pos, wl);
+      if (lane == wl) {
+#pragma unroll
+        for (int c = 0; c < 64; ++c)
+          if (c >= jj) prow[c] = p[c];
+      }
+      if (lane == jj) mypiv = bpos;
+      __syncthreads();
+      const double pv = prow[jj];
+      if (lane == wl) {
+        live = false;
+        pos = jj;
+      } else if (pos == jj) {
+        pos = bpos;
+      }
+      if (pv != 0.0) {
+        if (live) {
+          const double l = (fabs(pv) >= DBL_MIN) ? p[jj] * (1.0 / pv) : p[jj] / pv;
+          p[jj] = l;
+#pragma unroll
+          for (int c = 0; c < 64; ++c)
+            if (c > jj) p[c] -= l * prow[c];
+        }
+      } else if (lane == 0 && info == 0) {
+        info = r + jj + 1;
+      }
+      __syncthreads();
+    }
+  }
+  // (4) S's LU in LAPACK layout: rows r + pos get this lane's L2\U2 row and
+  // its B row (the interchanges apply to the first r columns too)
+  if (lane < r) {
+#pragma unroll
+    for (int c = 0; c < 64; ++c)
+      if (c < r) Cs[lane * 65 + c] = S[(int64_t)(r + lane) + (int64_t)c * lda];
+  }
+  __syncthreads();
+  if (lane < r) {
+    double* row = S + (int64_t)(r + pos);
+#pragma unroll
+    for (int c = 0; c < 64; ++c) {
+      if (c < r) {
+        row[(int64_t)c * lda] = Cs[lane * 65 + c];
+        row[(int64_t)(r + c) * lda] = p[c];
+      }
+    }
+  }
+  int* ipiv = ipiv0 + (int64_t)b * n2;
+  int* perm = perm0 + (int64_t)b * n2;
+  if (lane < r) {
+    ipiv[lane] = lane + 1;
+    perm[lane] = lane;
+    ipiv[r + lane] = r + mypiv + 1;
+    perm[r + pos] = r + lane;
+  }
+  if (lane == 0) info0[b] = info;
+}
+
+hipError_t launch_getrf_core(int n2, double* S, int64_t lda, int64_t bstride, int batch, int* ipiv,
+                             int* perm, int* info, int* general, hipStream_t st) {
+  if (batch <= 0 || n2 <= 0) return hipSuccess;
+  if (n2 > 128 || (n2 & 1)) return launch_getrf(n2, S, lda, bstride, batch, ipiv, perm, info, st);
+  hipLaunchKernelGGL(k_getrf_core, dim3(batch), dim3(64), 0, st, S, lda, bstride, n2, ipiv, perm,
+                     info, general);
+  hipLaunchKernelGGL(k_getrf_w64<32>, dim3(batch), dim3(64), 0, st, S, lda, bstride, n2, ipiv, perm,
+                     info, (const int*)general);
+  return hipGetLastError();
+}
+
 
 // =====================================================================
 // Inverses of the 32x32 diagonal blocks of L (unit lower) and U (upper),

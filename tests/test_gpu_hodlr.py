@@ -186,6 +186,27 @@ def test_padded_leading_dimension_is_bitwise_identical(gpu):
     assert np.array_equal(outs[0][0], outs[1][0])
     assert np.array_equal(outs[0][1], outs[1][1])
 
+
+@pytest.mark.parametrize("scale", [1.0, 200.0])
+def test_core_structured_and_general_lu(gpu, orc, scale):
+    """Cores S = [I, A; B, I] take the structured LU when max|B| <= 1 (plain
+    workload) and the general pivoted LU otherwise (U scaled by 200 makes
+    |B| > 1 at every level); both match the oracle to 1e-10."""
+    F = gpu
+    N, m, r, s = 2048, 128, 16, 2
+    say(f"case: core LU path, U scale {scale}")
+    K, U, V, B = F.workload_gaussian(N, m, r, s, seed=13)
+    U.mul_(scale)
+    Kh, Uh, Vh, Bh = (to_host(t) for t in (K, U, V, B))
+    ref = orc.hodlr_solve(Kh, Uh, Vh, Bh, m)
+    plan = F.HodlrPlan(N, m, 4, r, nrhs_max=s)
+    plan.factor_solve(K, U, V, B)
+    assert plan.status()[0] == 0
+    e = relerr(to_host(B), ref)
+    say(f"  rel err {e:.2e}")
+    assert e <= RTOL
+    plan.close()
+
 def test_factor_then_solve_many_rhs(gpu, orc):
     """hodlr_factor + hodlr_solve with more columns than nrhs_max (column
     batches), against the oracle."""
