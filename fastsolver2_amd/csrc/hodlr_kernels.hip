@@ -49,6 +49,9 @@ __device__ __forceinline__ unsigned long long readlane_u64(unsigned long long v,
   const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(v >> 32), l);
   return ((unsigned long long)hi << 32) | lo;
 }
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  return __longlong_as_double((long long)readlane_u64((unsigned long long)__double_as_longlong(v), l));
+}
 // Maximum over the 64 lanes: quad butterflies (xor 1, xor 2), half-row and
 // row mirrors leave every lane of a 16-lane row with the row maximum; the
 // four rows are combined through readlane.
@@ -1030,34 +1033,41 @@ __global__ void __launch_bounds__(64)
 k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
              int* __restrict__ ipiv0, int* __restrict__ perm0, int* __restrict__ info0,
              int* __restrict__ general) {
-  __shared__ double Cs[64 * 65];
-  __shared__ double prow[64];
+  __shared__ double Cs[64 * 65];  // C = I - B A, [row*65 + col]
+  __shared__ double Bs[64 * 65];  // B, [row*65 + k]
+  __shared__ double As[64 * 65];  // A, [k*65 + col]
   const int b = blockIdx.x, lane = threadIdx.x;
   const int lr = lane & 15, lk = lane >> 4;
   const int r = n2 >> 1;
   double* S = A0 + (int64_t)b * bstride;
-  // (1) max |B| (B = S(r:2r, 0:r)); a NaN makes the key maximal
+  // (1) stage B = S(r:2r, 0:r) and A = S(0:r, r:2r) (coalesced columns) and
+  // take max |B|; a NaN makes the key maximal
   unsigned long long mk = 0ULL;
-  for (int e = lane; e < r * r; e += 64) {
-    const double v = S[(int64_t)(r + e % r) + (int64_t)(e / r) * lda];
-    const unsigned long long k = (v == v) ? (unsigned long long)__double_as_longlong(fabs(v))
-                                          : ~0ULL;
-    mk = k > mk ? k : mk;
+  for (int c = 0; c < r; ++c) {
+    if (lane < r) {
+      const double v = S[(int64_t)(r + lane) + (int64_t)c * lda];
+      Bs[lane * 65 + c] = v;
+      const unsigned long long k = (v == v) ? (unsigned long long)__double_as_longlong(fabs(v))
+                                            : ~0ULL;
+      mk = k > mk ? k : mk;
+      As[lane * 65 + c] = S[(int64_t)lane + (int64_t)(r + c) * lda];
+    }
   }
   mk = wave_max_u64(mk);
   const bool fallback = mk > (unsigned long long)__double_as_longlong(1.0);
   if (lane == 0) general[b] = fallback ? 1 : 0;
   if (fallback) return;
-  // (2) C = I - B A on MFMA, 16 x 16 tiles into LDS (Cs[row*65 + col])
+  __syncthreads();
+  // (2) C = I - B A on MFMA from LDS, 16 x 16 tiles
   const int nt = (r + 15) >> 4;
   for (int t = 0; t < nt * nt; ++t) {
     const int tm = t % nt, tn = t / nt;
+    const int ar = tm * 16 + lr, bc = tn * 16 + lr;
     d4 acc = d4zero();
     for (int k0 = 0; k0 < r; k0 += 4) {
       const int k = k0 + lk;
-      const int ar = tm * 16 + lr, bc = tn * 16 + lr;
-      const double af = (k < r && ar < r) ? S[(int64_t)(r + ar) + (int64_t)k * lda] : 0.0;
-      const double bf = (k < r && bc < r) ? S[(int64_t)k + (int64_t)(r + bc) * lda] : 0.0;
+      const double af = (k < r && ar < r) ? Bs[ar * 65 + k] : 0.0;
+      const double bf = (k < r && bc < r) ? As[k * 65 + bc] : 0.0;
       acc = mfma(af, bf, acc);
     }
 #pragma unroll
@@ -1092,14 +1102,9 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
         wl = __ffsll((long long)__ballot(key == wmax && cpos == qq)) - 1;
       }
       const int bpos = __builtin_amdgcn_readlane(pos, wl);
-      if (lane == wl) {
-#pragma unroll
-        for (int c = 0; c < 64; ++c)
-          if (c >= jj) prow[c] = p[c];
-      }
       if (lane == jj) mypiv = bpos;
-      __syncthreads();
-      const double pv = prow[jj];
+      // the pivot row is broadcast lane-to-SGPR (readlane), no LDS
+      const double pv = readlane_f64(p[jj], wl);
       if (lane == wl) {
         live = false;
         pos = jj;
@@ -1107,33 +1112,28 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
         pos = bpos;
       }
       if (pv != 0.0) {
-        if (live) {
-          const double l = (fabs(pv) >= DBL_MIN) ? p[jj] * (1.0 / pv) : p[jj] / pv;
-          p[jj] = l;
+        const double l = (fabs(pv) >= DBL_MIN) ? p[jj] * (1.0 / pv) : p[jj] / pv;
+        if (live) p[jj] = l;
 #pragma unroll
-          for (int c = 0; c < 64; ++c)
-            if (c > jj) p[c] -= l * prow[c];
+        for (int c = 0; c < 64; ++c) {
+          if (c > jj) {
+            const double pc = readlane_f64(p[c], wl);
+            if (live) p[c] -= l * pc;
+          }
         }
       } else if (lane == 0 && info == 0) {
         info = r + jj + 1;
       }
-      __syncthreads();
     }
   }
   // (4) S's LU in LAPACK layout: rows r + pos get this lane's L2\U2 row and
   // its B row (the interchanges apply to the first r columns too)
   if (lane < r) {
-#pragma unroll
-    for (int c = 0; c < 64; ++c)
-      if (c < r) Cs[lane * 65 + c] = S[(int64_t)(r + lane) + (int64_t)c * lda];
-  }
-  __syncthreads();
-  if (lane < r) {
     double* row = S + (int64_t)(r + pos);
 #pragma unroll
     for (int c = 0; c < 64; ++c) {
       if (c < r) {
-        row[(int64_t)c * lda] = Cs[lane * 65 + c];
+        row[(int64_t)c * lda] = Bs[lane * 65 + c];
         row[(int64_t)(r + c) * lda] = p[c];
       }
     }
