@@ -1043,8 +1043,9 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
   // (1) stage B = S(r:2r, 0:r) and A = S(0:r, r:2r) (coalesced columns) and
   // take max |B|; a NaN makes the key maximal
   unsigned long long mk = 0ULL;
-  for (int c = 0; c < r; ++c) {
-    if (lane < r) {
+#pragma unroll 16
+  for (int c = 0; c < 64; ++c) {
+    if (c < r && lane < r) {
       const double v = S[(int64_t)(r + lane) + (int64_t)c * lda];
       Bs[lane * 65 + c] = v;
       const unsigned long long k = (v == v) ? (unsigned long long)__double_as_longlong(fabs(v))
@@ -1058,23 +1059,31 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
   if (lane == 0) general[b] = fallback ? 1 : 0;
   if (fallback) return;
   __syncthreads();
-  // (2) C = I - B A on MFMA from LDS, 16 x 16 tiles
+  // (2) C = I - B A on MFMA from LDS: per 16-row block of C, the four
+  // column tiles accumulate side by side (independent MFMA chains)
   const int nt = (r + 15) >> 4;
-  for (int t = 0; t < nt * nt; ++t) {
-    const int tm = t % nt, tn = t / nt;
-    const int ar = tm * 16 + lr, bc = tn * 16 + lr;
-    d4 acc = d4zero();
+  for (int tm = 0; tm < nt; ++tm) {
+    const int ar = tm * 16 + lr;
+    d4 acc[4];
+#pragma unroll
+    for (int tn = 0; tn < 4; ++tn) acc[tn] = d4zero();
     for (int k0 = 0; k0 < r; k0 += 4) {
       const int k = k0 + lk;
       const double af = (k < r && ar < r) ? Bs[ar * 65 + k] : 0.0;
-      const double bf = (k < r && bc < r) ? As[k * 65 + bc] : 0.0;
-      acc = mfma(af, bf, acc);
+#pragma unroll
+      for (int tn = 0; tn < 4; ++tn) {
+        const int bc = tn * 16 + lr;
+        const double bf = (tn < nt && k < r && bc < r) ? As[k * 65 + bc] : 0.0;
+        acc[tn] = mfma(af, bf, acc[tn]);
+      }
     }
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int row = tm * 16 + crow(lane, v), col = tn * 16 + lr;
-      if (row < r && col < r) Cs[row * 65 + col] = (row == col ? 1.0 : 0.0) - acc[v];
-    }
+    for (int tn = 0; tn < 4; ++tn)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = tm * 16 + crow(lane, v), col = tn * 16 + lr;
+        if (row < r && col < r) Cs[row * 65 + col] = (row == col ? 1.0 : 0.0) - acc[tn][v];
+      }
   }
   __syncthreads();
   // (3) LU of C with implicit pivoting: lane i keeps row i
@@ -2263,6 +2272,7 @@ __global__ void k_copy_cols(int64_t rows, int cols, const double* __restrict__ s
 // leaf's child at a given level.  16 x 16 output tiles are dealt to the
 // four waves.  D may alias C; A and B must not.
 // =====================================================================
+template <int GM, int GN>
 __global__ void __launch_bounds__(256) k_bgemm(BGemmArgs g) {
   const int i = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -2271,39 +2281,67 @@ __global__ void __launch_bounds__(256) k_bgemm(BGemmArgs g) {
   const int c = i >> g.bshift;
   const double* Bp = g.B + (int64_t)(c >> 1) * g.bs2 + (int64_t)((c & 1) ^ g.bflip) * g.bs1;
   const int mt = (g.M + 15) >> 4, nt = (g.Nc + 15) >> 4;
-  for (int tile = wid; tile < mt * nt; tile += 4) {
-    const int tm = tile % mt, tn = tile / mt;
-    const int col0 = tn * 16;
-    if (col0 + 16 <= g.csplit && !g.C1) continue;
-    if (col0 >= g.csplit && !g.C2) continue;
-    const int arow = tm * 16 + lr;
-    const int bcol = col0 + lr;
-    d4 acc = d4zero();
+  // each wave task covers GM x GN tiles of 16 x 16: GM*GN independent MFMA
+  // accumulation chains, one A value per row tile and one B value per
+  // column tile loaded per k-step
+  const int tgm = (mt + GM - 1) / GM, tgn = (nt + GN - 1) / GN;
+  for (int task = wid; task < tgm * tgn; task += 4) {
+    const int tm0 = (task % tgm) * GM, tn0 = (task / tgm) * GN;
+    bool on[GN];
+#pragma unroll
+    for (int j = 0; j < GN; ++j) {
+      const int col0 = (tn0 + j) * 16;
+      on[j] = tn0 + j < nt && !(col0 + 16 <= g.csplit && !g.C1) && !(col0 >= g.csplit && !g.C2);
+    }
+    d4 acc[GM][GN];
+#pragma unroll
+    for (int a = 0; a < GM; ++a)
+#pragma unroll
+      for (int j = 0; j < GN; ++j) acc[a][j] = d4zero();
     for (int k0 = 0; k0 < g.K; k0 += 4) {
       const int k = k0 + lk;
-      double af = 0.0, bf = 0.0;
-      if (k < g.K && arow < g.M)
-        af = !A ? (arow == k ? 1.0 : 0.0)
-                : (g.transA ? A[k + (int64_t)arow * g.lda] : A[arow + (int64_t)k * g.lda]);
-      if (k < g.K && bcol < g.Nc) bf = Bp[k + (int64_t)bcol * g.ldb];
-      acc = mfma(af, bf, acc);
+      double af[GM], bf[GN];
+#pragma unroll
+      for (int a = 0; a < GM; ++a) {
+        const int arow = (tm0 + a) * 16 + lr;
+        af[a] = 0.0;
+        if (k < g.K && arow < g.M)
+          af[a] = !A ? (arow == k ? 1.0 : 0.0)
+                     : (g.transA ? A[k + (int64_t)arow * g.lda] : A[arow + (int64_t)k * g.lda]);
+      }
+#pragma unroll
+      for (int j = 0; j < GN; ++j) {
+        const int bcol = (tn0 + j) * 16 + lr;
+        bf[j] = (on[j] && k < g.K && bcol < g.Nc) ? Bp[k + (int64_t)bcol * g.ldb] : 0.0;
+      }
+#pragma unroll
+      for (int a = 0; a < GM; ++a)
+#pragma unroll
+        for (int j = 0; j < GN; ++j) acc[a][j] = mfma(af[a], bf[j], acc[a][j]);
     }
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int orow = tm * 16 + crow(lane, v);
-      const int ocol = col0 + lr;
-      if (orow >= g.M || ocol >= g.Nc) continue;
-      if (ocol < g.csplit) {
-        if (g.C1) {
-          double o = g.alpha1 * acc[v];
-          if (g.beta1 != 0.0) o += g.beta1 * g.D1[(int64_t)i * g.sd1 + orow + (int64_t)ocol * g.ldd1];
-          g.C1[(int64_t)i * g.sc1 + orow + (int64_t)ocol * g.ldc1] = o;
+    for (int a = 0; a < GM; ++a) {
+#pragma unroll
+      for (int j = 0; j < GN; ++j) {
+        if (!on[j] || tm0 + a >= mt) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int orow = (tm0 + a) * 16 + crow(lane, v);
+          const int ocol = (tn0 + j) * 16 + lr;
+          if (orow >= g.M || ocol >= g.Nc) continue;
+          if (ocol < g.csplit) {
+            if (g.C1) {
+              double o = g.alpha1 * acc[a][j][v];
+              if (g.beta1 != 0.0) o += g.beta1 * g.D1[(int64_t)i * g.sd1 + orow + (int64_t)ocol * g.ldd1];
+              g.C1[(int64_t)i * g.sc1 + orow + (int64_t)ocol * g.ldc1] = o;
+            }
+          } else if (g.C2) {
+            const int oc = ocol - g.csplit;
+            double o = g.alpha2 * acc[a][j][v];
+            if (g.beta2 != 0.0) o += g.beta2 * g.D2[(int64_t)i * g.sd2 + orow + (int64_t)oc * g.ldd2];
+            g.C2[(int64_t)i * g.sc2 + orow + (int64_t)oc * g.ldc2] = o;
+          }
         }
-      } else if (g.C2) {
-        const int oc = ocol - g.csplit;
-        double o = g.alpha2 * acc[v];
-        if (g.beta2 != 0.0) o += g.beta2 * g.D2[(int64_t)i * g.sd2 + orow + (int64_t)oc * g.ldd2];
-        g.C2[(int64_t)i * g.sc2 + orow + (int64_t)oc * g.ldc2] = o;
       }
     }
   }
@@ -2311,7 +2349,18 @@ __global__ void __launch_bounds__(256) k_bgemm(BGemmArgs g) {
 
 hipError_t launch_bgemm(const BGemmArgs& g, hipStream_t st) {
   if (g.batch <= 0 || g.M <= 0 || g.Nc <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_bgemm, dim3((unsigned)g.batch), dim3(256), 0, st, g);
+  const int mt = (g.M + 15) >> 4, nt = (g.Nc + 15) >> 4;
+  // group tiles only while every wave keeps a task
+  if (mt * nt >= 16 && nt >= 4)
+    hipLaunchKernelGGL((k_bgemm<1, 4>), dim3((unsigned)g.batch), dim3(256), 0, st, g);
+  else if (mt * nt >= 16 && mt >= 4 && nt == 1)
+    hipLaunchKernelGGL((k_bgemm<4, 1>), dim3((unsigned)g.batch), dim3(256), 0, st, g);
+  else if (mt * nt >= 8 && nt >= 2)
+    hipLaunchKernelGGL((k_bgemm<1, 2>), dim3((unsigned)g.batch), dim3(256), 0, st, g);
+  else if (mt * nt >= 8)
+    hipLaunchKernelGGL((k_bgemm<2, 1>), dim3((unsigned)g.batch), dim3(256), 0, st, g);
+  else
+    hipLaunchKernelGGL((k_bgemm<1, 1>), dim3((unsigned)g.batch), dim3(256), 0, st, g);
   return hipGetLastError();
 }
 
