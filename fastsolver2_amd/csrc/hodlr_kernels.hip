@@ -83,6 +83,7 @@ __device__ __forceinline__ f4 mfma(float a, float b, f4 c) {
 template <typename T> struct Elem;
 template <> struct Elem<double> {
   typedef d4 V4;
+  static constexpr int DV = 4;  // crow(lane, v + 1) - crow(lane, v)
   __device__ static int crow(int lane, int v) { return (lane >> 4) + 4 * v; }
   __device__ static double sfmin() { return DBL_MIN; }
   // bits(|a|) + 1 orders non-negative values; NaN -> 1 (counts as zero)
@@ -92,6 +93,7 @@ template <> struct Elem<double> {
 };
 template <> struct Elem<float> {
   typedef f4 V4;
+  static constexpr int DV = 1;
   __device__ static int crow(int lane, int v) { return 4 * (lane >> 4) + v; }
   __device__ static float sfmin() { return FLT_MIN; }
   __device__ static unsigned long long key(float a) {
@@ -102,6 +104,36 @@ template <typename T>
 __device__ __forceinline__ typename Elem<T>::V4 zero4() {
   typename Elem<T>::V4 z = {T(0), T(0), T(0), T(0)};
   return z;
+}
+
+// Raw buffer access: a wave-uniform base in SGPRs and 32-bit byte offsets
+// (lane part in a VGPR, wave part in an SGPR), so a tile of loads costs one
+// offset register per lane instead of a 64-bit address per element.  The
+// base must be wave-uniform; the range check is off (num_records = max).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)0xFFFFFFFFu, 0x00020000);
+}
+template <typename T> __device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so);
+template <> __device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+}
+template <> __device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+}
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+// An opaque copy: the compiler cannot see through it, so address arithmetic
+// built on it stays in the loop body where the constant parts fold into the
+// instructions' offset fields (instead of one hoisted register per offset
+// held across the whole kernel).  Emits no instruction.
+template <typename V> __device__ __forceinline__ V opaque(V x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+__device__ __forceinline__ void buf_store(double x, __amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, vo, so, 0);
+}
+__device__ __forceinline__ void buf_store(float x, __amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, x), r, vo, so, 0);
 }
 
 __device__ __forceinline__ const double* seg_in(const ColSeg& s, int j) {
@@ -367,9 +399,25 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
   __shared__ int cidx[2][NW];
   __shared__ T crowv[2][NW][NB];
   __shared__ int pivs[NB];
-  __shared__ T L11s[S][NB * (NB + 1)];  // L11 of each panel of the outer block, [i*(NB+1) + k]
-  __shared__ T Xis[S][NB * (NB + 1)];   // their inverses, same layout
-  __shared__ T Tt[NB * (NB + 1)];    // doubling scratch
+  // LDS pool, in two lives.  Panel phase: L11 of each panel of the outer
+  // block ([i*(NB+1) + k]), their inverses (same layout) and the doubling
+  // scratch.  Trailing update: the negated L panel of a row band (k-major,
+  // [k*(BR+UPAD) + row]) and the U panel of a column tile ([k*(BC+UPAD) +
+  // col]).  The trailing update runs between two barriers after the last
+  // reader of the panel-phase arrays of its outer block.
+  constexpr int KC = S * NB;              // inner dimension of the trailing update
+  constexpr int TC = (NW >= 8) ? 4 : 2;   // 16-column tiles per wave
+  constexpr int BR = 32 * (NW / 2);       // macro tile: NW/2 x 2 waves of 32 x 16*TC
+  constexpr int BC = 2 * 16 * TC;
+  constexpr int UPAD = 2;
+  constexpr int LU_ELEMS = (2 * S + 1) * NB * (NB + 1);
+  constexpr int UP_ELEMS = KC * (BR + UPAD) + KC * (BC + UPAD);
+  __shared__ T pool[LU_ELEMS > UP_ELEMS ? LU_ELEMS : UP_ELEMS];
+  T* const L11s = pool;
+  T* const Xis = pool + S * NB * (NB + 1);
+  T* const Tt = pool + 2 * S * NB * (NB + 1);
+  T* const As = pool;
+  T* const Bs = pool + KC * (BR + UPAD);
   __shared__ int perm[NT];
   const int b = blockIdx.x;
   const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
@@ -378,14 +426,19 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
   if (t < n) perm[t] = t;
 
   for (int j0 = 0; j0 < n; j0 += NB) {
+    // the lane indices again, opaque: lane-dependent addresses are formed
+    // per panel instead of being hoisted into registers held for the whole
+    // kernel (the register file is the limit of this kernel)
+    const int t = opaque((int)threadIdx.x), lane = t & 63;
+    const int wid = __builtin_amdgcn_readfirstlane(t >> 6);
     const int nb = min(NB, n - j0);
     const int rows = n - j0;
     // outer block [J0, cend) of S panels; this is its panel sp
     const int sp = (j0 / NB) % S;
     const int J0 = j0 - sp * NB;
     const int cend = min(J0 + S * NB, n);
-    T* L11 = L11s[sp];
-    T* Xi = Xis[sp];
+    T* L11 = L11s + sp * NB * (NB + 1);
+    T* Xi = Xis + sp * NB * (NB + 1);
     const bool own = t < rows;
     T p[NB];
     if (own) {
@@ -606,6 +659,141 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
       }
     };
 
+    // The trailing update of a complete outer block, both panels staged in
+    // LDS: C(rb.., cb..) -= L(rb.., kb..kb+KC) U(kb..kb+KC, cb..) over
+    // nr x nc.  Macro tiles of BR x BC in row-band order; the L panel of a
+    // band is staged once (negated), the U panel once per tile, fetched into
+    // registers under the previous tile's MFMAs.  Accumulators
+    // start at zero and C, loaded under the MFMAs, is added at the end, so
+    // every element of C crosses HBM once each way per outer block.
+    static_assert(NT % BR == 0 && NT % KC == 0 && KC % 64 == 0 && (KC * BC) % NT == 0, "staging split");
+    // 32-bit byte offsets need 127 * lda * sizeof(T) < 2^32; the caller
+    // takes the plain update beyond lda = 2^22.
+    auto update_lds = [&](int rb, int nr, int cb, int nc, int kb) {
+      constexpr int PA = BR * KC / NT, PB = KC * BC / NT;
+      constexpr uint32_t SZ = sizeof(T);
+      const int nbc = (nc + BC - 1) / BC;
+      const int ntile = ((nr + BR - 1) / BR) * nbc;
+      const int wr = wid >> 1, wc = wid & 1;
+      const uint32_t ldb = (uint32_t)lda * SZ;  // column stride in bytes
+      // thread t stages row t % BR of the L panel (k = t / BR + i * NT / BR)
+      // and row t % KC of the U panel (column t / KC + i * NT / KC, the
+      // same for the whole wave since KC is a multiple of 64)
+      const int ra = t % BR, ka = t / BR;
+      const int kr = t % KC, cbt = __builtin_amdgcn_readfirstlane(t / KC);
+      const uint32_t voa = ra * SZ + ka * ldb;
+      const uint32_t vob = kr * SZ + cbt * ldb;
+      // C tile: lane offset of (crow(lane, 0), lane & 15)
+      const int lr = Elem<T>::crow(lane, 0);
+      const uint32_t voc = lr * SZ + (lane & 15) * ldb;
+      T pa[PA], pb[PB];
+      auto fetch_u = [&](int q) {
+        const int C0 = (q % nbc) * BC;
+        const __amdgpu_buffer_rsrc_t rs = buf_rsrc(A + kb + (int64_t)(cb + C0) * lda);
+        const uint32_t vo = opaque(vob);
+#pragma unroll
+        for (int i = 0; i < PB; ++i)  // wave-uniform column test
+          pb[i] = (C0 + cbt + i * (NT / KC) < nc) ? buf_load<T>(rs, vo, i * (NT / KC) * ldb) : T(0);
+      };
+      fetch_u(0);
+      for (int q = 0; q < ntile; ++q) {
+        const int R0 = (q / nbc) * BR, C0 = (q % nbc) * BC;
+        if (q % nbc == 0) {
+          // a new row band: its L panel is read here, under the wait for
+          // the other waves' last MFMAs of the previous band
+          const __amdgpu_buffer_rsrc_t rs = buf_rsrc(A + (rb + R0) + (int64_t)kb * lda);
+          const uint32_t vo = opaque(voa);
+          if (R0 + ra < nr) {
+#pragma unroll
+            for (int i = 0; i < PA; ++i) pa[i] = buf_load<T>(rs, vo, i * (NT / BR) * ldb);
+          } else {
+#pragma unroll
+            for (int i = 0; i < PA; ++i) pa[i] = T(0);
+          }
+        }
+        __syncthreads();  // the previous tile's MFMAs are done with As, Bs
+        if (q % nbc == 0) {
+          T* const dst = As + opaque(ka * (BR + UPAD) + ra);
+#pragma unroll
+          for (int i = 0; i < PA; ++i) dst[i * (NT / BR) * (BR + UPAD)] = -pa[i];
+        }
+        {
+          T* const dst = Bs + opaque(kr * (BC + UPAD) + cbt);
+#pragma unroll
+          for (int i = 0; i < PB; ++i) dst[i * (NT / KC)] = pb[i];
+        }
+        __syncthreads();
+        const int r0 = R0 + wr * 32, c0 = C0 + wc * 16 * TC;
+        const bool act = r0 < nr && c0 < nc;  // wave-uniform
+        const bool full = r0 + 32 <= nr && c0 + 16 * TC <= nc;
+        const __amdgpu_buffer_rsrc_t rc = buf_rsrc(A + (rb + r0) + (int64_t)(cb + c0) * lda);
+        const uint32_t vc = opaque(voc);
+        auto in_c = [&](int a, int bb, int v) {
+          return r0 + a * 16 + lr + v * Elem<T>::DV < nr && c0 + bb * 16 + (lane & 15) < nc;
+        };
+        T cv[2][TC][4];
+        if (full) {
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int bb = 0; bb < TC; ++bb)
+#pragma unroll
+              for (int v = 0; v < 4; ++v)
+                cv[a][bb][v] = buf_load<T>(rc, vc, bb * 16 * ldb + (a * 16 + v * Elem<T>::DV) * SZ);
+        } else if (act) {
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int bb = 0; bb < TC; ++bb)
+#pragma unroll
+              for (int v = 0; v < 4; ++v)
+                cv[a][bb][v] = in_c(a, bb, v)
+                                   ? buf_load<T>(rc, vc, bb * 16 * ldb + (a * 16 + v * Elem<T>::DV) * SZ)
+                                   : T(0);
+        }
+        if (q + 1 < ntile) fetch_u(q + 1);
+        if (act) {
+          V4 acc[2][TC];
+#pragma unroll
+          for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int bb = 0; bb < TC; ++bb) acc[a][bb] = zero4<T>();
+          const T* const sa = As + opaque((lane >> 4) * (BR + UPAD) + wr * 32 + (lane & 15));
+          const T* const sb = Bs + opaque((lane >> 4) * (BC + UPAD) + wc * 16 * TC + (lane & 15));
+#pragma unroll
+          for (int ks = 0; ks < KC / 4; ++ks) {
+            T af[2], bf[TC];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) af[a] = sa[ks * 4 * (BR + UPAD) + a * 16];
+#pragma unroll
+            for (int bb = 0; bb < TC; ++bb) bf[bb] = sb[ks * 4 * (BC + UPAD) + bb * 16];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+              for (int bb = 0; bb < TC; ++bb) acc[a][bb] = mfma(af[a], bf[bb], acc[a][bb]);
+          }
+          if (full) {
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+              for (int bb = 0; bb < TC; ++bb)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                  buf_store(acc[a][bb][v] + cv[a][bb][v], rc, vc, bb * 16 * ldb + (a * 16 + v * Elem<T>::DV) * SZ);
+          } else {
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+              for (int bb = 0; bb < TC; ++bb)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                  if (in_c(a, bb, v))
+                    buf_store(acc[a][bb][v] + cv[a][bb][v], rc, vc, bb * 16 * ldb + (a * 16 + v * Elem<T>::DV) * SZ);
+          }
+        }
+      }
+    };
+
     const int o = j0 + nb;
     // inside the outer block: U = L11^-1 A(j0.., o..cend), then the rows
     // below take the rank-nb update on those columns only
@@ -685,7 +873,7 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
                 const T bf = acc[a2][v];
 #pragma unroll
                 for (int a = 0; a < NB / 16; ++a)
-                  u[s2][a] = mfma(Xis[s2][(a * 16 + (lane & 15)) * (NB + 1) + k], bf, u[s2][a]);
+                  u[s2][a] = mfma(Xis[s2 * NB * (NB + 1) + (a * 16 + (lane & 15)) * (NB + 1) + k], bf, u[s2][a]);
               }
 #pragma unroll
             for (int a = 0; a < NB / 16; ++a)
@@ -696,7 +884,10 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
         }
       }
       __syncthreads();
-      update(cend, ntr, cend, ntr, J0, cend - J0);
+      if (lda <= (1 << 22))
+        update_lds(cend, ntr, cend, ntr, J0);
+      else
+        update(cend, ntr, cend, ntr, J0, cend - J0);
     }
     __syncthreads();
   }
