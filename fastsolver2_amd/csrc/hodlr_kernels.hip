@@ -64,6 +64,49 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
                 umax64(readlane_u64(v, 32), readlane_u64(v, 48)));
 }
 
+// 32-bit maximum over the wave, same butterfly; 0 is the identity, so the
+// DPP moves fold into v_max_u32.
+template <int CTRL> __device__ __forceinline__ unsigned dpp_u32(unsigned v) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
+  v = max(v, dpp_u32<0xB1>(v));
+  v = max(v, dpp_u32<0x4E>(v));
+  v = max(v, dpp_u32<0x141>(v));
+  v = max(v, dpp_u32<0x140>(v));
+  return max(max((unsigned)__builtin_amdgcn_readlane((int)v, 0), (unsigned)__builtin_amdgcn_readlane((int)v, 16)),
+             max((unsigned)__builtin_amdgcn_readlane((int)v, 32), (unsigned)__builtin_amdgcn_readlane((int)v, 48)));
+}
+// Lane of the largest 64-bit key, the smallest `q` among equal keys (0 if
+// every key is 0); `key` gives the maximum.  Stage one reduces a 32-bit
+// summary `k32` that is non-decreasing in the key (exact for keys below
+// 2^32): a single lane at its maximum is the answer; otherwise the lanes at
+// that maximum go through the full 64-bit reduction and the position rule.
+__device__ __forceinline__ int wave_argmax_key(unsigned long long k, unsigned k32, int q,
+                                               unsigned long long& key) {
+  if (__ballot(k != 0ULL) == 0ULL) {  // no candidate in the wave
+    key = 0ULL;
+    return 0;
+  }
+  const unsigned h = wave_max_u32(k32);
+  const unsigned long long bal = __ballot(k32 == h);
+  if (__popcll(bal) == 1) {
+    const int l = __ffsll((long long)bal) - 1;
+    key = readlane_u64(k, l);
+    return l;
+  }
+  const unsigned long long kk = (k32 == h) ? k : 0ULL;
+  const unsigned long long m = wave_max_u64(kk);
+  key = m;
+  if (m == 0ULL) return 0;
+  const unsigned long long b2 = __ballot(kk == m);
+  if (__popcll(b2) == 1) return __ffsll((long long)b2) - 1;
+  int qq = (kk == m) ? q : INT_MAX;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) qq = min(qq, __shfl_xor(qq, off, 64));
+  return __ffsll((long long)__ballot(kk == m && q == qq)) - 1;
+}
+
 // row of accumulator element v held by `lane`
 __device__ __forceinline__ int crow(int lane, int v) { return (lane >> 4) + 4 * v; }
 
@@ -90,6 +133,8 @@ template <> struct Elem<double> {
   __device__ static unsigned long long key(double a) {
     return (a == a) ? (unsigned long long)__double_as_longlong(fabs(a)) + 1ULL : 1ULL;
   }
+  __device__ static unsigned key32(unsigned long long k) { return (unsigned)(k >> 32); }
+  __device__ static double lane(double a, int l) { return readlane_f64(a, l); }
 };
 template <> struct Elem<float> {
   typedef f4 V4;
@@ -99,6 +144,8 @@ template <> struct Elem<float> {
   __device__ static unsigned long long key(float a) {
     return (a == a) ? (unsigned long long)__float_as_uint(fabsf(a)) + 1ULL : 1ULL;
   }
+  __device__ static unsigned key32(unsigned long long k) { return (unsigned)k; }  // k < 2^32
+  __device__ static float lane(float a, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(a), l)); }
 };
 template <typename T>
 __device__ __forceinline__ typename Elem<T>::V4 zero4() {
@@ -128,6 +175,15 @@ typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 template <typename V> __device__ __forceinline__ V opaque(V x) {
   asm volatile("" : "+v"(x));
   return x;
+}
+// Workgroup barrier that orders LDS only: it waits for this wave's LDS
+// operations, not for its global stores (which __syncthreads also drains).
+// For exchanges through LDS where no wave reads another's global writes
+// before the next full barrier.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 __device__ __forceinline__ void buf_store(double x, __amdgpu_buffer_rsrc_t r, uint32_t vo, uint32_t so) {
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, vo, so, 0);
@@ -454,30 +510,30 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
     // position, as idamax), `live` until the row becomes a pivot.
     int pos = t;
     bool live = own;
+    const bool wact = __ballot(own) != 0ULL;  // wave-uniform
 
 #pragma unroll
     for (int jj = 0; jj < NB; ++jj) {
       if (jj < nb) {
         const int par = jj & 1;
+        if (!wact) {
+          // no row of this wave is in the panel: publish "no candidate"
+          // and keep the barrier count
+          if (lane == 0) {
+            ckey[par][wid] = 0ULL;
+            cidx[par][wid] = INT_MAX;
+          }
+          lds_barrier();
+          continue;
+        }
         // key = bits(|a|) + 1 orders non-negative doubles; a NaN counts as
         // zero; 0 = not a candidate (retired rows, rows past n)
         const T aj = p[jj];
         const unsigned long long mykey = live ? Elem<T>::key(aj) : 0ULL;
         const int cpos = mykey ? pos : INT_MAX;
         // this wave's candidate: largest key, then smallest position
-        const unsigned long long wmax = wave_max_u64(mykey);
-        const unsigned long long bal = __ballot(mykey == wmax);
-        int wl = 0;
-        if (wmax != 0ULL) {
-          if (__popcll(bal) == 1) {
-            wl = __ffsll((long long)bal) - 1;
-          } else {
-            int qq = (mykey == wmax) ? cpos : INT_MAX;
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) qq = min(qq, __shfl_xor(qq, off, 64));
-            wl = __ffsll((long long)__ballot(mykey == wmax && cpos == qq)) - 1;
-          }
-        }
+        unsigned long long wmax;
+        const int wl = wave_argmax_key(mykey, Elem<T>::key32(mykey), cpos, wmax);
         if (lane == wl) {
 #pragma unroll
           for (int c = 0; c < NB; ++c)
@@ -485,22 +541,14 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
           ckey[par][wid] = wmax;
           cidx[par][wid] = wmax ? cpos : INT_MAX;
         }
-        __syncthreads();
+        lds_barrier();
         // winner across the waves, lane-parallel: lane w < NW holds wave w's
-        // candidate
+        // candidate (key, position and pivot value)
         const unsigned long long kw = (lane < NW) ? ckey[par][lane] : 0ULL;
         const int qw = (lane < NW) ? cidx[par][lane] : INT_MAX;
-        const unsigned long long kbest = wave_max_u64(kw);
-        const unsigned long long tb = __ballot(lane < NW && kw == kbest);
-        int bw;
-        if (__popcll(tb) == 1) {
-          bw = __ffsll((long long)tb) - 1;
-        } else {
-          int qq = (lane < NW && kw == kbest) ? qw : INT_MAX;
-#pragma unroll
-          for (int off = 32; off > 0; off >>= 1) qq = min(qq, __shfl_xor(qq, off, 64));
-          bw = __ffsll((long long)__ballot(lane < NW && kw == kbest && qw == qq)) - 1;
-        }
+        const T pvw = (lane < NW) ? crowv[par][lane][jj] : T(0);
+        unsigned long long kbest;
+        const int bw = wave_argmax_key(kw, Elem<T>::key32(kw), qw, kbest);
         const int bpos = __builtin_amdgcn_readlane(qw, bw);
         const bool winner = (lane == wl) && (wmax != 0ULL);
         const T* prow = crowv[par][bw];
@@ -513,10 +561,14 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
         } else if (pos == jj) {
           pos = bpos;
         }
-        const T pv = prow[jj];
+        const T pv = Elem<T>::lane(pvw, bw);  // wave-uniform
         if (pv != 0.0) {
           if (live) {
-            const T l = (fabs(pv) >= Elem<T>::sfmin()) ? p[jj] * (T(1) / pv) : p[jj] / pv;
+            T l;
+            if (fabs(pv) >= Elem<T>::sfmin())
+              l = p[jj] * (T(1) / pv);
+            else
+              l = p[jj] / pv;
             p[jj] = l;
 #pragma unroll
             for (int c = 0; c < NB; ++c)
@@ -527,7 +579,7 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
         }
       }
     }
-    __syncthreads();
+    lds_barrier();  // pivs
 
     // write the panel rows to their final positions; L11 to LDS
     if (own) {
@@ -546,17 +598,12 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
       for (int c = 0; c < NB; ++c) L11[t * (NB + 1) + c] = 0.0;
     }
     if (t < nb) ipiv0[(int64_t)b * n + j0 + t] = j0 + pivs[t] + 1;  // 1-based
-    if (t == 0) {
-      for (int jj = 0; jj < nb; ++jj) {
-        const int pp = pivs[jj];
-        if (pp != jj) {
-          const int tmp = perm[j0 + jj];
-          perm[j0 + jj] = perm[j0 + pp];
-          perm[j0 + pp] = tmp;
-        }
-      }
-    }
-    __syncthreads();
+    // the panel's interchanges on the row permutation: the row that started
+    // at j0 + t ends at j0 + pos, so perm moves the same way (one gather
+    // before the barrier, one scatter after it)
+    const int pold = own ? perm[j0 + t] : 0;
+    lds_barrier();  // L11, perm; the row stores above are read after a full barrier
+    if (own && pos != t) perm[j0 + pos] = pold;
 
     // interchanges on the other columns (sequential in jj, as dlaswp)
     const int m2 = n - j0 - nb;
@@ -580,7 +627,7 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
         const int i = e / NB, j = e % NB;
         Xi[i * (NB + 1) + j] = (i == j) ? 1.0 : 0.0;
       }
-      __syncthreads();
+      lds_barrier();
       for (int h = 1; h < NB; h <<= 1) {
         for (int e = t; e < NB * h / 2; e += NT) {
           const int q = e / (h * h), r = e % (h * h);
@@ -589,7 +636,7 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
           for (int k = j; k < s0 + h; ++k) acc += L11[i * (NB + 1) + k] * Xi[k * (NB + 1) + j];
           Tt[i * (NB + 1) + j] = acc;
         }
-        __syncthreads();
+        lds_barrier();
         for (int e = t; e < NB * h / 2; e += NT) {
           const int q = e / (h * h), r = e % (h * h);
           const int s0 = q * 2 * h, i = s0 + h + r / h, j = s0 + r % h;
@@ -597,9 +644,10 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
           for (int k = s0 + h; k <= i; ++k) acc += Xi[i * (NB + 1) + k] * Tt[k * (NB + 1) + j];
           Xi[i * (NB + 1) + j] = -acc;
         }
-        __syncthreads();
+        lds_barrier();
       }
     }
+    __syncthreads();  // the interchanged columns are read by other waves from here on
 
     // A(rb.., cb..) -= A(rb.., kb..kb+kc) * A(kb..kb+kc, cb..) over nr x ncl,
     // 32 x 32 blocks per wave, the inner dimension in 16-wide chunks.
@@ -711,7 +759,7 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
             for (int i = 0; i < PA; ++i) pa[i] = T(0);
           }
         }
-        __syncthreads();  // the previous tile's MFMAs are done with As, Bs
+        lds_barrier();  // the previous tile's MFMAs are done with As, Bs
         if (q % nbc == 0) {
           T* const dst = As + opaque(ka * (BR + UPAD) + ra);
 #pragma unroll
@@ -722,7 +770,7 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
 #pragma unroll
           for (int i = 0; i < PB; ++i) dst[i * (NT / KC)] = pb[i];
         }
-        __syncthreads();
+        lds_barrier();
         const int r0 = R0 + wr * 32, c0 = C0 + wc * 16 * TC;
         const bool act = r0 < nr && c0 < nc;  // wave-uniform
         const bool full = r0 + 32 <= nr && c0 + 16 * TC <= nc;
