@@ -1627,6 +1627,11 @@ k_getrs_reg(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
   const int lr = lane & 15, lk = lane >> 4;
   const int nblk = ntri(n);
   const int nrt = nblk * 2;  // 16-row tiles
+  // factor fragment addressing (bytes): lane part and the step of kk;
+  // 31 * lda * sizeof(T) < 2^32 is checked by the launcher
+  const uint32_t ldb = (uint32_t)lda * sizeof(T);
+  const uint32_t lane_off = TRANS ? lk * (uint32_t)sizeof(T) + lr * ldb : lr * (uint32_t)sizeof(T) + lk * ldb;
+  const uint32_t kstep = TRANS ? 4 * (uint32_t)sizeof(T) : 4 * ldb;
 
   V4 acc[4][NCT];
   // gather (rows by perm for the forward solve)
@@ -1680,30 +1685,60 @@ k_getrs_reg(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
       }
       __syncthreads();
       // (b) owners take the solved block back; other tiles on the far side
-      // of the block are updated with the off-diagonal factor block.
+      // of the block are updated with the off-diagonal factor block.  The
+      // tiles to update form a run of q; the factor fragments of the next
+      // tile are fetched under the MFMAs of the current one.  Fragment
+      // (kk, lane) of tile rt is F[rt*16 + lr][i0 + kk*4 + lk] with F = L or
+      // U (F^T from the transposed storage when TRANS): a per-tile scalar
+      // base, a lane offset and a per-kk scalar offset, all 32-bit.
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int rt = wid + 8 * q;
-        if (rt < nrt) {
-          if ((rt >> 1) == ib) {
+        if (rt < nrt && (rt >> 1) == ib) {
+#pragma unroll
+          for (int ct = 0; ct < NCT; ++ct)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+              acc[q][ct][v] = xs[((rt & 1) * 16 + Elem<T>::crow(lane, v)) * (W + 1) + ct * 16 + lr];
+        }
+      }
+      auto upd = [&](int q) {
+        const int rt = wid + 8 * q;
+        return rt < nrt && (fwd ? (rt >> 1) > ib : (rt >> 1) < ib);
+      };
+      const uint32_t vo = opaque(lane_off);
+      auto fetch = [&](int q, T (&f)[kTri / 4]) {
+        const int rt = wid + 8 * q;
+        const T* base = TRANS ? LU + i0 + (int64_t)(rt * 16) * lda : LU + (int64_t)i0 * lda + rt * 16;
+        const __amdgpu_buffer_rsrc_t rs = buf_rsrc(base);
+        if (rt * 16 + 16 <= n && i0 + kTri <= n) {
+#pragma unroll
+          for (int kk = 0; kk < kTri / 4; ++kk) f[kk] = buf_load<T>(rs, vo, kk * kstep);
+        } else {
+#pragma unroll
+          for (int kk = 0; kk < kTri / 4; ++kk)
+            f[kk] = (rt * 16 + lr < n && i0 + kk * 4 + lk < n) ? buf_load<T>(rs, vo, kk * kstep) : T(0);
+        }
+      };
+      T af[kTri / 4], afn[kTri / 4];
+      bool have = false;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        if (upd(q)) {
+          if (!have) {
+            fetch(q, af);
+          } else {
+#pragma unroll
+            for (int kk = 0; kk < kTri / 4; ++kk) af[kk] = afn[kk];
+          }
+          have = true;
+          if (q + 1 < 4 && upd(q + 1)) fetch(q + 1, afn);
+#pragma unroll
+          for (int kk = 0; kk < kTri / 4; ++kk) {
+            const int k = kk * 4 + lk;
 #pragma unroll
             for (int ct = 0; ct < NCT; ++ct)
-#pragma unroll
-              for (int v = 0; v < 4; ++v)
-                acc[q][ct][v] = xs[((rt & 1) * 16 + Elem<T>::crow(lane, v)) * (W + 1) + ct * 16 + lr];
-          } else if (fwd ? (rt >> 1) > ib : (rt >> 1) < ib) {
-            const int row = rt * 16 + lr;
-#pragma unroll
-            for (int kk = 0; kk < kTri / 4; ++kk) {
-              const int k = kk * 4 + lk;
-              const int kcol = i0 + k;
-              T af = 0.0;
-              if (row < n && kcol < n)
-                af = TRANS ? LU[(int64_t)kcol + (int64_t)row * lda] : LU[(int64_t)row + (int64_t)kcol * lda];
-#pragma unroll
-              for (int ct = 0; ct < NCT; ++ct)
-                acc[q][ct] = mfma(-af, xs[k * (W + 1) + ct * 16 + lr], acc[q][ct]);
-            }
+              acc[q][ct] = mfma(-af[kk], xs[k * (W + 1) + ct * 16 + lr], acc[q][ct]);
           }
         }
       }
@@ -1721,6 +1756,11 @@ k_getrs_reg(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
         if (row < n && col < ncols) seg_out(X, col)[xoff + (TRANS ? perm[row] : row)] = acc[q][ct][v];
       }
     }
+}
+
+// the register solve addresses factor fragments with 32-bit byte offsets
+template <typename T> static bool getrs_reg_ld_ok(int64_t lda) {
+  return lda > 0 && lda * (int64_t)sizeof(T) * 32 < (int64_t)1 << 32;
 }
 
 template <int NCT, bool TRANS, typename T>
@@ -1743,7 +1783,7 @@ static hipError_t launch_getrs_impl(int n, const double* LU, int64_t lda, int64_
                                     hipStream_t st) {
   if (batch <= 0 || n <= 0 || ncols <= 0) return hipSuccess;
   if (n > kMaxBlock) return hipErrorInvalidValue;
-  if (n <= 512) {
+  if (n <= 512 && getrs_reg_ld_ok<double>(lda)) {
     // column tile: 16, 32 or 80 right-hand sides per workgroup, by width
     // unless a tuner fixed it
     int nct = ncols <= 16 ? 1 : (ncols <= 32 ? 2 : 5);
@@ -1824,7 +1864,7 @@ hipError_t launch_getrs_f32(int n, const float* LU, int64_t lda, int64_t bstride
                             const int* perm, const float* dinv, const ColSeg& X, int64_t xbstride,
                             int ncols, bool trans, hipStream_t st) {
   if (batch <= 0 || n <= 0 || ncols <= 0) return hipSuccess;
-  if (n > 512) return hipErrorInvalidValue;
+  if (n > 512 || !getrs_reg_ld_ok<float>(lda)) return hipErrorInvalidValue;
   if (trans) {
     if (ncols <= 16) launch_getrs_reg<1, true, float>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
     else if (ncols <= 32) launch_getrs_reg<2, true, float>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
