@@ -1257,51 +1257,61 @@ hipError_t launch_getrf(int n, double* A, int64_t lda, int64_t bstride, int batc
 }
 
 // =====================================================================
-// LU of the HODLR cores S = [I, A; B, I] (2r x 2r, r <= 64), one wave each.
-// With max|B| <= 1 partial pivoting takes the diagonal in each of the
-// first r columns (ties go to the first index, as idamax), so
+// LU of the HODLR cores S = [I, A; B, I] (2r x 2r, r <= 64), one workgroup
+// of four waves each.  With max|B| <= 1 partial pivoting takes the diagonal
+// in each of the first r columns (ties go to the first index, as idamax), so
 //   L = [I, 0; P2 B, L2],  U = [I, A; 0, U2],  L2 U2 = P2 (I - B A)
 // -- the same factors dgetrf produces (its first 64-column panel is the
-// identity block, its trailing update the GEMM I - B A).  The kernel forms
-// C = I - B A on MFMA, factors C with implicit pivoting (row i of C in lane
-// i, 64 columns fully in registers) and writes S's LU in LAPACK layout.
-// Cores with max|B| > 1 or a NaN in B are flagged in `general` and left to
-// k_getrf_w64.
+// identity block, its trailing update the GEMM I - B A).  The four waves
+// stage A and B in LDS and form C = I - B A on MFMA (one 16-row block of C
+// each); wave 0 then factors C alone, without barriers (row i of C in lane
+// i, 64 columns in registers, the pivot row broadcast by readlane) and
+// writes S's LU in LAPACK layout.  Cores with max|B| > 1 or a NaN in B are
+// flagged in `general` and left to k_getrf_w64.
 // =====================================================================
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(256)
 k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
              int* __restrict__ ipiv0, int* __restrict__ perm0, int* __restrict__ info0,
              int* __restrict__ general) {
   __shared__ double Cs[64 * 65];  // C = I - B A, [row*65 + col]
   __shared__ double Bs[64 * 65];  // B, [row*65 + k]
   __shared__ double As[64 * 65];  // A, [k*65 + col]
-  const int b = blockIdx.x, lane = threadIdx.x;
+  __shared__ unsigned long long wmk[4];
+  const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lr = lane & 15, lk = lane >> 4;
   const int r = n2 >> 1;
   double* S = A0 + (int64_t)b * bstride;
-  // (1) stage B = S(r:2r, 0:r) and A = S(0:r, r:2r) (coalesced columns) and
-  // take max |B|; a NaN makes the key maximal
+  // (1) stage B = S(r:2r, 0:r) and A = S(0:r, r:2r) (coalesced columns,
+  // wave w takes columns w, w+4, ...) and take max |B|; a NaN makes the key
+  // maximal
   unsigned long long mk = 0ULL;
-#pragma unroll 16
-  for (int c = 0; c < 64; ++c) {
-    if (c < r && lane < r) {
-      const double v = S[(int64_t)(r + lane) + (int64_t)c * lda];
-      Bs[lane * 65 + c] = v;
-      const unsigned long long k = (v == v) ? (unsigned long long)__double_as_longlong(fabs(v))
-                                            : ~0ULL;
-      mk = k > mk ? k : mk;
-      As[lane * 65 + c] = S[(int64_t)lane + (int64_t)(r + c) * lda];
+  if (lane < r) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int c = wid + 4 * i;
+      if (c < r) {
+        const double v = S[(int64_t)(r + lane) + (int64_t)c * lda];
+        Bs[lane * 65 + c] = v;
+        const unsigned long long k = (v == v) ? (unsigned long long)__double_as_longlong(fabs(v))
+                                              : ~0ULL;
+        mk = k > mk ? k : mk;
+        As[lane * 65 + c] = S[(int64_t)lane + (int64_t)(r + c) * lda];
+      }
     }
   }
   mk = wave_max_u64(mk);
-  const bool fallback = mk > (unsigned long long)__double_as_longlong(1.0);
-  if (lane == 0) general[b] = fallback ? 1 : 0;
-  if (fallback) return;
+  if (lane == 0) wmk[wid] = mk;
   __syncthreads();
-  // (2) C = I - B A on MFMA from LDS: per 16-row block of C, the four
-  // column tiles accumulate side by side (independent MFMA chains)
+  mk = umax64(umax64(wmk[0], wmk[1]), umax64(wmk[2], wmk[3]));
+  const bool fallback = mk > (unsigned long long)__double_as_longlong(1.0);
+  if (tid == 0) general[b] = fallback ? 1 : 0;
+  if (fallback) return;  // uniform over the workgroup
+  // (2) C = I - B A on MFMA from LDS: wave w forms 16-row block w of C,
+  // its four column tiles side by side (independent MFMA chains)
   const int nt = (r + 15) >> 4;
-  for (int tm = 0; tm < nt; ++tm) {
+  if (wid < nt) {
+    const int tm = wid;
     const int ar = tm * 16 + lr;
     d4 acc[4];
 #pragma unroll
@@ -1325,6 +1335,7 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
       }
   }
   __syncthreads();
+  if (wid != 0) return;
   // (3) LU of C with implicit pivoting: lane i keeps row i
   double p[64];
 #pragma unroll
@@ -1338,17 +1349,8 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
     if (jj < r) {
       const unsigned long long key = live ? Elem<double>::key(p[jj]) : 0ULL;
       const int cpos = key ? pos : INT_MAX;
-      const unsigned long long wmax = wave_max_u64(key);
-      const unsigned long long bal = __ballot(key == wmax);
-      int wl;
-      if (__popcll(bal) == 1) {
-        wl = __ffsll((long long)bal) - 1;
-      } else {
-        int qq = (key == wmax) ? cpos : INT_MAX;
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) qq = min(qq, __shfl_xor(qq, off, 64));
-        wl = __ffsll((long long)__ballot(key == wmax && cpos == qq)) - 1;
-      }
+      unsigned long long wmax;
+      const int wl = wave_argmax_key(key, Elem<double>::key32(key), cpos, wmax);
       const int bpos = __builtin_amdgcn_readlane(pos, wl);
       if (lane == jj) mypiv = bpos;
       // the pivot row is broadcast lane-to-SGPR (readlane), no LDS
@@ -1360,7 +1362,11 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
         pos = bpos;
       }
       if (pv != 0.0) {
-        const double l = (fabs(pv) >= DBL_MIN) ? p[jj] * (1.0 / pv) : p[jj] / pv;
+        double l;
+        if (fabs(pv) >= DBL_MIN)
+          l = p[jj] * (1.0 / pv);
+        else
+          l = p[jj] / pv;
         if (live) p[jj] = l;
 #pragma unroll
         for (int c = 0; c < 64; ++c) {
@@ -1401,7 +1407,7 @@ hipError_t launch_getrf_core(int n2, double* S, int64_t lda, int64_t bstride, in
                              int* perm, int* info, int* general, hipStream_t st) {
   if (batch <= 0 || n2 <= 0) return hipSuccess;
   if (n2 > 128 || (n2 & 1)) return launch_getrf(n2, S, lda, bstride, batch, ipiv, perm, info, st);
-  hipLaunchKernelGGL(k_getrf_core, dim3(batch), dim3(64), 0, st, S, lda, bstride, n2, ipiv, perm,
+  hipLaunchKernelGGL(k_getrf_core, dim3(batch), dim3(256), 0, st, S, lda, bstride, n2, ipiv, perm,
                      info, general);
   hipLaunchKernelGGL(k_getrf_w64<32>, dim3(batch), dim3(64), 0, st, S, lda, bstride, n2, ipiv, perm,
                      info, (const int*)general);
