@@ -114,6 +114,7 @@ def lib():
         "hodlr_selftest_mfma_f32": ([vp, vp], ci),
         "hodlr_dense_blocks": ([i64, ci, ci, vp, i64, vp, i64, vp, dbl, vp, i64, vp], ci),
         "hodlr_profile_enable": ([ci], ci),
+        "hodlr_profile_mask": ([C.c_uint], ci),
         "hodlr_profile_reset": ([], ci),
         "hodlr_profile_read": ([ci, C.POINTER(i64), C.POINTER(dbl), C.POINTER(dbl),
                                 C.POINTER(dbl)], ci),
@@ -535,7 +536,10 @@ PROF_KINDS = ["getrf_leaf", "getrs_leaf", "trtri", "reduce", "assemble", "getrf_
               "getrs_core", "update", "leaf_level"]
 
 
-def profile_enable(on=True):
+def profile_enable(on=True, kinds=None):
+    """Record launches (of `kinds`, names from PROF_KINDS; default all)."""
+    mask = 0xFFFFFFFF if kinds is None else sum(1 << PROF_KINDS.index(k) for k in kinds)
+    lib().hodlr_profile_mask(mask)
     lib().hodlr_profile_enable(1 if on else 0)
 
 

@@ -114,6 +114,7 @@ struct ProfRec {
 struct Profiler {
   std::mutex mu;
   bool on = false;
+  unsigned mask = ~0u;  // kinds recorded while on
   std::vector<ProfRec> recs;
 };
 Profiler g_prof;
@@ -122,7 +123,8 @@ struct ProfScope {
   bool on;
   ProfRec rec;
   hipStream_t st;
-  ProfScope(int kind, hipStream_t s, double flops, double bytes) : on(g_prof.on), st(s) {
+  ProfScope(int kind, hipStream_t s, double flops, double bytes)
+      : on(g_prof.on && ((g_prof.mask >> kind) & 1u)), st(s) {
     if (!on) return;
     rec.kind = kind; rec.flops = flops; rec.bytes = bytes;
     if (hipEventCreate(&rec.a) != hipSuccess || hipEventCreate(&rec.b) != hipSuccess) {
@@ -199,6 +201,7 @@ struct hodlr_plan_s {
   int64_t N = 0;
   int m = 0, L = 0, r = 0, smax = 0;
   int64_t P = 0;
+  int64_t wst = 0;  // per-leaf stride of the r x r blocks W_t,i and Q_i (r*r)
   // leaf factorization
   int* leaf_ipiv = nullptr;
   int* leaf_perm = nullptr;
@@ -356,7 +359,7 @@ int leaf_phase(hodlr_plan p, double* K, int64_t ldk, const double* U, int64_t ld
   return HODLR_OK;
 }
 
-double* W_at(hodlr_plan p, int t) { return p->W + (int64_t)t * p->P * p->r * p->r; }
+double* W_at(hodlr_plan p, int t) { return p->W + (int64_t)t * p->P * p->wst; }
 
 // Per-leaf projections part_i = V_i^T X_i (r x nc, ld r, stride r*nc).
 int leaf_projections(hodlr_plan p, const double* Vm, int64_t ldv, const ColSeg& x, int nc,
@@ -400,9 +403,8 @@ int level_phase(hodlr_plan p, double* B, int64_t ldb, int nrhs, hipStream_t st) 
   int rc = leaf_projections(p, p->V, p->ldv, x, nc, Pc, st);
   if (rc) return rc;
   // Q_i = V_i^T Y_i: the first r columns of each projection
-  HIP_TRY(hodlr::launch_copy_cols((int64_t)r * r, (int)P, Pc, (int64_t)r * nc, p->Q,
-                                  (int64_t)r * r, st));
-  HIP_TRY(hodlr::launch_eye((int)P, r, W_at(p, L - 1), st));
+  HIP_TRY(hodlr::launch_copy_cols((int64_t)r * r, (int)P, Pc, (int64_t)r * nc, p->Q, p->wst, st));
+  HIP_TRY(hodlr::launch_eye((int)P, r, p->wst, W_at(p, L - 1), st));
   for (int t = L - 1; t >= 0; --t) {
     const int nnode = 1 << t;
     const int shift = L - 1 - t;  // leaf -> child at level t
@@ -420,12 +422,12 @@ int level_phase(hodlr_plan p, double* B, int64_t ldb, int nrhs, hipStream_t st) 
     // W_t E_c -> W_{t-1} = W_t - W_t E^Y, G (+)= W_t E^B
     hodlr::BGemmArgs g;
     g.batch = (int)P; g.M = r; g.K = r; g.Nc = nc; g.csplit = r;
-    g.A = W_at(p, t); g.lda = r; g.sa = (int64_t)r * r;
+    g.A = W_at(p, t); g.lda = r; g.sa = p->wst;
     g.B = c.R; g.ldb = 2 * r; g.bshift = shift; g.bs2 = ens; g.bs1 = r;
     if (t > 0) {
       g.alpha1 = -1.0; g.beta1 = 1.0;
-      g.D1 = W_at(p, t); g.ldd1 = r; g.sd1 = (int64_t)r * r;
-      g.C1 = W_at(p, t - 1); g.ldc1 = r; g.sc1 = (int64_t)r * r;
+      g.D1 = W_at(p, t); g.ldd1 = r; g.sd1 = p->wst;
+      g.C1 = W_at(p, t - 1); g.ldc1 = r; g.sc1 = p->wst;
     }
     if (nrhs > 0) {
       g.alpha2 = 1.0; g.beta2 = (t == L - 1) ? 0.0 : 1.0;
@@ -472,6 +474,12 @@ const char* hodlr_strerror(int status) {
 }
 
 const char* hodlr_last_error_detail(void) { return g_detail; }
+
+int hodlr_profile_mask(unsigned mask) {
+  std::lock_guard<std::mutex> lk(g_prof.mu);
+  g_prof.mask = mask;
+  return HODLR_OK;
+}
 
 int hodlr_profile_enable(int on) {
   std::lock_guard<std::mutex> lk(g_prof.mu);
@@ -540,8 +548,9 @@ int hodlr_plan_create(hodlr_plan* plan, const hodlr_desc* desc) {
     rc = rc ? rc : palloc(p, &p->leaf_dinv, p->P * hodlr::dinv_elems(p->m));
   }
   rc = rc ? rc : palloc(p, &p->Y, p->L > 0 ? p->N * r : 0);
-  rc = rc ? rc : palloc(p, &p->W, (int64_t)p->L * p->P * r * r);
-  rc = rc ? rc : palloc(p, &p->Q, p->L > 0 ? p->P * r * r : 0);
+  p->wst = (int64_t)r * r;
+  rc = rc ? rc : palloc(p, &p->W, (int64_t)p->L * p->P * p->wst);
+  rc = rc ? rc : palloc(p, &p->Q, p->L > 0 ? p->P * p->wst : 0);
   int64_t so = 0, po = 0, dof = 0, io = 0;
   for (int t = 0; t < p->L; ++t) {
     const int64_t nn = (int64_t)1 << t;
@@ -638,7 +647,7 @@ int hodlr_solve(hodlr_plan p, double* B, int64_t ldb, int nrhs, hipStream_t st) 
       if (t < L - 1) {
         hodlr::BGemmArgs h;
         h.batch = (int)P; h.M = r; h.K = r; h.Nc = nc; h.csplit = nc;
-        h.A = p->Q; h.lda = r; h.sa = (int64_t)r * r;
+        h.A = p->Q; h.lda = r; h.sa = p->wst;
         h.B = p->G; h.ldb = r; h.bs1 = ps; h.bs2 = 2 * ps;
         h.alpha1 = -1.0; h.beta1 = 1.0;
         h.D1 = p->Pb[0]; h.ldd1 = r; h.sd1 = ps;
@@ -653,7 +662,7 @@ int hodlr_solve(hodlr_plan p, double* B, int64_t ldb, int nrhs, hipStream_t st) 
       // G_i (+)= W_t,i eta_c
       hodlr::BGemmArgs g2;
       g2.batch = (int)P; g2.M = r; g2.K = r; g2.Nc = nc; g2.csplit = nc;
-      g2.A = W_at(p, t); g2.lda = r; g2.sa = (int64_t)r * r;
+      g2.A = W_at(p, t); g2.lda = r; g2.sa = p->wst;
       g2.B = c.R; g2.ldb = 2 * r; g2.bshift = shift; g2.bs2 = (int64_t)2 * r * nc; g2.bs1 = r;
       g2.alpha1 = 1.0; g2.beta1 = (t == L - 1) ? 0.0 : 1.0;
       g2.D1 = p->G; g2.ldd1 = r; g2.sd1 = ps;
@@ -691,7 +700,7 @@ int hodlr_solve_adjoint(hodlr_plan p, double* G, int64_t ldg, int nrhs, hipStrea
         if (t > 0) {
           hodlr::BGemmArgs h;
           h.batch = (int)P; h.M = r; h.K = r; h.Nc = nc; h.csplit = nc; h.transA = 1;
-          h.A = p->Q; h.lda = r; h.sa = (int64_t)r * r;
+          h.A = p->Q; h.lda = r; h.sa = p->wst;
           h.B = p->G; h.ldb = r; h.bs1 = ps; h.bs2 = 2 * ps;
           h.alpha1 = -1.0; h.beta1 = 1.0;
           h.D1 = p->Pb[0]; h.ldd1 = r; h.sd1 = ps;
@@ -701,7 +710,7 @@ int hodlr_solve_adjoint(hodlr_plan p, double* G, int64_t ldg, int nrhs, hipStrea
         }
         hodlr::BGemmArgs z;
         z.batch = (int)P; z.M = r; z.K = r; z.Nc = nc; z.csplit = nc; z.transA = 1;
-        z.A = W_at(p, t); z.lda = r; z.sa = (int64_t)r * r;
+        z.A = W_at(p, t); z.lda = r; z.sa = p->wst;
         z.B = u; z.ldb = r; z.bs1 = ps; z.bs2 = 2 * ps;
         z.alpha1 = 1.0; z.beta1 = 0.0;
         z.C1 = p->Pb[2]; z.ldc1 = r; z.sc1 = ps;

@@ -167,7 +167,7 @@ struct BGemmArgs {
   int64_t ldc2 = 0, sc2 = 0;
 };
 hipError_t launch_bgemm(const BGemmArgs& g, hipStream_t st);
-hipError_t launch_eye(int batch, int r, double* W, hipStream_t st);
+hipError_t launch_eye(int batch, int r, int64_t sw, double* W, hipStream_t st);
 hipError_t launch_selftest_mfma(double* out, hipStream_t st);
 hipError_t launch_selftest_mfma_f32(double* out, hipStream_t st);
 hipError_t launch_copy_cols(int64_t rows, int cols, const double* src, int64_t lds,

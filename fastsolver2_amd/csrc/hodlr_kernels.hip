@@ -2583,6 +2583,7 @@ __global__ void __launch_bounds__(256) k_bgemm(BGemmArgs g) {
     for (int a = 0; a < GM; ++a)
 #pragma unroll
       for (int j = 0; j < GN; ++j) acc[a][j] = d4zero();
+#pragma unroll 4
     for (int k0 = 0; k0 < g.K; k0 += 4) {
       const int k = k0 + lk;
       double af[GM], bf[GN];
@@ -2604,31 +2605,48 @@ __global__ void __launch_bounds__(256) k_bgemm(BGemmArgs g) {
 #pragma unroll
         for (int j = 0; j < GN; ++j) acc[a][j] = mfma(af[a], bf[j], acc[a][j]);
     }
+    // epilogue: each lane's output column picks its side (C1 or C2); all D
+    // values of the task are read before any C is written (C may alias D
+    // element for element), so the reads are in flight together
+    double dv[GM][GN][4];
+    double* cp[GN];
+    int64_t co[GN], dof[GN];
+    double al[GN], be[GN];
+    bool colok[GN];
 #pragma unroll
-    for (int a = 0; a < GM; ++a) {
+    for (int j = 0; j < GN; ++j) {
+      const int ocol = (tn0 + j) * 16 + lr;
+      const bool s1 = ocol < g.csplit;
+      const int oc = s1 ? ocol : ocol - g.csplit;
+      cp[j] = s1 ? g.C1 : g.C2;
+      al[j] = s1 ? g.alpha1 : g.alpha2;
+      be[j] = s1 ? g.beta1 : g.beta2;
+      co[j] = (int64_t)i * (s1 ? g.sc1 : g.sc2) + (int64_t)oc * (s1 ? g.ldc1 : g.ldc2);
+      dof[j] = (int64_t)i * (s1 ? g.sd1 : g.sd2) + (int64_t)oc * (s1 ? g.ldd1 : g.ldd2);
+      colok[j] = on[j] && ocol < g.Nc && cp[j] != nullptr;
+    }
+#pragma unroll
+    for (int a = 0; a < GM; ++a)
 #pragma unroll
       for (int j = 0; j < GN; ++j) {
-        if (!on[j] || tm0 + a >= mt) continue;
+        const int ocol = (tn0 + j) * 16 + lr;
+        const double* dp = ocol < g.csplit ? g.D1 : g.D2;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int orow = (tm0 + a) * 16 + crow(lane, v);
-          const int ocol = (tn0 + j) * 16 + lr;
-          if (orow >= g.M || ocol >= g.Nc) continue;
-          if (ocol < g.csplit) {
-            if (g.C1) {
-              double o = g.alpha1 * acc[a][j][v];
-              if (g.beta1 != 0.0) o += g.beta1 * g.D1[(int64_t)i * g.sd1 + orow + (int64_t)ocol * g.ldd1];
-              g.C1[(int64_t)i * g.sc1 + orow + (int64_t)ocol * g.ldc1] = o;
-            }
-          } else if (g.C2) {
-            const int oc = ocol - g.csplit;
-            double o = g.alpha2 * acc[a][j][v];
-            if (g.beta2 != 0.0) o += g.beta2 * g.D2[(int64_t)i * g.sd2 + orow + (int64_t)oc * g.ldd2];
-            g.C2[(int64_t)i * g.sc2 + orow + (int64_t)oc * g.ldc2] = o;
-          }
+          dv[a][j][v] = (colok[j] && tm0 + a < mt && orow < g.M && be[j] != 0.0) ? dp[dof[j] + orow] : 0.0;
         }
       }
-    }
+#pragma unroll
+    for (int a = 0; a < GM; ++a)
+#pragma unroll
+      for (int j = 0; j < GN; ++j)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int orow = (tm0 + a) * 16 + crow(lane, v);
+          if (colok[j] && tm0 + a < mt && orow < g.M)
+            cp[j][co[j] + orow] = al[j] * acc[a][j][v] + be[j] * dv[a][j][v];
+        }
   }
 }
 
@@ -2649,22 +2667,23 @@ hipError_t launch_bgemm(const BGemmArgs& g, hipStream_t st) {
   return hipGetLastError();
 }
 
-// W_i = I (r x r, ld r) for i < batch.
-__global__ void k_eye(int batch, int r, double* __restrict__ W) {
+// W_i = I (r x r, ld r, stride `sw` >= r*r between blocks) for i < batch.
+__global__ void k_eye(int batch, int r, int64_t sw, double* __restrict__ W) {
   const int64_t total = (int64_t)batch * r * r;
   for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * 256) {
-    const int64_t q = e % ((int64_t)r * r);
-    W[e] = (q % r == q / r) ? 1.0 : 0.0;
+    const int64_t i = e / ((int64_t)r * r), q = e % ((int64_t)r * r);
+    W[i * sw + q] = (q % r == q / r) ? 1.0 : 0.0;
   }
 }
 
-hipError_t launch_eye(int batch, int r, double* W, hipStream_t st) {
+hipError_t launch_eye(int batch, int r, int64_t sw, double* W, hipStream_t st) {
   const int64_t total = (int64_t)batch * r * r;
   if (total <= 0) return hipSuccess;
+  if (sw < (int64_t)r * r) return hipErrorInvalidValue;
   int64_t g = (total + 255) / 256;
   if (g > 16384) g = 16384;
-  hipLaunchKernelGGL(k_eye, dim3((unsigned)g), dim3(256), 0, st, batch, r, W);
+  hipLaunchKernelGGL(k_eye, dim3((unsigned)g), dim3(256), 0, st, batch, r, sw, W);
   return hipGetLastError();
 }
 

@@ -285,6 +285,10 @@ enum hodlr_prof_kind {
  * events around every launch on its stream, with the launch's algorithmic
  * FLOPs and minimum HBM bytes. */
 int hodlr_profile_enable(int on);
+/* Kinds recorded while enabled: bit k for kind k (default all).  Each
+ * recorded launch costs two event records on its stream, so a timing run
+ * can keep the events to the kinds it reports. */
+int hodlr_profile_mask(unsigned mask);
 int hodlr_profile_reset(void);
 /* Synchronizes on the recorded events; totals over all launches of `kind`. */
 int hodlr_profile_read(int kind, int64_t *count, double *total_ms, double *flops,
