@@ -2038,42 +2038,71 @@ hipError_t launch_reduce(const double* V, int64_t ldv, int r, const ColSeg& X, i
 // Core assembly (node_solve.cc:60-81): sum each child's partials in chunk
 // order, then S = [I, G_1; G_0, I] and R = [H_1; H_0] (halves swapped).
 // =====================================================================
+// Sum of the projections of leaves [q0, q1) under `child` at offset `off`.
 __device__ __forceinline__ double sum_part(const double* part, int child, int cpc, int64_t pstride,
-                                           int64_t off) {
+                                           int64_t off, int q0, int q1) {
   double s = 0.0;
   const double* p = part + (int64_t)child * cpc * pstride + off;
-  for (int q = 0; q < cpc; ++q) s += p[(int64_t)q * pstride];
+  for (int q = q0; q < q1; ++q) s += p[(int64_t)q * pstride];
   return s;
 }
 
+// Core assembly.  A block covers 256 / KS elements of one node; with KS > 1
+// the cpc leaves under a child are split into KS contiguous ranges summed
+// by separate thread groups and combined in a fixed order (deterministic),
+// so the upper levels, where a child spans many leaves and there are few
+// nodes, still spread over the chip.
+template <int KS>
 __global__ void __launch_bounds__(256)
 k_assemble(const double* __restrict__ part, int cpc, int r, int ncols_part, int gcol, int hcol,
            int nh, double* __restrict__ S, double* __restrict__ R, int swap) {
+  constexpr int EPB = 256 / KS;
+  __shared__ double ps[KS > 1 ? KS : 1][EPB];
   const int node = blockIdx.y;
   const int n2 = 2 * r;
+  const int g = threadIdx.x / EPB, le = threadIdx.x % EPB;
+  const int q0 = (int)((int64_t)cpc * g / KS), q1 = (int)((int64_t)cpc * (g + 1) / KS);
   const int64_t pstride = (int64_t)r * ncols_part;
   const int64_t nS = S ? (int64_t)n2 * n2 : 0;
   const int64_t total = nS + (int64_t)n2 * nh;
-  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * 256) {
+  for (int64_t e0 = (int64_t)blockIdx.x * EPB; e0 < total; e0 += (int64_t)gridDim.x * EPB) {
+    const int64_t e = e0 + le;
+    double v = 0.0;
+    bool sum = false;  // v is a partial sum of this group's leaves
+    double* dst = nullptr;
     if (e < nS) {
       const int i = (int)(e % n2), j = (int)(e / n2);
-      double v;
+      dst = S + (int64_t)node * nS + i + (int64_t)j * n2;
       if ((i < r) == (j < r)) {
         v = (i == j) ? 1.0 : 0.0;
       } else if (i >= r) {  // S(r+a, j) = V0^T u0 (a, j)
-        v = sum_part(part, 2 * node, cpc, pstride, (i - r) + (int64_t)(gcol + j) * r);
+        v = sum_part(part, 2 * node, cpc, pstride, (i - r) + (int64_t)(gcol + j) * r, q0, q1);
+        sum = true;
       } else {              // S(a, r+j) = V1^T u1 (a, j)
-        v = sum_part(part, 2 * node + 1, cpc, pstride, i + (int64_t)(gcol + j - r) * r);
+        v = sum_part(part, 2 * node + 1, cpc, pstride, i + (int64_t)(gcol + j - r) * r, q0, q1);
+        sum = true;
       }
-      S[(int64_t)node * nS + i + (int64_t)j * n2] = v;
-    } else {
+    } else if (e < total) {
       const int64_t f = e - nS;
       const int i = (int)(f % n2), j = (int)(f / n2);
       const int hi = i >= r;  // lower half of R
       const int child = 2 * node + (swap ? 1 - hi : hi);
-      const double v = sum_part(part, child, cpc, pstride, (i - hi * r) + (int64_t)(hcol + j) * r);
-      R[(int64_t)node * n2 * nh + i + (int64_t)j * n2] = v;
+      dst = R + (int64_t)node * n2 * nh + i + (int64_t)j * n2;
+      v = sum_part(part, child, cpc, pstride, (i - hi * r) + (int64_t)(hcol + j) * r, q0, q1);
+      sum = true;
+    }
+    if (KS > 1) {
+      ps[g][le] = v;
+      __syncthreads();
+      if (g == 0 && dst) {
+        if (sum)
+#pragma unroll
+          for (int h = 1; h < KS; ++h) v += ps[h][le];
+        *dst = v;
+      }
+      __syncthreads();
+    } else if (dst) {
+      *dst = v;
     }
   }
 }
@@ -2083,11 +2112,19 @@ hipError_t launch_assemble(const double* part, int cpc, int r, int ncols_part, i
                            bool swap) {
   if (nnode <= 0) return hipSuccess;
   const int64_t total = (S ? 4LL * r * r : 0) + 2LL * r * nh;
-  int gx = (int)((total + 255) / 256);
-  if (gx > 64) gx = 64;
-  if (gx < 1) gx = 1;
-  hipLaunchKernelGGL(k_assemble, dim3(gx, nnode), dim3(256), 0, st, part, cpc, r, ncols_part,
-                     gcol, hcol, nh, S, R, swap ? 1 : 0);
+  if (cpc >= 16) {
+    int gx = (int)((total + 63) / 64);
+    if (gx > 1024) gx = 1024;
+    if (gx < 1) gx = 1;
+    hipLaunchKernelGGL(k_assemble<4>, dim3(gx, nnode), dim3(256), 0, st, part, cpc, r, ncols_part,
+                       gcol, hcol, nh, S, R, swap ? 1 : 0);
+  } else {
+    int gx = (int)((total + 255) / 256);
+    if (gx > 256) gx = 256;
+    if (gx < 1) gx = 1;
+    hipLaunchKernelGGL(k_assemble<1>, dim3(gx, nnode), dim3(256), 0, st, part, cpc, r, ncols_part,
+                       gcol, hcol, nh, S, R, swap ? 1 : 0);
+  }
   return hipGetLastError();
 }
 
