@@ -875,60 +875,88 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
       // outer block complete (all its panels full): block forward
       // substitution U_s = L_ss^-1 (A_s - sum_{q<s} L_sq U_q) on the columns
       // right of it, then the trailing update with inner dimension cend - J0
-      const int nsp = (cend - J0) / NB;
+      // (cend = J0 + S*NB here, so all S panels are full).  The L_sq
+      // blocks (q < s) are the same for every column tile: staged once in
+      // the part of the LDS pool the panel phase does not use; a column
+      // tile's right-hand sides of all S panels are fetched together.
       const int ntr = n - cend;
       const int nct = (ntr + 15) >> 4;
+      static_assert(LU_ELEMS + S * (S - 1) / 2 * NB * (NB + 1) <= (LU_ELEMS > UP_ELEMS ? LU_ELEMS : UP_ELEMS),
+                    "L_sq staging fits the pool");
+      // block (s2, q), q < s2, at (s2*(s2-1)/2 + q)*NB*(NB+1), [row*(NB+1) + k]
+      T* const Lq = pool + LU_ELEMS;
+#pragma unroll
+      for (int s2 = 0; s2 < S; ++s2)
+#pragma unroll
+        for (int q = 0; q < S; ++q)
+          if (q < s2) {
+            for (int e = t; e < NB * NB; e += NT) {
+              const int row = e % NB, k = e / NB;
+              Lq[(s2 * (s2 - 1) / 2 + q) * NB * (NB + 1) + row * (NB + 1) + k] =
+                  A[(int64_t)(J0 + s2 * NB + row) + (int64_t)(J0 + q * NB + k) * lda];
+            }
+          }
+      lds_barrier();
+      // element (s2, a, v) of a tile: row J0 + s2*NB + a*16 + crow(lane, v),
+      // column cend + ct*16 + (lane & 15)
+      const int64_t lo = (int64_t)Elem<T>::crow(lane, 0) + (int64_t)(lane & 15) * lda;
+      T cur[S][NB / 16][4];
+      auto fetch = [&](int ct, T (&d)[S][NB / 16][4]) {
+        const T* bp = A + J0 + (int64_t)(cend + ct * 16) * lda + lo;
+        const bool cok = ct * 16 + (lane & 15) < ntr;
+#pragma unroll
+        for (int s2 = 0; s2 < S; ++s2)
+#pragma unroll
+          for (int a = 0; a < NB / 16; ++a)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) d[s2][a][v] = cok ? bp[s2 * NB + a * 16 + v * Elem<T>::DV] : T(0);
+      };
       for (int ct = wid; ct < nct; ct += NW) {
-        const int col = ct * 16 + (lane & 15);
-        const bool cok = col < ntr;
+        fetch(ct, cur);
+        const bool cok = ct * 16 + (lane & 15) < ntr;
+        T* const bp = A + J0 + (int64_t)(cend + ct * 16) * lda + lo;
         V4 u[S][NB / 16];
 #pragma unroll
         for (int s2 = 0; s2 < S; ++s2) {
-          if (s2 < nsp) {
-            const int r0 = J0 + s2 * NB;
-            V4 acc[NB / 16];
+          V4 acc[NB / 16];
 #pragma unroll
-            for (int a = 0; a < NB / 16; ++a)
+          for (int a = 0; a < NB / 16; ++a)
 #pragma unroll
-              for (int v = 0; v < 4; ++v)
-                acc[a][v] = cok ? A[(int64_t)(r0 + a * 16 + Elem<T>::crow(lane, v)) + (int64_t)(cend + col) * lda] : 0.0;
+            for (int v = 0; v < 4; ++v) acc[a][v] = cur[s2][a][v];
 #pragma unroll
-            for (int q = 0; q < S; ++q) {
-              if (q < s2) {
-                // k-step (a2, v): rows a2*16 + crow(lane, v) of U_q are
-                // exactly the accumulator element v of tile a2
+          for (int q = 0; q < S; ++q) {
+            if (q < s2) {
+              // k-step (a2, v): rows a2*16 + crow(lane, v) of U_q are
+              // exactly the accumulator element v of tile a2
 #pragma unroll
-                for (int a2 = 0; a2 < NB / 16; ++a2)
+              for (int a2 = 0; a2 < NB / 16; ++a2)
 #pragma unroll
-                  for (int v = 0; v < 4; ++v) {
-                    const int k = a2 * 16 + Elem<T>::crow(lane, v);
-                    const T bf = u[q][a2][v];
+                for (int v = 0; v < 4; ++v) {
+                  const T bf = u[q][a2][v];
+                  const int k = a2 * 16 + Elem<T>::crow(lane, v);
 #pragma unroll
-                    for (int a = 0; a < NB / 16; ++a) {
-                      const T af = A[(int64_t)(r0 + a * 16 + (lane & 15)) + (int64_t)(J0 + q * NB + k) * lda];
-                      acc[a] = mfma(-af, bf, acc[a]);
-                    }
-                  }
-              }
+                  for (int a = 0; a < NB / 16; ++a)
+                    acc[a] = mfma(-Lq[(s2 * (s2 - 1) / 2 + q) * NB * (NB + 1) + (a * 16 + (lane & 15)) * (NB + 1) + k], bf, acc[a]);
+                }
+            }
+          }
+#pragma unroll
+          for (int a = 0; a < NB / 16; ++a) u[s2][a] = zero4<T>();
+#pragma unroll
+          for (int a2 = 0; a2 < NB / 16; ++a2)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int k = a2 * 16 + Elem<T>::crow(lane, v);
+              const T bf = acc[a2][v];
+#pragma unroll
+              for (int a = 0; a < NB / 16; ++a)
+                u[s2][a] = mfma(Xis[s2 * NB * (NB + 1) + (a * 16 + (lane & 15)) * (NB + 1) + k], bf, u[s2][a]);
             }
 #pragma unroll
-            for (int a = 0; a < NB / 16; ++a) u[s2][a] = zero4<T>();
+          for (int a = 0; a < NB / 16; ++a)
 #pragma unroll
-            for (int a2 = 0; a2 < NB / 16; ++a2)
-#pragma unroll
-              for (int v = 0; v < 4; ++v) {
-                const int k = a2 * 16 + Elem<T>::crow(lane, v);
-                const T bf = acc[a2][v];
-#pragma unroll
-                for (int a = 0; a < NB / 16; ++a)
-                  u[s2][a] = mfma(Xis[s2 * NB * (NB + 1) + (a * 16 + (lane & 15)) * (NB + 1) + k], bf, u[s2][a]);
-              }
-#pragma unroll
-            for (int a = 0; a < NB / 16; ++a)
-#pragma unroll
-              for (int v = 0; v < 4; ++v)
-                if (cok) A[(int64_t)(r0 + a * 16 + Elem<T>::crow(lane, v)) + (int64_t)(cend + col) * lda] = u[s2][a][v];
-          }
+            for (int v = 0; v < 4; ++v)
+              if (cok) bp[s2 * NB + a * 16 + v * Elem<T>::DV] = u[s2][a][v];
         }
       }
       __syncthreads();
