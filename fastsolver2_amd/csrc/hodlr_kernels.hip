@@ -844,9 +844,15 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
 
     const int o = j0 + nb;
     // inside the outer block: U = L11^-1 A(j0.., o..cend), then the rows
-    // below take the rank-nb update on those columns only
+    // below take the rank-nb update on those columns only.  U also goes to
+    // the part of the LDS pool the panel phase does not use, so the update
+    // reads it from there; each wave's 16-row tiles load their L fragments
+    // and C values together.
     if (o < cend) {
-      const int m2b = cend - o;
+      const int m2b = cend - o;  // <= NB: the outer block's remaining panels
+      static_assert(LU_ELEMS + NB * (NB + 1) <= (LU_ELEMS > UP_ELEMS ? LU_ELEMS : UP_ELEMS),
+                    "U staging fits the pool");
+      T* const Us = pool + LU_ELEMS;  // [k*(NB+1) + col]
       const int nct = (m2b + 15) >> 4;
       for (int ct = wid; ct < nct; ct += NW) {
         const int col = ct * 16 + (lane & 15);
@@ -866,11 +872,47 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
             const int r = q * 16 + Elem<T>::crow(lane, v);
+            // rows past nb and columns past m2b are exact zeros
+            Us[r * (NB + 1) + col] = u[q][v];
             if (col < m2b && r < nb) A[(int64_t)(j0 + r) + (int64_t)(o + col) * lda] = u[q][v];
           }
       }
-      __syncthreads();
-      update(o, n - o, o, m2b, j0, nb);
+      lds_barrier();  // Us; the L panel was written before the last full barrier
+      const int nr = n - o;
+      const int nrt = (nr + 15) >> 4;
+      constexpr int TN = NB / 16;  // column tiles of the outer block's remainder
+      for (int rt = wid; rt < nrt; rt += NW) {
+        const bool full = rt * 16 + 16 <= nr && m2b == NB;
+        T af[NB / 4];
+        V4 c[TN];
+        const int arow = rt * 16 + (lane & 15);
+#pragma unroll
+        for (int kk = 0; kk < NB / 4; ++kk) {
+          const int k = kk * 4 + (lane >> 4);
+          af[kk] = (full || (arow < nr && k < nb)) ? A[(int64_t)(o + arow) + (int64_t)(j0 + k) * lda] : T(0);
+        }
+#pragma unroll
+        for (int tn = 0; tn < TN; ++tn)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = rt * 16 + Elem<T>::crow(lane, v), col = tn * 16 + (lane & 15);
+            c[tn][v] = (full || (row < nr && col < m2b)) ? A[(int64_t)(o + row) + (int64_t)(o + col) * lda] : T(0);
+          }
+#pragma unroll
+        for (int kk = 0; kk < NB / 4; ++kk) {
+          const int k = kk * 4 + (lane >> 4);
+#pragma unroll
+          for (int tn = 0; tn < TN; ++tn)
+            c[tn] = mfma(-af[kk], Us[k * (NB + 1) + tn * 16 + (lane & 15)], c[tn]);
+        }
+#pragma unroll
+        for (int tn = 0; tn < TN; ++tn)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            const int row = rt * 16 + Elem<T>::crow(lane, v), col = tn * 16 + (lane & 15);
+            if (full || (row < nr && col < m2b)) A[(int64_t)(o + row) + (int64_t)(o + col) * lda] = c[tn][v];
+          }
+      }
     } else if (cend < n) {
       // outer block complete (all its panels full): block forward
       // substitution U_s = L_ss^-1 (A_s - sum_{q<s} L_sq U_q) on the columns
