@@ -1502,37 +1502,53 @@ k_trtri_diag(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
   const int bs = min(kTri, n - i0);
   const T* LU = LU0 + (int64_t)b * bstride;
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < kTri * kTri; idx += 64) {
-    const int c = idx / kTri, r = idx % kTri;
-    T v;
-    if (r < bs && c < bs) v = LU[(int64_t)(i0 + r) + (int64_t)(i0 + c) * lda];
-    else v = (r == c) ? 1.0 : 0.0;
-    Ts[c][r] = v;
+  // all 16 loads of a lane in flight together
+  T v[kTri * kTri / 64];
+#pragma unroll
+  for (int q = 0; q < kTri * kTri / 64; ++q) {
+    const int idx = tid + 64 * q, c = idx / kTri, r = idx % kTri;
+    v[q] = (r < bs && c < bs) ? LU[(int64_t)(i0 + r) + (int64_t)(i0 + c) * lda] : ((r == c) ? T(1) : T(0));
+  }
+#pragma unroll
+  for (int q = 0; q < kTri * kTri / 64; ++q) {
+    const int idx = tid + 64 * q;
+    Ts[idx / kTri][idx % kTri] = v[q];
   }
   __syncthreads();
   T* out = dinv0 + (int64_t)b * dinv_elems(n) + (int64_t)ib * 2 * kTri * kTri;
   const int c = tid & 31;
+  // column-oriented substitution: once x[k] is final, every later row takes
+  // its term at once (independent updates, the LDS reads of a factor column
+  // in flight together) -- the dependent chain is 32 steps, not 496
   T x[kTri];
+#pragma unroll
+  for (int i = 0; i < kTri; ++i) x[i] = (i == c) ? T(1) : T(0);
   if (tid < 32) {
-    // L x = e_c, unit lower
+    // L x = e_c, unit lower: rows i > k take -L[i][k] x[k], k ascending
 #pragma unroll
-    for (int i = 0; i < kTri; ++i) {
-      T acc = (i == c) ? 1.0 : 0.0;
+    for (int k = 0; k < kTri; ++k) {
+      // opaque: x[k] is formed here, so the updates below cannot be sunk
+      // into later per-row chains
+      const T xk = opaque((k < c) ? T(0) : x[k]);
+      x[k] = xk;
+      // an opaque column offset per step keeps the step's reads in the step
+      const T* const col = &Ts[0][0] + opaque(k * (kTri + 1));
 #pragma unroll
-      for (int k = 0; k < i; ++k) acc -= Ts[k][i] * x[k];
-      x[i] = (i < c) ? 0.0 : acc;
+      for (int i = k + 1; i < kTri; ++i) x[i] -= col[i] * xk;
     }
 #pragma unroll
     for (int i = 0; i < kTri; ++i) out[i + c * kTri] = x[i];
   } else {
-    // U x = e_c, upper non-unit
+    // U x = e_c, upper non-unit: rows i < k take -U[i][k] x[k], k descending
 #pragma unroll
-    for (int ii = 0; ii < kTri; ++ii) {
-      const int i = kTri - 1 - ii;
-      T acc = (i == c) ? 1.0 : 0.0;
+    for (int kk = 0; kk < kTri; ++kk) {
+      const int k = kTri - 1 - kk;
+      const T* const col = &Ts[0][0] + opaque(k * (kTri + 1));
+      const T d = opaque(x[k] / col[k]);  // always formed (no branch around the divide)
+      const T xk = opaque((k > c) ? T(0) : d);
+      x[k] = xk;
 #pragma unroll
-      for (int k = i + 1; k < kTri; ++k) acc -= Ts[k][i] * x[k];
-      x[i] = (i > c) ? 0.0 : acc / Ts[i][i];
+      for (int i = 0; i < k; ++i) x[i] -= col[i] * xk;
     }
 #pragma unroll
     for (int i = 0; i < kTri; ++i) out[kTri * kTri + i + c * kTri] = x[i];
