@@ -2706,27 +2706,45 @@ __global__ void __launch_bounds__(256) k_bgemm(BGemmArgs g) {
     for (int a = 0; a < GM; ++a)
 #pragma unroll
       for (int j = 0; j < GN; ++j) acc[a][j] = d4zero();
-#pragma unroll 4
-    for (int k0 = 0; k0 < g.K; k0 += 4) {
-      const int k = k0 + lk;
-      double af[GM], bf[GN];
+    // the inner dimension in chunks of 16 (four k-steps); the next chunk's
+    // A and B values are fetched under the current chunk's MFMAs
+    double afc[4][GM], bfc[4][GN], afn[4][GM], bfn[4][GN];
+    auto fetch = [&](int k0, double (&af)[4][GM], double (&bf)[4][GN]) {
 #pragma unroll
-      for (int a = 0; a < GM; ++a) {
-        const int arow = (tm0 + a) * 16 + lr;
-        af[a] = 0.0;
-        if (k < g.K && arow < g.M)
-          af[a] = !A ? (arow == k ? 1.0 : 0.0)
-                     : (g.transA ? A[k + (int64_t)arow * g.lda] : A[arow + (int64_t)k * g.lda]);
+      for (int kk = 0; kk < 4; ++kk) {
+        const int k = k0 + kk * 4 + lk;
+#pragma unroll
+        for (int a = 0; a < GM; ++a) {
+          const int arow = (tm0 + a) * 16 + lr;
+          double v = 0.0;
+          if (k < g.K && arow < g.M)
+            v = !A ? (arow == k ? 1.0 : 0.0)
+                   : (g.transA ? A[k + (int64_t)arow * g.lda] : A[arow + (int64_t)k * g.lda]);
+          af[kk][a] = v;
+        }
+#pragma unroll
+        for (int j = 0; j < GN; ++j) {
+          const int bcol = (tn0 + j) * 16 + lr;
+          bf[kk][j] = (on[j] && k < g.K && bcol < g.Nc) ? Bp[k + (int64_t)bcol * g.ldb] : 0.0;
+        }
       }
+    };
+    fetch(0, afc, bfc);
+    for (int k0 = 0; k0 < g.K; k0 += 16) {
+      if (k0 + 16 < g.K) fetch(k0 + 16, afn, bfn);
 #pragma unroll
-      for (int j = 0; j < GN; ++j) {
-        const int bcol = (tn0 + j) * 16 + lr;
-        bf[j] = (on[j] && k < g.K && bcol < g.Nc) ? Bp[k + (int64_t)bcol * g.ldb] : 0.0;
+      for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+        for (int a = 0; a < GM; ++a)
+#pragma unroll
+          for (int j = 0; j < GN; ++j) acc[a][j] = mfma(afc[kk][a], bfc[kk][j], acc[a][j]);
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk) {
+#pragma unroll
+        for (int a = 0; a < GM; ++a) afc[kk][a] = afn[kk][a];
+#pragma unroll
+        for (int j = 0; j < GN; ++j) bfc[kk][j] = bfn[kk][j];
       }
-#pragma unroll
-      for (int a = 0; a < GM; ++a)
-#pragma unroll
-        for (int j = 0; j < GN; ++j) acc[a][j] = mfma(af[a], bf[j], acc[a][j]);
     }
     // epilogue: each lane's output column picks its side (C1 or C2); all D
     // values of the task are read before any C is written (C may alias D
