@@ -1705,6 +1705,10 @@ k_getrs_reg(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
   typedef typename Elem<T>::V4 V4;
   __shared__ T xa[kTri * (W + 1)];  // raw block (owners -> all)
   __shared__ T xs[kTri * (W + 1)];  // solved block
+  // the 32x32 diagonal inverse of each step, double-buffered: copied from
+  // global memory straight into LDS (global_load_lds, 16 bytes per lane)
+  // during the previous step's updates
+  __shared__ __align__(16) T Ds[2][kTri * kTri];
   const int G = gridDim.x;
   const int gq = G >> 3, grem = G & 7;
   const int xcd = blockIdx.x & 7;
@@ -1724,6 +1728,24 @@ k_getrs_reg(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
   const uint32_t ldb = (uint32_t)lda * sizeof(T);
   const uint32_t lane_off = TRANS ? lk * (uint32_t)sizeof(T) + lr * ldb : lr * (uint32_t)sizeof(T) + lk * ldb;
   const uint32_t kstep = TRANS ? 4 * (uint32_t)sizeof(T) : 4 * ldb;
+
+  // Di of global step gs (sweep-major) into Ds[gs & 1]: waves w <
+  // kTri*kTri*sizeof(T)/1024 copy 1 KB each; the copy lands before the next
+  // full barrier (it waits for vmcnt)
+  constexpr int DSW = kTri * kTri * (int)sizeof(T) / 1024;
+  static_assert(DSW <= 8 && DSW * 1024 == kTri * kTri * (int)sizeof(T), "Di copy split");
+  auto prefetch_di = [&](int gs) {
+    const bool f = gs < nblk;
+    const int ibn = f ? gs : nblk - 1 - (gs - nblk);
+    const int isl = TRANS ? (f ? 1 : 0) : (f ? 0 : 1);
+    if (wid < DSW) {
+      const char* src = (const char*)(dinv + (int64_t)ibn * 2 * kTri * kTri + isl * kTri * kTri) + wid * 1024 + lane * 16;
+      char* dst = (char*)&Ds[gs & 1][0] + wid * 1024;
+      __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                       (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+    }
+  };
+  prefetch_di(0);
 
   V4 acc[4][NCT];
   // gather (rows by perm for the forward solve)
@@ -1747,7 +1769,8 @@ k_getrs_reg(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
     for (int step = 0; step < nblk; ++step) {
       const int ib = fwd ? step : nblk - 1 - step;
       const int i0 = ib * kTri;
-      const T* Di = dinv + (int64_t)ib * 2 * kTri * kTri + islot * kTri * kTri;
+      const int gs = sweep * nblk + step;
+      const T* Di = Ds[gs & 1];
       // (a1) owners of row tiles 2ib, 2ib+1 stage the raw block
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -1776,6 +1799,7 @@ k_getrs_reg(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
         for (int v = 0; v < 4; ++v) xs[(rtl * 16 + Elem<T>::crow(lane, v)) * (W + 1) + ct * 16 + lr] = a4[v];
       }
       __syncthreads();
+      if (gs + 1 < 2 * nblk) prefetch_di(gs + 1);
       // (b) owners take the solved block back; other tiles on the far side
       // of the block are updated with the off-diagonal factor block.  The
       // tiles to update form a run of q; the factor fragments of the next
