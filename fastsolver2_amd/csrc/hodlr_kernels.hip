@@ -731,9 +731,13 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
       const int kr = t % KC, cbt = __builtin_amdgcn_readfirstlane(t / KC);
       const uint32_t voa = ra * SZ + ka * ldb;
       const uint32_t vob = kr * SZ + cbt * ldb;
-      // C tile: lane offset of (crow(lane, 0), lane & 15)
-      const int lr = Elem<T>::crow(lane, 0);
-      const uint32_t voc = lr * SZ + (lane & 15) * ldb;
+      // The MFMAs form the transposed tile (U^T L^T, the U panel as the A
+      // operand): accumulator element (i, j) is C(row j, column i), so a
+      // lane holds rows lane & 15 of columns crow(lane, v) and each 16-lane
+      // group of a load or store covers 16 consecutive rows of a column
+      // (128-byte lines).  Lane offset of (lane & 15, crow(lane, 0)):
+      const int lc = Elem<T>::crow(lane, 0);
+      const uint32_t voc = (lane & 15) * SZ + lc * ldb;
       T pa[PA], pb[PB];
       auto fetch_u = [&](int q) {
         const int C0 = (q % nbc) * BC;
@@ -777,7 +781,7 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
         const __amdgpu_buffer_rsrc_t rc = buf_rsrc(A + (rb + r0) + (int64_t)(cb + c0) * lda);
         const uint32_t vc = opaque(voc);
         auto in_c = [&](int a, int bb, int v) {
-          return r0 + a * 16 + lr + v * Elem<T>::DV < nr && c0 + bb * 16 + (lane & 15) < nc;
+          return r0 + a * 16 + (lane & 15) < nr && c0 + bb * 16 + lc + v * Elem<T>::DV < nc;
         };
         T cv[2][TC][4];
         if (full) {
@@ -787,7 +791,7 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
             for (int bb = 0; bb < TC; ++bb)
 #pragma unroll
               for (int v = 0; v < 4; ++v)
-                cv[a][bb][v] = buf_load<T>(rc, vc, bb * 16 * ldb + (a * 16 + v * Elem<T>::DV) * SZ);
+                cv[a][bb][v] = buf_load<T>(rc, vc, (bb * 16 + v * Elem<T>::DV) * ldb + a * 16 * SZ);
         } else if (act) {
 #pragma unroll
           for (int a = 0; a < 2; ++a)
@@ -796,7 +800,7 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
 #pragma unroll
               for (int v = 0; v < 4; ++v)
                 cv[a][bb][v] = in_c(a, bb, v)
-                                   ? buf_load<T>(rc, vc, bb * 16 * ldb + (a * 16 + v * Elem<T>::DV) * SZ)
+                                   ? buf_load<T>(rc, vc, (bb * 16 + v * Elem<T>::DV) * ldb + a * 16 * SZ)
                                    : T(0);
         }
         if (q + 1 < ntile) fetch_u(q + 1);
@@ -818,7 +822,7 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
 #pragma unroll
             for (int a = 0; a < 2; ++a)
 #pragma unroll
-              for (int bb = 0; bb < TC; ++bb) acc[a][bb] = mfma(af[a], bf[bb], acc[a][bb]);
+              for (int bb = 0; bb < TC; ++bb) acc[a][bb] = mfma(bf[bb], af[a], acc[a][bb]);
           }
           if (full) {
 #pragma unroll
@@ -827,7 +831,7 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
               for (int bb = 0; bb < TC; ++bb)
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
-                  buf_store(acc[a][bb][v] + cv[a][bb][v], rc, vc, bb * 16 * ldb + (a * 16 + v * Elem<T>::DV) * SZ);
+                  buf_store(acc[a][bb][v] + cv[a][bb][v], rc, vc, (bb * 16 + v * Elem<T>::DV) * ldb + a * 16 * SZ);
           } else {
 #pragma unroll
             for (int a = 0; a < 2; ++a)
@@ -836,7 +840,7 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
                   if (in_c(a, bb, v))
-                    buf_store(acc[a][bb][v] + cv[a][bb][v], rc, vc, bb * 16 * ldb + (a * 16 + v * Elem<T>::DV) * SZ);
+                    buf_store(acc[a][bb][v] + cv[a][bb][v], rc, vc, (bb * 16 + v * Elem<T>::DV) * ldb + a * 16 * SZ);
           }
         }
       }
