@@ -895,11 +895,13 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
           const int k = kk * 4 + (lane >> 4);
           af[kk] = (full || (arow < nr && k < nb)) ? A[(int64_t)(o + arow) + (int64_t)(j0 + k) * lda] : T(0);
         }
+        // transposed tiles (U as the A operand): a lane holds row lane & 15
+        // of columns crow(lane, v), so C moves in whole 128-byte lines
 #pragma unroll
         for (int tn = 0; tn < TN; ++tn)
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            const int row = rt * 16 + Elem<T>::crow(lane, v), col = tn * 16 + (lane & 15);
+            const int row = rt * 16 + (lane & 15), col = tn * 16 + Elem<T>::crow(lane, v);
             c[tn][v] = (full || (row < nr && col < m2b)) ? A[(int64_t)(o + row) + (int64_t)(o + col) * lda] : T(0);
           }
 #pragma unroll
@@ -907,13 +909,13 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
           const int k = kk * 4 + (lane >> 4);
 #pragma unroll
           for (int tn = 0; tn < TN; ++tn)
-            c[tn] = mfma(-af[kk], Us[k * (NB + 1) + tn * 16 + (lane & 15)], c[tn]);
+            c[tn] = mfma(Us[k * (NB + 1) + tn * 16 + (lane & 15)], -af[kk], c[tn]);
         }
 #pragma unroll
         for (int tn = 0; tn < TN; ++tn)
 #pragma unroll
           for (int v = 0; v < 4; ++v) {
-            const int row = rt * 16 + Elem<T>::crow(lane, v), col = tn * 16 + (lane & 15);
+            const int row = rt * 16 + (lane & 15), col = tn * 16 + Elem<T>::crow(lane, v);
             if (full || (row < nr && col < m2b)) A[(int64_t)(o + row) + (int64_t)(o + col) * lda] = c[tn][v];
           }
       }
