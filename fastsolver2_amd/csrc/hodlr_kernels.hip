@@ -2099,18 +2099,39 @@ k_reduce(const double* __restrict__ V, int64_t ldv, int r, ColSeg X, int ncols,
 #pragma unroll
   for (int ct = 0; ct < NCT; ++ct) acc[ct] = d4zero();
 
-  for (int64_t k0 = rbeg; k0 < rend; k0 += KS) {
-    for (int idx = tid; idx < 64 * KS; idx += 256) {
-      const int a = idx / KS, k = idx % KS;
+  // row chunks of KS; the next chunk's V and X values are fetched into
+  // registers under the current chunk's MFMAs
+  constexpr int VPT = 64 * KS / 256, XPT = NCT * 16 * KS / 256;
+  static_assert(VPT * 256 == 64 * KS && XPT * 256 == NCT * 16 * KS, "staging split");
+  double vr[VPT], xr[XPT];
+  auto fetch = [&](int64_t k0) {
+#pragma unroll
+    for (int i = 0; i < VPT; ++i) {
+      const int idx = tid + 256 * i, a = idx / KS, k = idx % KS;
       const int64_t row = k0 + k;
-      Vs[a * (KS + 1) + k] = (a0 + a < r && row < rend) ? V[row + (int64_t)(a0 + a) * ldv] : 0.0;
+      vr[i] = (a0 + a < r && row < rend) ? V[row + (int64_t)(a0 + a) * ldv] : 0.0;
     }
-    for (int idx = tid; idx < NCT * 16 * KS; idx += 256) {
-      const int j = idx / KS, k = idx % KS;
+#pragma unroll
+    for (int i = 0; i < XPT; ++i) {
+      const int idx = tid + 256 * i, j = idx / KS, k = idx % KS;
       const int64_t row = k0 + k;
-      Xs[j * (KS + 1) + k] = (j0 + j < ncols && row < rend) ? seg_in(X, j0 + j)[row] : 0.0;
+      xr[i] = (j0 + j < ncols && row < rend) ? seg_in(X, j0 + j)[row] : 0.0;
+    }
+  };
+  if (rbeg < rend) fetch(rbeg);
+  for (int64_t k0 = rbeg; k0 < rend; k0 += KS) {
+#pragma unroll
+    for (int i = 0; i < VPT; ++i) {
+      const int idx = tid + 256 * i;
+      Vs[(idx / KS) * (KS + 1) + idx % KS] = vr[i];
+    }
+#pragma unroll
+    for (int i = 0; i < XPT; ++i) {
+      const int idx = tid + 256 * i;
+      Xs[(idx / KS) * (KS + 1) + idx % KS] = xr[i];
     }
     __syncthreads();
+    if (k0 + KS < rend) fetch(k0 + KS);
 #pragma unroll
     for (int kk = 0; kk < KS / 4; ++kk) {
       const double af = Vs[(wid * 16 + lr) * (KS + 1) + kk * 4 + lk];
