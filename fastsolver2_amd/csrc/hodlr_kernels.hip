@@ -512,20 +512,22 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
     bool live = own;
     const bool wact = __ballot(own) != 0ULL;  // wave-uniform
 
+    // a wave with no row in the panel publishes "no candidate" and keeps
+    // the barrier count, in a loop of its own (no merge with the register-
+    // resident panel below, so no register copies per column)
+    if (!wact) {
+      for (int jj = 0; jj < nb; ++jj) {
+        if (lane == 0) {
+          ckey[jj & 1][wid] = 0ULL;
+          cidx[jj & 1][wid] = INT_MAX;
+        }
+        lds_barrier();
+      }
+    } else
 #pragma unroll
     for (int jj = 0; jj < NB; ++jj) {
       if (jj < nb) {
         const int par = jj & 1;
-        if (!wact) {
-          // no row of this wave is in the panel: publish "no candidate"
-          // and keep the barrier count
-          if (lane == 0) {
-            ckey[par][wid] = 0ULL;
-            cidx[par][wid] = INT_MAX;
-          }
-          lds_barrier();
-          continue;
-        }
         // key = bits(|a|) + 1 orders non-negative doubles; a NaN counts as
         // zero; 0 = not a candidate (retired rows, rows past n)
         const T aj = p[jj];
