@@ -114,6 +114,30 @@ def test_dgesv_batched_pivoting(gpu, orc, n):
             assert np.abs(LU[:, b * n:(b + 1) * n] - LUo).max() <= 1e-11 * np.abs(LUo).max()
 
 
+@pytest.mark.parametrize("n,lda", [(300, (1 << 22) + 64), (40, (1 << 24) + 64)])
+def test_dgesv_large_leading_dimension(gpu, n, lda):
+    """Leading dimensions past the 32-bit byte-offset limits of the fast
+    paths (the LDS-staged trailing update needs lda <= 2^22, the register
+    solve 32 * lda * 8 < 2^32) take the 64-bit-address paths; their results
+    must match numpy (the operand is a strided view into a large buffer)."""
+    say(f"case: dgesv_batched n={n} lda={lda}")
+    F = gpu
+    rng = np.random.default_rng(n)
+    Ah = rng.standard_normal((n, n))
+    Bh = rng.standard_normal((n, 3))
+    buf = torch.empty(lda * (n - 1) + n, dtype=torch.float64, device="cuda")
+    A = torch.as_strided(buf, (n, n), (1, lda))
+    A.copy_(torch.from_numpy(np.asfortranarray(Ah)))
+    B = to_dev(F, Bh)
+    info, ipiv = F.dgesv_batched(A, B, n, 1)
+    assert info == 0
+    X = to_host(B)
+    ref = np.linalg.solve(Ah, Bh)
+    assert relerr(X, ref) <= 1e-15 * np.linalg.cond(Ah) * n + 1e-13
+    del A, buf
+    torch.cuda.empty_cache()
+
+
 def test_dgesv_batched_singular(gpu):
     say("case: dgesv_batched singular")
     F = gpu
