@@ -2848,7 +2848,9 @@ hipError_t launch_bgemm(const BGemmArgs& g, hipStream_t st) {
   if (g.batch <= 0 || g.M <= 0 || g.Nc <= 0) return hipSuccess;
   const int mt = (g.M + 15) >> 4, nt = (g.Nc + 15) >> 4;
   // group tiles only while every wave keeps a task
-  if (mt * nt >= 16 && nt >= 4)
+  if (nt == 5 && mt >= 4)  // r + s in (64, 80]: one task per row tile, no idle column tiles
+    hipLaunchKernelGGL((k_bgemm<1, 5>), dim3((unsigned)g.batch), dim3(256), 0, st, g);
+  else if (mt * nt >= 16 && nt >= 4)
     hipLaunchKernelGGL((k_bgemm<1, 4>), dim3((unsigned)g.batch), dim3(256), 0, st, g);
   else if (mt * nt >= 16 && mt >= 4 && nt == 1)
     hipLaunchKernelGGL((k_bgemm<4, 1>), dim3((unsigned)g.batch), dim3(256), 0, st, g);
