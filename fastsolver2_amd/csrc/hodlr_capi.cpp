@@ -434,9 +434,9 @@ int level_phase(hodlr_plan p, double* B, int64_t ldb, int nrhs, hipStream_t st) 
       g.D2 = p->G; g.ldd2 = r; g.sd2 = (int64_t)r * nrhs;
       g.C2 = p->G; g.ldc2 = r; g.sc2 = (int64_t)r * nrhs;
     }
-    if (g.C1 || g.C2) HIP_TRY(hodlr::launch_bgemm(g, st));
     if (t > 0) {
-      // P_{t-1,i} = P_t,i - P_t,i(:, 0:r) E_c
+      // P_{t-1,i} = P_t,i - P_t,i(:, 0:r) E_c, independent of the W/G
+      // product above: both go in one launch
       hodlr::BGemmArgs h;
       h.batch = (int)P; h.M = r; h.K = r; h.Nc = nc; h.csplit = nc;
       h.A = Pc; h.lda = r; h.sa = (int64_t)r * nc;
@@ -444,8 +444,13 @@ int level_phase(hodlr_plan p, double* B, int64_t ldb, int nrhs, hipStream_t st) 
       h.alpha1 = -1.0; h.beta1 = 1.0;
       h.D1 = Pc; h.ldd1 = r; h.sd1 = (int64_t)r * nc;
       h.C1 = Pn; h.ldc1 = r; h.sc1 = (int64_t)r * nc;
-      HIP_TRY(hodlr::launch_bgemm(h, st));
+      if (g.C1 || g.C2)
+        HIP_TRY(hodlr::launch_bgemm2(g, h, st));
+      else
+        HIP_TRY(hodlr::launch_bgemm(h, st));
       std::swap(Pc, Pn);
+    } else if (g.C1 || g.C2) {
+      HIP_TRY(hodlr::launch_bgemm(g, st));
     }
   }
   if (nrhs > 0) return final_update(p, p->Y, p->N, p->G, nrhs, B, ldb, st);

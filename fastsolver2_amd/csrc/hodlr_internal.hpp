@@ -167,6 +167,9 @@ struct BGemmArgs {
   int64_t ldc2 = 0, sc2 = 0;
 };
 hipError_t launch_bgemm(const BGemmArgs& g, hipStream_t st);
+// Two independent products (no aliasing between them) in one launch when
+// their batch and tile shapes agree, else two launches.
+hipError_t launch_bgemm2(const BGemmArgs& g0, const BGemmArgs& g1, hipStream_t st);
 hipError_t launch_eye(int batch, int r, int64_t sw, double* W, hipStream_t st);
 hipError_t launch_selftest_mfma(double* out, hipStream_t st);
 hipError_t launch_selftest_mfma_f32(double* out, hipStream_t st);

@@ -2734,7 +2734,10 @@ __global__ void k_copy_cols(int64_t rows, int cols, const double* __restrict__ s
 // four waves.  D may alias C; A and B must not.
 // =====================================================================
 template <int GM, int GN>
-__global__ void __launch_bounds__(256) k_bgemm(BGemmArgs g) {
+__global__ void __launch_bounds__(256) k_bgemm(BGemmArgs g0, BGemmArgs g1) {
+  // blockIdx.y selects one of two independent products of the same shape
+  // (one launch for both of a level's per-leaf updates)
+  const BGemmArgs& g = blockIdx.y ? g1 : g0;
   const int i = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
@@ -2844,23 +2847,40 @@ __global__ void __launch_bounds__(256) k_bgemm(BGemmArgs g) {
   }
 }
 
-hipError_t launch_bgemm(const BGemmArgs& g, hipStream_t st) {
+// One launch for g0 and (when non-null) g1, which must have g0's batch and
+// tile shape.
+static hipError_t launch_bgemm_pair(const BGemmArgs& g0, const BGemmArgs* g1, hipStream_t st) {
+  const BGemmArgs& g = g0;
   if (g.batch <= 0 || g.M <= 0 || g.Nc <= 0) return hipSuccess;
   const int mt = (g.M + 15) >> 4, nt = (g.Nc + 15) >> 4;
+  const dim3 grid((unsigned)g.batch, g1 ? 2u : 1u);
+  const BGemmArgs& h = g1 ? *g1 : g0;
   // group tiles only while every wave keeps a task
   if (nt == 5 && mt >= 4)  // r + s in (64, 80]: one task per row tile, no idle column tiles
-    hipLaunchKernelGGL((k_bgemm<1, 5>), dim3((unsigned)g.batch), dim3(256), 0, st, g);
+    hipLaunchKernelGGL((k_bgemm<1, 5>), grid, dim3(256), 0, st, g, h);
   else if (mt * nt >= 16 && nt >= 4)
-    hipLaunchKernelGGL((k_bgemm<1, 4>), dim3((unsigned)g.batch), dim3(256), 0, st, g);
+    hipLaunchKernelGGL((k_bgemm<1, 4>), grid, dim3(256), 0, st, g, h);
   else if (mt * nt >= 16 && mt >= 4 && nt == 1)
-    hipLaunchKernelGGL((k_bgemm<4, 1>), dim3((unsigned)g.batch), dim3(256), 0, st, g);
+    hipLaunchKernelGGL((k_bgemm<4, 1>), grid, dim3(256), 0, st, g, h);
   else if (mt * nt >= 8 && nt >= 2)
-    hipLaunchKernelGGL((k_bgemm<1, 2>), dim3((unsigned)g.batch), dim3(256), 0, st, g);
+    hipLaunchKernelGGL((k_bgemm<1, 2>), grid, dim3(256), 0, st, g, h);
   else if (mt * nt >= 8)
-    hipLaunchKernelGGL((k_bgemm<2, 1>), dim3((unsigned)g.batch), dim3(256), 0, st, g);
+    hipLaunchKernelGGL((k_bgemm<2, 1>), grid, dim3(256), 0, st, g, h);
   else
-    hipLaunchKernelGGL((k_bgemm<1, 1>), dim3((unsigned)g.batch), dim3(256), 0, st, g);
+    hipLaunchKernelGGL((k_bgemm<1, 1>), grid, dim3(256), 0, st, g, h);
   return hipGetLastError();
+}
+
+hipError_t launch_bgemm(const BGemmArgs& g, hipStream_t st) { return launch_bgemm_pair(g, nullptr, st); }
+
+hipError_t launch_bgemm2(const BGemmArgs& g0, const BGemmArgs& g1, hipStream_t st) {
+  const bool same = g0.batch == g1.batch && ((g0.M + 15) >> 4) == ((g1.M + 15) >> 4) &&
+                    ((g0.Nc + 15) >> 4) == ((g1.Nc + 15) >> 4) && g1.M > 0 && g1.Nc > 0;
+  if (!same) {
+    hipError_t e = launch_bgemm_pair(g0, nullptr, st);
+    return e != hipSuccess ? e : launch_bgemm_pair(g1, nullptr, st);
+  }
+  return launch_bgemm_pair(g0, &g1, st);
 }
 
 // W_i = I (r x r, ld r, stride `sw` >= r*r between blocks) for i < batch.
