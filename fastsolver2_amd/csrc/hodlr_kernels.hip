@@ -1445,13 +1445,13 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
           l = p[jj] * (1.0 / pv);
         else
           l = p[jj] / pv;
-        if (live) p[jj] = l;
+        // retired rows (the winner among them) are left out by the exec
+        // mask; readlane still reads the winner's inactive lane
+        if (live) {
+          p[jj] = l;
 #pragma unroll
-        for (int c = 0; c < 64; ++c) {
-          if (c > jj) {
-            const double pc = readlane_f64(p[c], wl);
-            if (live) p[c] -= l * pc;
-          }
+          for (int c = 0; c < 64; ++c)
+            if (c > jj) p[c] -= l * readlane_f64(p[c], wl);
         }
       } else if (lane == 0 && info == 0) {
         info = r + jj + 1;
