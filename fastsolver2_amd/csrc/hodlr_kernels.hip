@@ -107,6 +107,38 @@ __device__ __forceinline__ int wave_argmax_key(unsigned long long k, unsigned k3
   return __ffsll((long long)__ballot(kk == m && q == qq)) - 1;
 }
 
+// The interchanges of a panel on the columns outside it.  dlaswp's
+// sequence of nb swaps is one permutation of the rows from j0 on that moves
+// at most 2*nb <= 64 of them: mv[k] = src | dst << 16 sends the row at
+// position j0 + src to j0 + dst (k < M).  A wave takes whole columns, lane k
+// moving row k: a column's load (all its moving rows) completes before its
+// store, and U columns are in flight at once, so a column costs a fraction
+// of one memory round trip instead of nb dependent ones.
+template <int U, typename T>
+__device__ __forceinline__ void move_rows(T* A, int64_t lda, int n, int j0, int nb, const int* mv,
+                                          int M, int wid, int nw, int lane) {
+  if (M == 0) return;
+  const int m = lane < M ? mv[lane] : 0;
+  const bool act = lane < M;
+  const int64_t src = j0 + (m & 0xFFFF), dst = j0 + (m >> 16);
+  const int ncol = n - nb;
+  for (int c0 = wid; c0 < ncol; c0 += U * nw) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int c = c0 + u * nw;
+      const int64_t col = c < j0 ? c : c + nb;
+      v[u] = (act && c < ncol) ? A[col * lda + src] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int c = c0 + u * nw;
+      const int64_t col = c < j0 ? c : c + nb;
+      if (act && c < ncol) A[col * lda + dst] = v[u];
+    }
+  }
+}
+
 // row of accumulator element v held by `lane`
 __device__ __forceinline__ int crow(int lane, int v) { return (lane >> 4) + 4 * v; }
 
@@ -443,18 +475,22 @@ k_getrf(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
 // so the trailing matrix crosses HBM once per S panels.
 // =====================================================================
 template <int NB, int NT, int S, typename T>
-__global__ void __launch_bounds__(NT)
+__global__ void __launch_bounds__(NT, 2)  // two waves per SIMD: 256 registers
 k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
             int* __restrict__ ipiv0, int* __restrict__ perm0, int* __restrict__ info0) {
   // per-column candidates, T-buffered by column parity: each wave
   // publishes its best row (key, index and the whole panel row) so the
   // winner's row is readable right after the single barrier of the column
   constexpr int NW = NT / 64;
+  static_assert(NW >= 2 && (NW & (NW - 1)) == 0, "a power-of-two count of waves");
   typedef typename Elem<T>::V4 V4;
   __shared__ unsigned long long ckey[2][NW];
   __shared__ int cidx[2][NW];
   __shared__ T crowv[2][NW][NB];
   __shared__ int pivs[NB];
+  __shared__ int mv[2 * NB];  // the panel's moving rows (move_rows)
+  static_assert(2 * NB <= 64, "a moving row per lane");
+  __shared__ int mcount;
   // LDS pool, in two lives.  Panel phase: L11 of each panel of the outer
   // block ([i*(NB+1) + k]), their inverses (same layout) and the doubling
   // scratch.  Trailing update: the negated L panel of a row band (k-major,
@@ -496,6 +532,7 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
     T* L11 = L11s + sp * NB * (NB + 1);
     T* Xi = Xis + sp * NB * (NB + 1);
     const bool own = t < rows;
+    if (t == 0) mcount = 0;  // read after the panel's first barrier
     T p[NB];
     if (own) {
       const T* Arow = A + (int64_t)(j0 + t) + (int64_t)j0 * lda;
@@ -546,14 +583,25 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
         lds_barrier();
         // winner across the waves, lane-parallel: lane w < NW holds wave w's
         // candidate (key, position and pivot value)
-        const unsigned long long kw = (lane < NW) ? ckey[par][lane] : 0ULL;
-        const int qw = (lane < NW) ? cidx[par][lane] : INT_MAX;
-        const T pvw = (lane < NW) ? crowv[par][lane][jj] : T(0);
+        // (every lane reads, so the three reads are in flight together)
+        const int lw = lane & (NW - 1);
+        const unsigned long long kw0 = ckey[par][lw];
+        const int qw0 = cidx[par][lw];
+        const T pvw0 = crowv[par][lw][jj];
+        const unsigned long long kw = (lane < NW) ? kw0 : 0ULL;
+        const int qw = (lane < NW) ? qw0 : INT_MAX;
+        const T pvw = (lane < NW) ? pvw0 : T(0);
         unsigned long long kbest;
         const int bw = wave_argmax_key(kw, Elem<T>::key32(kw), qw, kbest);
         const int bpos = __builtin_amdgcn_readlane(qw, bw);
         const bool winner = (lane == wl) && (wmax != 0ULL);
+        // the pivot row, every read issued before the first use (under the
+        // reciprocal), not one LDS round trip per pair of columns
         const T* prow = crowv[par][bw];
+        T w[NB];
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+          if (c > jj) w[c] = prow[c];
         if (t == 0) pivs[jj] = bpos;
         // the global winner retires to position jj; the row at jj takes the
         // winner's old position
@@ -574,7 +622,7 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
             p[jj] = l;
 #pragma unroll
             for (int c = 0; c < NB; ++c)
-              if (c > jj) p[c] -= l * prow[c];
+              if (c > jj) p[c] -= l * w[c];
           }
         } else if (t == 0 && info == 0) {
           info = j0 + jj + 1;  // the live part of the column is zero
@@ -604,23 +652,13 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
     // at j0 + t ends at j0 + pos, so perm moves the same way (one gather
     // before the barrier, one scatter after it)
     const int pold = own ? perm[j0 + t] : 0;
-    lds_barrier();  // L11, perm; the row stores above are read after a full barrier
+    if (own && pos != t) mv[atomicAdd(&mcount, 1)] = t | (pos << 16);
+    lds_barrier();  // L11, perm, mv; the row stores above are read after a full barrier
     if (own && pos != t) perm[j0 + pos] = pold;
 
-    // interchanges on the other columns (sequential in jj, as dlaswp)
+    // the panel's interchanges on the other columns
     const int m2 = n - j0 - nb;
-    if (t < n - nb) {
-      const int col = t < j0 ? t : t + nb;
-      T* Ac = A + (int64_t)col * lda;
-      for (int jj = 0; jj < nb; ++jj) {
-        const int pp = pivs[jj];
-        if (pp != jj) {
-          const T tmp = Ac[j0 + jj];
-          Ac[j0 + jj] = Ac[j0 + pp];
-          Ac[j0 + pp] = tmp;
-        }
-      }
-    }
+    move_rows<32>(A, lda, n, j0, nb, mv, mcount, wid, NW, lane);
     // Xi = L11^-1 by doubling: after the step of size h the diagonal blocks
     // of size 2h of Xi invert those of L11; the off-diagonal h x h block of
     // a 2h block is -B^-1 (C A^-1), C the matching block of L11.
@@ -1043,6 +1081,9 @@ k_getrf_w64(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
   __shared__ double Xi[NB * (NB + 1)];
   __shared__ double Tt[NB * (NB + 1)];
   __shared__ int perm[128];
+  __shared__ int mv[2 * NB];  // the panel's moving rows (move_rows)
+  static_assert(2 * NB <= 64, "a moving row per lane");
+  __shared__ int mcount;
   const int b = blockIdx.x;
   const int lane = threadIdx.x;
   double* A = A0 + (int64_t)b * bstride;
@@ -1056,6 +1097,7 @@ k_getrf_w64(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
     double p[R][NB];
     int pos[R];
     bool live[R];
+    if (lane == 0) mcount = 0;
 #pragma unroll
     for (int q = 0; q < R; ++q) {
       const int rr = q * 64 + lane;
@@ -1165,20 +1207,14 @@ k_getrf_w64(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
         }
       }
     }
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int rr = q * 64 + lane;
+      if (rr < rows && pos[q] != rr) mv[atomicAdd(&mcount, 1)] = rr | (pos[q] << 16);
+    }
     __syncthreads();
     const int m2 = n - j0 - nb;
-    for (int t = lane; t < n - nb; t += 64) {
-      const int col = t < j0 ? t : t + nb;
-      double* Ac = A + (int64_t)col * lda;
-      for (int jj = 0; jj < nb; ++jj) {
-        const int pp = pivs[jj];
-        if (pp != jj) {
-          const double tmp = Ac[j0 + jj];
-          Ac[j0 + jj] = Ac[j0 + pp];
-          Ac[j0 + pp] = tmp;
-        }
-      }
-    }
+    move_rows<32>(A, lda, n, j0, nb, mv, mcount, 0, 1, lane);
     if (m2 > 0) {
       for (int e = lane; e < NB * NB; e += 64) {
         const int i = e / NB, j = e % NB;
