@@ -796,7 +796,7 @@ int hodlr_solve_ir(hodlr_plan p, const double* K, int64_t ldk, const double* U, 
   }
   // residual block (N x nrhs), matvec workspace, norms of B and R
   const int64_t mvw = hodlr::matvec_work_elems(N, p->m, p->r, nrhs);
-  const int64_t need = N * nrhs + mvw + 2 * (int64_t)nrhs;
+  const int64_t need = N * nrhs + mvw + 2 * (int64_t)nrhs + (int64_t)hodlr::kColmaxParts * nrhs;
   if (p->ir_elems < need) {
     HIP_TRY(hipStreamSynchronize(st));
     if (p->ir_buf) HIP_TRY(hipFree(p->ir_buf));
@@ -812,8 +812,9 @@ int hodlr_solve_ir(hodlr_plan p, const double* K, int64_t ldk, const double* U, 
   double* work = R + N * nrhs;
   double* nb = work + mvw;
   double* nr = nb + nrhs;
+  double* cpart = nr + nrhs;  // partial column maxima
   std::vector<double> hb(nrhs), hr(nrhs);
-  HIP_TRY(hodlr::launch_colmax(N, nrhs, B, ldb, nb, st));
+  HIP_TRY(hodlr::launch_colmax(N, nrhs, B, ldb, nb, cpart, st));
   HIP_TRY(hipMemcpyAsync(hb.data(), nb, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
   int rc = HODLR_OK;
   if (!(flags & HODLR_IR_X_GIVEN)) {  // X = solve(B)
@@ -826,7 +827,7 @@ int hodlr_solve_ir(hodlr_plan p, const double* K, int64_t ldk, const double* U, 
     HIP_TRY(hodlr::launch_copy_cols(N, nrhs, B, ldb, R, N, st));
     HIP_TRY(hodlr::launch_matvec(N, p->m, p->r, nrhs, K, ldk, U, ldu, V, ldv, X, ldx, -1.0, 1.0, R,
                                  N, work, st));
-    HIP_TRY(hodlr::launch_colmax(N, nrhs, R, N, nr, st));
+    HIP_TRY(hodlr::launch_colmax(N, nrhs, R, N, nr, cpart, st));
     HIP_TRY(hipMemcpyAsync(hr.data(), nr, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     double res = 0.0;

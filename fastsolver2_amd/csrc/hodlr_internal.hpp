@@ -55,8 +55,11 @@ hipError_t launch_getrs_f32(int n, const float* LU, int64_t lda, int64_t bstride
                             int ncols, bool trans, hipStream_t st);
 hipError_t launch_convert_f32(int64_t rows, int cols, const double* src, int64_t lds, float* dst,
                               int64_t ldd, hipStream_t st);
+// max |A(:, j)| into out[j]; `part` holds kColmaxParts * cols doubles of
+// partial maxima
+constexpr int kColmaxParts = 128;
 hipError_t launch_colmax(int64_t rows, int cols, const double* A, int64_t lda, double* out,
-                         hipStream_t st);
+                         double* part, hipStream_t st);
 hipError_t launch_add_cols(int64_t rows, int cols, const double* X, int64_t ldx, double* Y,
                            int64_t ldy, hipStream_t st);
 // LU of HODLR cores S = [I, A; B, I] (n2 = 2r): the block structure when

@@ -15,6 +15,7 @@
 // -> gemmBro (spmd_driver/solver_irregular_part.cc:117-140).  The kernels
 // here are the batched, level-synchronous building blocks of that loop.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <utility>
 
 #include <cfloat>
@@ -2059,32 +2060,46 @@ hipError_t launch_convert_f32(int64_t rows, int cols, const double* src, int64_t
 }
 
 // Column max-norms: out[j] = max_i |A(i, j)| (one workgroup per column).
-__global__ void __launch_bounds__(256) k_colmax(int64_t rows, const double* __restrict__ A,
-                                                int64_t lda, double* __restrict__ out) {
-  __shared__ double red[256];
-  const int j = blockIdx.x;
+// max |A(:, j)| per column, a NaN propagating (max is exact, so the order of
+// the comparisons does not matter).  Stage one: kColmaxParts blocks per
+// column each reduce a strided share of the rows into part[j * G + g];
+// stage two: one wave per column reduces the G partials.
+__device__ __forceinline__ double max_nan(double a, double m) { return (a > m || a != a) ? a : m; }
+
+__global__ void __launch_bounds__(256) k_colmax_part(int64_t rows, const double* __restrict__ A,
+                                                     int64_t lda, double* __restrict__ part) {
+  __shared__ double wm[4];
+  const int j = blockIdx.y, g = blockIdx.x, G = gridDim.x;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const double* Aj = A + (int64_t)j * lda;
   double m = 0.0;
-  for (int64_t i = threadIdx.x; i < rows; i += 256) {
-    const double a = fabs(A[i + (int64_t)j * lda]);
-    m = (a > m || a != a) ? a : m;  // a NaN propagates
-  }
-  red[threadIdx.x] = m;
+  for (int64_t i = (int64_t)g * 256 + threadIdx.x; i < rows; i += (int64_t)G * 256)
+    m = max_nan(fabs(Aj[i]), m);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = max_nan(__shfl_xor(m, off, 64), m);
+  if (lane == 0) wm[wid] = m;
   __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
-      const double o = red[threadIdx.x + w];
-      const double c = red[threadIdx.x];
-      red[threadIdx.x] = (o > c || o != o) ? o : c;
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) out[j] = red[0];
+  if (threadIdx.x == 0)
+    part[(int64_t)j * G + g] = max_nan(max_nan(wm[0], wm[1]), max_nan(wm[2], wm[3]));
+}
+
+__global__ void __launch_bounds__(64) k_colmax_final(const double* __restrict__ part, int G,
+                                                     double* __restrict__ out) {
+  const int j = blockIdx.x, lane = threadIdx.x;
+  double m = 0.0;
+  for (int g = lane; g < G; g += 64) m = max_nan(part[(int64_t)j * G + g], m);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = max_nan(__shfl_xor(m, off, 64), m);
+  if (lane == 0) out[j] = m;
 }
 
 hipError_t launch_colmax(int64_t rows, int cols, const double* A, int64_t lda, double* out,
-                         hipStream_t st) {
+                         double* part, hipStream_t st) {
   if (cols <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_colmax, dim3(cols), dim3(256), 0, st, rows, A, lda, out);
+  const int64_t chunks = (rows + 2047) / 2048;  // at least 2048 rows per block
+  const int G = (int)std::max<int64_t>(1, std::min<int64_t>(kColmaxParts, chunks));
+  hipLaunchKernelGGL(k_colmax_part, dim3(G, cols), dim3(256), 0, st, rows, A, lda, part);
+  hipLaunchKernelGGL(k_colmax_final, dim3(cols), dim3(64), 0, st, part, G, out);
   return hipGetLastError();
 }
 
@@ -2441,13 +2456,25 @@ hipError_t launch_sum_slots(const double* part, int nper, int nslot, int ka, int
 // tot = sum_i part_i (fixed order), then per leaf
 //   y_i = alpha * (K_i x_i + U_i (tot - part_i)) + beta * y_i.
 // =====================================================================
-__global__ void k_mv_total(const double* __restrict__ part, int nleaf, int64_t elems,
-                           double* __restrict__ tot) {
-  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (e >= elems) return;
+// 16 elements x 16 leaf chunks per block: thread (c, e) sums leaves c,
+// c + 16, ... of element e, then the 16 chunk sums are added in chunk order
+// (a fixed order: the result does not depend on the launch)
+__global__ void __launch_bounds__(256) k_mv_total(const double* __restrict__ part, int nleaf,
+                                                  int64_t elems, double* __restrict__ tot) {
+  __shared__ double red[16][17];
+  const int el = threadIdx.x & 15, c = threadIdx.x >> 4;
+  const int64_t e = (int64_t)blockIdx.x * 16 + el;
   double s = 0.0;
-  for (int i = 0; i < nleaf; ++i) s += part[(int64_t)i * elems + e];
-  tot[e] = s;
+  if (e < elems)
+    for (int i = c; i < nleaf; i += 16) s += part[(int64_t)i * elems + e];
+  red[c][el] = s;
+  __syncthreads();
+  if (c == 0 && e < elems) {
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) t += red[q][el];
+    tot[e] = t;
+  }
 }
 
 template <int NCT, bool TRANS>
@@ -2523,7 +2550,7 @@ hipError_t launch_leaf_projections(int64_t n, int leaf, int rank, int nrhs, cons
   hipError_t e = launch_reduce(V, ldv, rank, xs, nrhs, leaf, (int)nleaf, leaf, part, nullptr, st);
   if (e != hipSuccess) return e;
   const int64_t elems = (int64_t)rank * nrhs;
-  hipLaunchKernelGGL(k_mv_total, dim3((unsigned)((elems + 255) / 256)), dim3(256), 0, st, part,
+  hipLaunchKernelGGL(k_mv_total, dim3((unsigned)((elems + 15) / 16)), dim3(256), 0, st, part,
                      (int)nleaf, elems, tot);
   return hipGetLastError();
 }
@@ -2605,7 +2632,7 @@ hipError_t launch_matvec(int64_t n, int leaf, int rank, int nrhs, const double* 
   hipError_t e = launch_reduce(V, ldv, rank, xs, nrhs, leaf, (int)nleaf, leaf, part, nullptr, st);
   if (e != hipSuccess) return e;
   const int64_t elems = (int64_t)rank * nrhs;
-  hipLaunchKernelGGL(k_mv_total, dim3((unsigned)((elems + 255) / 256)), dim3(256), 0, st, part,
+  hipLaunchKernelGGL(k_mv_total, dim3((unsigned)((elems + 15) / 16)), dim3(256), 0, st, part,
                      (int)nleaf, elems, tot);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const int gpl = (leaf + 63) / 64;
