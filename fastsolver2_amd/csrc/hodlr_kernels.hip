@@ -2477,7 +2477,11 @@ __global__ void __launch_bounds__(256) k_mv_total(const double* __restrict__ par
   }
 }
 
-template <int NCT, bool TRANS>
+// Block: 4 waves over 256 rows of one leaf and one 16-column tile of the
+// right-hand sides; a wave owns 64 rows (four 16-row tiles), so each X
+// fragment feeds four MFMAs, and the K fragments of 4 k-steps (16 loads) are
+// in flight together.
+template <bool TRANS>
 __global__ void __launch_bounds__(256)
 k_mv_apply(int leaf, int r, int nrhs, const double* __restrict__ K, int64_t ldk,
            const double* __restrict__ U, int64_t ldu, const double* __restrict__ X, int64_t ldx,
@@ -2487,54 +2491,58 @@ k_mv_apply(int leaf, int r, int nrhs, const double* __restrict__ K, int64_t ldk,
   const int lr = lane & 15, lk = lane >> 4;
   const int li = blockIdx.x / gpl, g = blockIdx.x % gpl;
   const int64_t l0 = (int64_t)li * leaf;
-  const int rloc = g * 64 + wid * 16;  // row within the leaf
-  const int j0 = blockIdx.y * NCT * 16;
+  const int rloc = g * 256 + wid * 64;  // first row of this wave within the leaf
+  const int col = blockIdx.y * 16 + lr;
+  const bool cok = col < nrhs;
   const double* P = part + (int64_t)li * r * nrhs;
-  d4 acc[NCT];
+  d4 acc[4];
 #pragma unroll
-  for (int ct = 0; ct < NCT; ++ct) acc[ct] = d4zero();
-  for (int k0 = 0; k0 < leaf; k0 += 4) {
-    const int k = k0 + lk;
-    const int arow = rloc + lr;
-    double af = 0.0;
-    if (k < leaf && arow < leaf)
-      af = TRANS ? K[l0 + k + (int64_t)arow * ldk] : K[l0 + arow + (int64_t)k * ldk];
+  for (int a = 0; a < 4; ++a) acc[a] = d4zero();
+  bool rok[4];
 #pragma unroll
-    for (int ct = 0; ct < NCT; ++ct) {
-      if (j0 + ct * 16 < nrhs) {
-        const int col = j0 + ct * 16 + lr;
-        const double bf = (k < leaf && col < nrhs) ? X[l0 + k + (int64_t)col * ldx] : 0.0;
-        acc[ct] = mfma(af, bf, acc[ct]);
+  for (int a = 0; a < 4; ++a) rok[a] = rloc + a * 16 + lr < leaf;
+  for (int k0 = 0; k0 < leaf; k0 += 16) {
+    double af[4][4], bf[4];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const int k = k0 + kk * 4 + lk;
+      const bool kok = k < leaf;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int64_t arow = rloc + a * 16 + lr;
+        af[kk][a] = (kok && rok[a]) ? (TRANS ? K[l0 + k + arow * ldk] : K[l0 + arow + (int64_t)k * ldk])
+                                    : 0.0;
       }
+      bf[kk] = (kok && cok) ? X[l0 + k + (int64_t)col * ldx] : 0.0;
     }
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+      for (int a = 0; a < 4; ++a) acc[a] = mfma(af[kk][a], bf[kk], acc[a]);
   }
   for (int k0 = 0; k0 < r; k0 += 4) {
     const int k = k0 + lk;
-    const int arow = rloc + lr;
-    const double af = (k < r && arow < leaf) ? U[l0 + arow + (int64_t)k * ldu] : 0.0;
+    const bool kok = k < r;
+    const double bf = (kok && cok) ? tot[k + (int64_t)col * r] - P[k + (int64_t)col * r] : 0.0;
 #pragma unroll
-    for (int ct = 0; ct < NCT; ++ct) {
-      if (j0 + ct * 16 < nrhs) {
-        const int col = j0 + ct * 16 + lr;
-        double bf = 0.0;
-        if (k < r && col < nrhs) bf = tot[k + (int64_t)col * r] - P[k + (int64_t)col * r];
-        acc[ct] = mfma(af, bf, acc[ct]);
-      }
+    for (int a = 0; a < 4; ++a) {
+      const int64_t arow = rloc + a * 16 + lr;
+      const double af = (kok && rok[a]) ? U[l0 + arow + (int64_t)k * ldu] : 0.0;
+      acc[a] = mfma(af, bf, acc[a]);
     }
   }
+  if (!cok) return;
 #pragma unroll
-  for (int ct = 0; ct < NCT; ++ct) {
-    const int col = j0 + ct * 16 + lr;
+  for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int row = rloc + crow(lane, v);
-      if (row < leaf && col < nrhs) {
+      const int row = rloc + a * 16 + crow(lane, v);
+      if (row < leaf) {
         double* y = Y + l0 + row + (int64_t)col * ldy;
         const double prev = (beta == 0.0) ? 0.0 : beta * *y;
-        *y = alpha * acc[ct][v] + prev;
+        *y = alpha * acc[a][v] + prev;
       }
     }
-  }
 }
 
 int64_t matvec_work_elems(int64_t n, int leaf, int rank, int nrhs) {
@@ -2635,13 +2643,13 @@ hipError_t launch_matvec(int64_t n, int leaf, int rank, int nrhs, const double* 
   hipLaunchKernelGGL(k_mv_total, dim3((unsigned)((elems + 15) / 16)), dim3(256), 0, st, part,
                      (int)nleaf, elems, tot);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  const int gpl = (leaf + 63) / 64;
-  dim3 grid((unsigned)(nleaf * gpl), (unsigned)((nrhs + 63) / 64));
+  const int gpl = (leaf + 255) / 256;
+  dim3 grid((unsigned)(nleaf * gpl), (unsigned)((nrhs + 15) / 16));
   if (trans)
-    hipLaunchKernelGGL((k_mv_apply<4, true>), grid, dim3(256), 0, st, leaf, rank, nrhs, K, ldk, U,
+    hipLaunchKernelGGL((k_mv_apply<true>), grid, dim3(256), 0, st, leaf, rank, nrhs, K, ldk, U,
                        ldu, X, ldx, part, tot, alpha, beta, Y, ldy, gpl);
   else
-    hipLaunchKernelGGL((k_mv_apply<4, false>), grid, dim3(256), 0, st, leaf, rank, nrhs, K, ldk, U,
+    hipLaunchKernelGGL((k_mv_apply<false>), grid, dim3(256), 0, st, leaf, rank, nrhs, K, ldk, U,
                        ldu, X, ldx, part, tot, alpha, beta, Y, ldy, gpl);
   return hipGetLastError();
 }
