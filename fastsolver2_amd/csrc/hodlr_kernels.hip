@@ -1581,8 +1581,6 @@ k_trtri_diag(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
 #pragma unroll
       for (int i = k + 1; i < kTri; ++i) x[i] -= col[i] * xk;
     }
-#pragma unroll
-    for (int i = 0; i < kTri; ++i) out[i + c * kTri] = x[i];
   } else {
     // U x = e_c, upper non-unit: rows i < k take -U[i][k] x[k], k descending
 #pragma unroll
@@ -1595,8 +1593,22 @@ k_trtri_diag(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
 #pragma unroll
       for (int i = 0; i < k; ++i) x[i] -= col[i] * xk;
     }
+  }
+  // the columns go out through the LDS tile (Linv, then Uinv), so every
+  // store of the wave covers 512 consecutive bytes
 #pragma unroll
-    for (int i = 0; i < kTri; ++i) out[kTri * kTri + i + c * kTri] = x[i];
+  for (int h = 0; h < 2; ++h) {
+    __syncthreads();  // the tile's last readers are done
+    if ((tid >> 5) == h) {
+#pragma unroll
+      for (int i = 0; i < kTri; ++i) Ts[c][i] = x[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kTri * kTri / 64; ++q) {
+      const int idx = tid + 64 * q;
+      out[h * kTri * kTri + idx] = Ts[idx / kTri][idx % kTri];
+    }
   }
 }
 
