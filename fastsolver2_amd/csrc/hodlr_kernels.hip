@@ -1540,7 +1540,11 @@ template <typename T>
 __global__ void __launch_bounds__(64)
 k_trtri_diag(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
              T* __restrict__ dinv0) {
-  __shared__ T Ts[kTri][kTri + 1];  // Ts[col][row]
+  // Both triangles as lower-triangular tiles, [col][row]: Ms[0] = L with
+  // its unit diagonal, Ms[1] = U index-reversed (Ms[1][31-c][31-r] =
+  // U[r][c]), so lanes 0..31 (columns of Linv) and 32..63 (columns of
+  // Uinv, reversed) run the same forward substitution side by side
+  __shared__ T Ms[2][kTri][kTri + 1];
   const int nblk = ntri(n);
   const int b = blockIdx.x / nblk, ib = blockIdx.x % nblk;
   const int i0 = ib * kTri;
@@ -1556,58 +1560,50 @@ k_trtri_diag(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
   }
 #pragma unroll
   for (int q = 0; q < kTri * kTri / 64; ++q) {
-    const int idx = tid + 64 * q;
-    Ts[idx / kTri][idx % kTri] = v[q];
+    const int idx = tid + 64 * q, c = idx / kTri, r = idx % kTri;
+    Ms[0][c][r] = (r > c) ? v[q] : ((r == c) ? T(1) : T(0));
+    Ms[1][kTri - 1 - c][kTri - 1 - r] = (r <= c) ? v[q] : T(0);
   }
   __syncthreads();
   T* out = dinv0 + (int64_t)b * dinv_elems(n) + (int64_t)ib * 2 * kTri * kTri;
-  const int c = tid & 31;
+  const int h = tid >> 5, c = tid & 31;
+  // column c of Linv solves L x = e_c; column c of Uinv, reversed, solves
+  // the reversed U with the unit vector at 31 - c
+  const int c0 = h ? kTri - 1 - c : c;
   // column-oriented substitution: once x[k] is final, every later row takes
-  // its term at once (independent updates, the LDS reads of a factor column
+  // its term at once (independent updates, the LDS reads of a tile column
   // in flight together) -- the dependent chain is 32 steps, not 496
   T x[kTri];
 #pragma unroll
-  for (int i = 0; i < kTri; ++i) x[i] = (i == c) ? T(1) : T(0);
-  if (tid < 32) {
-    // L x = e_c, unit lower: rows i > k take -L[i][k] x[k], k ascending
+  for (int i = 0; i < kTri; ++i) x[i] = (i == c0) ? T(1) : T(0);
+  const T* const tile = &Ms[h][0][0];
 #pragma unroll
-    for (int k = 0; k < kTri; ++k) {
-      // opaque: x[k] is formed here, so the updates below cannot be sunk
-      // into later per-row chains
-      const T xk = opaque((k < c) ? T(0) : x[k]);
-      x[k] = xk;
-      // an opaque column offset per step keeps the step's reads in the step
-      const T* const col = &Ts[0][0] + opaque(k * (kTri + 1));
+  for (int k = 0; k < kTri; ++k) {
+    // an opaque column offset per step keeps the step's reads in the step
+    const T* const col = tile + opaque(k * (kTri + 1));
+    // opaque: x[k] is formed here (the divide by L's unit diagonal is
+    // exact), so the updates below cannot be sunk into later per-row chains
+    const T d = opaque(x[k] / col[k]);
+    const T xk = opaque((k < c0) ? T(0) : d);
+    x[k] = xk;
 #pragma unroll
-      for (int i = k + 1; i < kTri; ++i) x[i] -= col[i] * xk;
-    }
-  } else {
-    // U x = e_c, upper non-unit: rows i < k take -U[i][k] x[k], k descending
-#pragma unroll
-    for (int kk = 0; kk < kTri; ++kk) {
-      const int k = kTri - 1 - kk;
-      const T* const col = &Ts[0][0] + opaque(k * (kTri + 1));
-      const T d = opaque(x[k] / col[k]);  // always formed (no branch around the divide)
-      const T xk = opaque((k > c) ? T(0) : d);
-      x[k] = xk;
-#pragma unroll
-      for (int i = 0; i < k; ++i) x[i] -= col[i] * xk;
-    }
+    for (int i = k + 1; i < kTri; ++i) x[i] -= col[i] * xk;
   }
-  // the columns go out through the LDS tile (Linv, then Uinv), so every
+  // the columns go out through an LDS tile (Linv, then Uinv), so every
   // store of the wave covers 512 consecutive bytes
+  T (*const Ts)[kTri + 1] = Ms[0];
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
+  for (int hh = 0; hh < 2; ++hh) {
     __syncthreads();  // the tile's last readers are done
-    if ((tid >> 5) == h) {
+    if (h == hh) {
 #pragma unroll
-      for (int i = 0; i < kTri; ++i) Ts[c][i] = x[i];
+      for (int i = 0; i < kTri; ++i) Ts[c][i] = hh ? x[kTri - 1 - i] : x[i];
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kTri * kTri / 64; ++q) {
       const int idx = tid + 64 * q;
-      out[h * kTri * kTri + idx] = Ts[idx / kTri][idx % kTri];
+      out[hh * kTri * kTri + idx] = Ts[idx / kTri][idx % kTri];
     }
   }
 }
