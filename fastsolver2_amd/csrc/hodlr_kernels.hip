@@ -484,6 +484,9 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
   // winner's row is readable right after the single barrier of the column
   constexpr int NW = NT / 64;
   static_assert(NW >= 2 && (NW & (NW - 1)) == 0, "a power-of-two count of waves");
+  // outer blocks of two panels are the only size checked against the
+  // oracle (a four-panel FP32 build gave a wrong factor)
+  static_assert(S == 2, "outer blocks of two panels");
   typedef typename Elem<T>::V4 V4;
   __shared__ unsigned long long ckey[2][NW];
   __shared__ int cidx[2][NW];
