@@ -122,6 +122,43 @@ def test_mixed_ir_conditioned_1e4(gpu):
     plan.close()
 
 
+@pytest.mark.parametrize("m,shift", [(256, 0.0), (512, 1.0)])
+def test_mixed_pivoting_leaves(gpu, m, shift):
+    """Gaussian leaves with no or a small diagonal shift (K_i = G_i + shift I,
+    shift below the columns' largest entries: the FP32 leaf LU pivots, its
+    row interchanges included): the mixed first pass agrees with the FP64
+    plan to FP32 accuracy and refinement reaches 1e-12."""
+    from fastsolver2_amd.workloads import condition_estimate
+    say(f"case: mixed pivoting leaves m={m} shift={shift}")
+    F = gpu
+    N, r, s = 8 * m, 32, 3
+    K, U, V, B = F.workload_gaussian(N, m, r, s, seed=11, shift=shift, rhs="normal")
+    cond = condition_estimate(K, U, V, m)[2]
+    L = int(round(np.log2(N // m)))
+    p64 = F.HodlrPlan(N, m, L, r, nrhs_max=s)
+    X64 = F.fortran_clone(B)
+    p64.factor_solve(F.fortran_clone(K), U, V, X64)
+    assert p64.status()[0] == 0
+    X64, st64 = p64.solve_ir(K, U, V, B, X=X64, tol=1e-12, maxit=20, x_given=True)
+    assert st64["converged"]
+    plan = F.HodlrPlan(N, m, L, r, nrhs_max=s, precision=F.MIXED_IR)
+    X = F.fortran_clone(B)
+    plan.factor_solve(K, U, V, X)
+    assert plan.status()[0] == 0
+    e1 = relerr(to_host(X), to_host(X64))
+    X, st = plan.solve_ir(K, U, V, B, X=X, tol=1e-12, maxit=20, x_given=True)
+    e = relerr(to_host(X), to_host(X64))
+    say(f"  cond {cond:.3g}, first pass {e1:.2e}, first residual {st['history'][0]:.2e}, "
+        f"{st['iterations']} corrections, final {e:.2e}")
+    # a correct FP32 factor: the first pass error is FP32 rounding times the
+    # condition number (a wrong factor is off by far more on a system this
+    # small)
+    assert 1e-13 < e1 < 30.0 * cond * 2.0 ** -24
+    assert st["converged"] and st["residual"] <= 1e-12 and e < 1e-9
+    plan.close()
+    p64.close()
+
+
 def test_transposed_matvec_and_condition_estimate(gpu, orc):
     """A^T x against the dense assembly, and the cond_2 estimate against
     numpy's SVD of the dense matrix (N = 2048)."""
