@@ -20,6 +20,9 @@ FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          # fully unroll the register-resident panel loops (static register indexing)
          "-mllvm", "-pragma-unroll-threshold=1000000",
+         # MFMA accumulators in VGPRs (gfx950 allows it): the AGPR form made the
+         # compiler copy the accumulators across every k-chunk in the small GEMMs
+         "-mllvm", "-amdgpu-mfma-vgpr-form",
          "-Wall", "-Wno-unused-result"]
 
 
