@@ -2335,51 +2335,57 @@ hipError_t launch_assemble(const double* part, int cpc, int r, int ncols_part, i
 // 64-row groups that never straddle a child (one 16-row MFMA tile per
 // wave) x NCT*16 columns; u streamed as the A operand, eta as B.
 // =====================================================================
-template <int NCT>
 __global__ void __launch_bounds__(256)
 k_update(const double* __restrict__ u, int64_t ldu, int r, const double* __restrict__ eta,
          int64_t eta_ld, int64_t eta_ns, int csz, int64_t n_child, int gpc, double alpha,
          ColSeg X, int ncols, int half_xor) {
+  // block: 4 waves over 256 rows of one child and one 16-column tile; a
+  // wave owns 64 rows (four 16-row tiles), so each eta fragment feeds four
+  // MFMAs, and the u fragments of 4 k-steps (16 loads) are in flight together
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const int child = blockIdx.x / gpc, g = blockIdx.x % gpc;
   const int64_t cend = (int64_t)(child + 1) * n_child;
-  const int64_t row0 = (int64_t)child * n_child + (int64_t)g * 64 + wid * 16;
-  const int j0 = blockIdx.y * NCT * 16;
+  const int64_t rbase = (int64_t)child * n_child + (int64_t)g * 256 + wid * 64;
+  const int col = blockIdx.y * 16 + lr;
+  const bool cok = col < ncols;
   const int slot = child / csz;
   const double* E = eta + (int64_t)(slot >> 1) * eta_ns + (int64_t)((slot & 1) ^ half_xor) * r;
-  d4 acc[NCT];
+  d4 acc[4];
 #pragma unroll
-  for (int ct = 0; ct < NCT; ++ct) {
-    const int col = j0 + ct * 16 + lr;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int64_t row = row0 + crow(lane, v);
-      acc[ct][v] = (row < cend && col < ncols) ? seg_in(X, col)[row] : 0.0;
-    }
-  }
-  for (int k0 = 0; k0 < r; k0 += 4) {
-    const int k = k0 + lk;
-    const int64_t arow = row0 + lr;
-    const double af = (k < r && arow < cend) ? alpha * u[arow + (int64_t)k * ldu] : 0.0;
-#pragma unroll
-    for (int ct = 0; ct < NCT; ++ct) {
-      if (j0 + ct * 16 < ncols) {
-        const int col = j0 + ct * 16 + lr;
-        const double bf = (k < r && col < ncols) ? E[k + (int64_t)col * eta_ld] : 0.0;
-        acc[ct] = mfma(af, bf, acc[ct]);
-      }
-    }
-  }
-#pragma unroll
-  for (int ct = 0; ct < NCT; ++ct) {
-    const int col = j0 + ct * 16 + lr;
+  for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-      const int64_t row = row0 + crow(lane, v);
-      if (row < cend && col < ncols) seg_out(X, col)[row] = acc[ct][v];
+      const int64_t row = rbase + a * 16 + crow(lane, v);
+      acc[a][v] = (row < cend && cok) ? seg_in(X, col)[row] : 0.0;
     }
+  bool rok[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) rok[a] = rbase + a * 16 + lr < cend;
+  for (int k0 = 0; k0 < r; k0 += 16) {
+    double af[4][4], bf[4];
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk) {
+      const int k = k0 + kk * 4 + lk;
+      const bool kok = k < r;
+#pragma unroll
+      for (int a = 0; a < 4; ++a)
+        af[kk][a] = (kok && rok[a]) ? alpha * u[rbase + a * 16 + lr + (int64_t)k * ldu] : 0.0;
+      bf[kk] = (kok && cok) ? E[k + (int64_t)col * eta_ld] : 0.0;
+    }
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+      for (int a = 0; a < 4; ++a) acc[a] = mfma(af[kk][a], bf[kk], acc[a]);
   }
+  if (!cok) return;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t row = rbase + a * 16 + crow(lane, v);
+      if (row < cend) seg_out(X, col)[row] = acc[a][v];
+    }
 }
 
 hipError_t launch_update(const double* u, int64_t ldu, int r, const double* eta, int64_t eta_ld,
@@ -2387,9 +2393,9 @@ hipError_t launch_update(const double* u, int64_t ldu, int r, const double* eta,
                          double alpha, const ColSeg& X, int ncols, hipStream_t st,
                          int half_xor) {
   if (nchild <= 0 || ncols <= 0) return hipSuccess;
-  const int64_t gpc = (n_child + 63) / 64;
-  dim3 grid((unsigned)(nchild * gpc), (unsigned)((ncols + 127) / 128));
-  hipLaunchKernelGGL(k_update<8>, grid, dim3(256), 0, st, u, ldu, r, eta, eta_ld,
+  const int64_t gpc = (n_child + 255) / 256;
+  dim3 grid((unsigned)(nchild * gpc), (unsigned)((ncols + 15) / 16));
+  hipLaunchKernelGGL(k_update, grid, dim3(256), 0, st, u, ldu, r, eta, eta_ld,
                      eta_node_stride, csz, n_child, (int)gpc, alpha, X, ncols, half_xor & 1);
   return hipGetLastError();
 }
