@@ -1957,8 +1957,11 @@ static hipError_t launch_getrs_impl(int n, const double* LU, int64_t lda, int64_
   if (n > kMaxBlock) return hipErrorInvalidValue;
   if (n <= 512 && getrs_reg_ld_ok<double>(lda)) {
     // column tile: 16, 32 or 80 right-hand sides per workgroup, by width
-    // unless a tuner fixed it
-    int nct = ncols <= 16 ? 1 : (ncols <= 32 ? 2 : 5);
+    // (16 for blocks of up to 128 rows, the 2r x 2r cores: their 8 steps
+    // are latency, and more, narrower workgroups shorten them; a block's
+    // factor is re-read per workgroup, which is cheap at that size) unless
+    // a tuner fixed it
+    int nct = (ncols <= 16 || n <= 128) ? 1 : (ncols <= 32 ? 2 : 5);
     if (g_getrs_tile == 1 || g_getrs_tile == 2 || g_getrs_tile == 5) nct = g_getrs_tile;
     if (nct == 1)
       launch_getrs_reg<1, TRANS, double>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
