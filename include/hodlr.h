@@ -94,8 +94,8 @@ int hodlr_factor_solve(hodlr_plan plan, double *K, int64_t ldk,
                        const double *U, int64_t ldu, const double *V, int64_t ldv,
                        double *B, int64_t ldb, int nrhs, hipStream_t stream);
 
-/* Iterative refinement in FP64 (north-star mixed-precision mode; also valid
- * for an FP64 plan).  X (N x nrhs) <- A^-1 B refined:
+/* Iterative refinement in FP64 (the HODLR_MIXED_IR plans' second half; also
+ * valid for an FP64 plan).  X (N x nrhs) <- A^-1 B refined:
  *   X = solve(B); repeat { R = B - A X (FP64 HODLR matvec with the ORIGINAL
  *   K, U, V); stop if max_j |R_j|_inf / |B_j|_inf <= tol; X += solve(R) }
  * with solve() the plan's stored factorization (FP32 leaf LU for a
