@@ -60,7 +60,17 @@ def flops_exec(N, m, r, s):
     return f
 
 
-def cpu_baseline(K, U, V, B, m, nthreads, runs=3):
+def host_threads():
+    """CPUs this process may run on: its affinity set, capped by
+    OMP_NUM_THREADS when that is set (the host's CPU share per GPU)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def cpu_baseline(K, U, V, B, m, nthreads, runs=5):
     """The reference algorithm (oracle restatement of hsolve + the level loop)
     with the best CPU BLAS found, on the same arrays."""
     import numpy as np
@@ -437,14 +447,16 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        nthreads = args.cpu_threads or host_threads()
         log(f"cpu baseline: reference algorithm (oracle), {nthreads} threads, full workload")
         Kh, Uh, Vh, Bh = (np.asfortranarray(t.cpu().numpy()) for t in (K0, U, V, B0))
         t_cpu, blas, tl = cpu_baseline(Kh, Uh, Vh, Bh, m, nthreads)
         cpu = {"value": round(1.0 / t_cpu, 4), "unit": "solves/s", "cores": nthreads,
                "kind": "port",
                "sample": f"full workload N={N} leaf={m} rank={r} nrhs={s}; restated hsolve + "
-                         f"level loop, {2 ** tl} leaf tasks, BLAS {blas}; median of 3 after 1 "
+                         f"level loop, {2 ** tl} leaf tasks on {nthreads} threads (every CPU "
+                         f"of this process's affinity set, capped by OMP_NUM_THREADS; "
+                         f"{os.cpu_count()} CPUs visible), BLAS {blas}; median of 5 after 1 "
                          "warm-up",
                "seconds_per_solve": round(t_cpu, 4),
                "gflops_alg": round(fa / t_cpu / 1e9, 2)}

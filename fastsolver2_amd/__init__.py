@@ -82,14 +82,15 @@ def lib():
         "hodlr_factor": ([vp, vp, i64, vp, i64, vp, i64, vp], ci),
         "hodlr_solve": ([vp, vp, i64, ci, vp], ci),
         "hodlr_factor_solve": ([vp, vp, i64, vp, i64, vp, i64, vp, i64, ci, vp], ci),
-        "hodlr_plan_status": ([vp, vp, C.POINTER(ci)], ci),
+        "hodlr_plan_status": ([vp, vp, C.POINTER(ci), C.POINTER(ci)], ci),
+        "hodlr_workspace_release": ([vp], ci),
         "hodlr_solve_adjoint": ([vp, vp, i64, ci, vp], ci),
         "hodlr_batch_create": ([C.POINTER(vp), ci, vp], ci),
         "hodlr_batch_destroy": ([vp], ci),
         "hodlr_batch_size": ([vp], ci),
         "hodlr_batch_plan": ([vp, ci], vp),
         "hodlr_batch_factor_solve": ([vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp], ci),
-        "hodlr_batch_status": ([vp, vp, C.POINTER(ci), C.POINTER(ci)], ci),
+        "hodlr_batch_status": ([vp, vp, C.POINTER(ci), C.POINTER(ci), C.POINTER(ci)], ci),
         "hodlr_batch_set_tuning": ([vp, ci, ci], ci),
         "hodlr_batch_get_tuning": ([vp, C.POINTER(ci), C.POINTER(ci)], ci),
         "hodlr_batch_tune": ([vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, C.POINTER(dbl)], ci),
@@ -104,9 +105,9 @@ def lib():
         "hodlr_hsolve_ref": ([ci, ci, ci, ci, ci, vp, vp, vp, vp], ci),
         "hodlr_gemm_red": ([i64, ci, ci, dbl, vp, i64, ci, vp, i64, ci, vp, i64, vp], ci),
         "hodlr_gemm_bro": ([i64, ci, ci, dbl, vp, i64, ci, vp, i64, vp, i64, ci, vp], ci),
-        "hodlr_node_solve": ([ci, ci, vp, i64, vp, i64, ci, vp], ci),
+        "hodlr_node_solve": ([ci, ci, vp, i64, vp, i64, ci, vp, vp], ci),
         "hodlr_dgetrf_batched": ([ci, vp, i64, i64, ci, vp, vp, vp], ci),
-        "hodlr_dgesv_batched": ([ci, ci, vp, i64, i64, ci, vp, vp, i64, i64, vp], ci),
+        "hodlr_dgesv_batched": ([ci, ci, vp, i64, i64, ci, vp, vp, vp, i64, i64, vp], ci),
         "hodlr_generate": ([C.c_uint64, C.c_uint32, C.c_uint32, ci, i64, ci, dbl, vp, i64, vp],
                            ci),
         "hodlr_shift_leaf_diagonal": ([i64, ci, dbl, vp, i64, vp], ci),
@@ -280,10 +281,13 @@ class HodlrPlan:
               where="hodlr_solve_adjoint")
         return G
 
-    def status(self):
-        w = C.c_int(-1)
-        st = lib().hodlr_plan_status(self._h, _stream(), C.byref(w))
-        return st, w.value
+    def status(self, level=False):
+        """(status, where) of the last factorization, or (status, where,
+        level) with level=True: the leaf index for a singular leaf, the node
+        index within tree level `level` (root 0) for a singular core."""
+        w, lv = C.c_int(-1), C.c_int(-1)
+        st = lib().hodlr_plan_status(self._h, _stream(), C.byref(w), C.byref(lv))
+        return (st, w.value, lv.value) if level else (st, w.value)
 
 
 class HodlrBatch:
@@ -362,9 +366,10 @@ class HodlrBatch:
         return l.value, t.value
 
     def status(self):
-        sy, w = C.c_int(-1), C.c_int(-1)
-        st = lib().hodlr_batch_status(self._h, _stream(), C.byref(sy), C.byref(w))
-        return st, sy.value, w.value
+        """(status, system, where, level) of the first failing system."""
+        sy, w, lv = C.c_int(-1), C.c_int(-1), C.c_int(-1)
+        st = lib().hodlr_batch_status(self._h, _stream(), C.byref(sy), C.byref(w), C.byref(lv))
+        return st, sy.value, w.value, lv.value
 
 
 # ---------------------------------------------------------------- operators
@@ -402,11 +407,14 @@ def gemm_bro(A, E, D, nchunk, nslot, alpha=-1.0):
 
 
 def node_solve(VTu, VTd, r):
-    """LMatrix::node_solve (lmatrix.cc:534-581); returns INFO."""
+    """LMatrix::node_solve (lmatrix.cc:534-581); returns the per-node LAPACK
+    INFO as a device int32 tensor."""
     _check(VTu, "VTu"); _check(VTd, "VTd")
     nnode = VTu.shape[0] // (2 * r)
-    return lib().hodlr_node_solve(nnode, r, _ptr(VTu), _ld(VTu), _ptr(VTd), _ld(VTd),
-                                  VTd.shape[1], _stream())
+    info = torch.zeros(nnode, dtype=torch.int32, device=VTu.device)
+    _call(lib().hodlr_node_solve, nnode, r, _ptr(VTu), _ld(VTu), _ptr(VTd), _ld(VTd),
+          VTd.shape[1], _ptr(info), _stream(), where="hodlr_node_solve")
+    return info
 
 
 def matvec(K, U, V, X, leaf, alpha=1.0, beta=0.0, Y=None, trans=False):
@@ -494,13 +502,15 @@ def hodlr_solve(K, U, V, B, leaf):
 
 def dgesv_batched(A, B, n, batch):
     """Batched dgesv on column-major blocks stacked by columns:
-    A is n x (n*batch), B is n x (nrhs*batch).  Returns (INFO, ipiv)."""
+    A is n x (n*batch), B is n x (nrhs*batch).  Returns (info, ipiv): the
+    per-item LAPACK INFO (batch int32) and the 1-based pivots (batch*n)."""
     _check(A, "A"); _check(B, "B")
     ipiv = torch.empty(batch * n, dtype=torch.int32, device=A.device)
+    info = torch.zeros(batch, dtype=torch.int32, device=A.device)
     nrhs = B.shape[1] // batch
-    st = lib().hodlr_dgesv_batched(n, nrhs, _ptr(A), _ld(A), _ld(A) * n, batch, _ptr(ipiv),
-                                   _ptr(B), _ld(B), _ld(B) * nrhs, _stream())
-    return st, ipiv
+    _call(lib().hodlr_dgesv_batched, n, nrhs, _ptr(A), _ld(A), _ld(A) * n, batch, _ptr(ipiv),
+          _ptr(info), _ptr(B), _ld(B), _ld(B) * nrhs, _stream(), where="hodlr_dgesv_batched")
+    return info, ipiv
 
 
 def generate(seed, stream_id, system, kind, rows, cols, scale=1.0, out=None, device="cuda",

@@ -122,6 +122,12 @@ std::vector<uint64_t> make_key(hodlr_batch b, const Ops& o) {
   return k;
 }
 
+void unmark(hodlr_batch b) {
+  for (hodlr_plan p : b->plans) hodlr::plan_mark_unfactored(p);
+}
+// a replayed graph holds each plan's complete factor + solve
+void mark(hodlr_plan p) { hodlr::plan_mark_factored(p); }
+
 struct DeviceScope {
   int prev = -1;
   explicit DeviceScope(int dev) {
@@ -208,6 +214,9 @@ int hodlr_batch_factor_solve(hodlr_batch b, double* const* K, const int64_t* ldk
     const int rc = enqueue(b, o, stream);
     hipGraph_t graph = nullptr;
     const hipError_t e = hipStreamEndCapture(stream, &graph);
+    // the capture marked every plan factored, but nothing has run yet: until
+    // the graph launches successfully no plan holds a factorization
+    unmark(b);
     if (rc != HODLR_OK || e != hipSuccess || !graph) {
       if (graph) (void)hipGraphDestroy(graph);
       (void)hipGetLastError();
@@ -222,24 +231,29 @@ int hodlr_batch_factor_solve(hodlr_batch b, double* const* K, const int64_t* ldk
     }
     b->key = std::move(key);
   }
-  return hipGraphLaunch(b->exec, stream) == hipSuccess ? HODLR_OK : HODLR_ERR_HIP;
+  unmark(b);
+  if (hipGraphLaunch(b->exec, stream) != hipSuccess) return HODLR_ERR_HIP;
+  for (hodlr_plan p : b->plans) mark(p);
+  return HODLR_OK;
 }
 
-int hodlr_batch_status(hodlr_batch b, hipStream_t stream, int* sys, int* where) {
+int hodlr_batch_status(hodlr_batch b, hipStream_t stream, int* sys, int* where, int* level) {
   if (!b) return HODLR_ERR_ARG;
   DeviceScope ds(b->device);
   if (hipStreamSynchronize(stream) != hipSuccess) return HODLR_ERR_HIP;
   for (size_t i = 0; i < b->plans.size(); ++i) {
-    int w = -1;
-    const int st = hodlr_plan_status(b->plans[i], b->streams[0], &w);
+    int w = -1, l = -1;
+    const int st = hodlr_plan_status(b->plans[i], b->streams[0], &w, &l);
     if (st != HODLR_OK) {
       if (sys) *sys = (int)i;
       if (where) *where = w;
+      if (level) *level = l;
       return st;
     }
   }
   if (sys) *sys = -1;
   if (where) *where = -1;
+  if (level) *level = -1;
   return HODLR_OK;
 }
 

@@ -31,6 +31,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <vector>
 
@@ -67,39 +68,49 @@ struct DeviceGuard {
 // Row-chunk length of the split-K level reduce.
 int64_t reduce_rc(int64_t n_child) { return n_child < 512 ? n_child : 512; }
 
-// Library workspace for entry points without a plan: grown on demand,
-// one buffer per device, released at process exit.
+// Library workspace for entry points without a plan: one buffer per
+// (device, stream).  Calls on different streams never share scratch; calls
+// on one stream are ordered by the stream itself.  Growth is stream-ordered
+// as well (hipFreeAsync / hipMallocAsync on that stream): no device-wide
+// synchronization and no other stream is touched.  A buffer cannot grow
+// while its stream is being captured (the allocation would become a graph
+// node); the call then fails with HODLR_ERR_STATE and a first call outside
+// capture sizes it.
 struct Workspace {
   std::mutex mu;
-  std::vector<void*> ptr;
-  std::vector<size_t> size;
+  std::map<std::pair<int, uintptr_t>, std::pair<void*, size_t>> buf;
 };
 Workspace g_ws;
 
-int workspace(size_t bytes, void** out) {
+int workspace(size_t bytes, hipStream_t st, void** out) {
   int dev = 0;
   HIP_TRY(hipGetDevice(&dev));
   std::lock_guard<std::mutex> lk(g_ws.mu);
-  if ((int)g_ws.ptr.size() <= dev) {
-    g_ws.ptr.resize(dev + 1, nullptr);
-    g_ws.size.resize(dev + 1, 0);
-  }
-  if (g_ws.size[dev] < bytes) {
-    if (g_ws.ptr[dev]) {
-      HIP_TRY(hipDeviceSynchronize());
-      HIP_TRY(hipFree(g_ws.ptr[dev]));
-      g_ws.ptr[dev] = nullptr;
-      g_ws.size[dev] = 0;
+  auto& slot = g_ws.buf[{dev, (uintptr_t)st}];
+  if (slot.second < bytes) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (st != nullptr) HIP_TRY(hipStreamIsCapturing(st, &cs));
+    if (cs != hipStreamCaptureStatusNone) {
+      snprintf(g_detail, sizeof g_detail,
+               "workspace of %zu bytes needed during stream capture; call once outside capture "
+               "first", bytes);
+      return HODLR_ERR_STATE;
     }
-    size_t want = bytes + bytes / 4 + 4096;
-    if (hipMalloc(&g_ws.ptr[dev], want) != hipSuccess) {
+    if (slot.first) {
+      HIP_TRY(hipFreeAsync(slot.first, st));
+      slot.first = nullptr;
+      slot.second = 0;
+    }
+    const size_t want = bytes + bytes / 4 + 4096;
+    if (hipMallocAsync(&slot.first, want, st) != hipSuccess) {
+      (void)hipGetLastError();
       snprintf(g_detail, sizeof g_detail, "workspace allocation of %zu bytes failed", want);
-      g_ws.ptr[dev] = nullptr;
+      slot.first = nullptr;
       return HODLR_ERR_NOMEM;
     }
-    g_ws.size[dev] = want;
+    slot.second = want;
   }
-  *out = g_ws.ptr[dev];
+  *out = slot.first;
   return HODLR_OK;
 }
 
@@ -250,6 +261,13 @@ struct hodlr_plan_s {
   int64_t bytes = 0;
   std::vector<void*> allocs;
 };
+
+void hodlr::plan_mark_unfactored(hodlr_plan p) {
+  if (p) p->factored = false;
+}
+void hodlr::plan_mark_factored(hodlr_plan p) {
+  if (p) p->factored = true;
+}
 
 namespace {
 
@@ -479,6 +497,18 @@ const char* hodlr_strerror(int status) {
 }
 
 const char* hodlr_last_error_detail(void) { return g_detail; }
+
+int hodlr_workspace_release(hipStream_t st) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_ws.mu);
+  auto it = g_ws.buf.find({dev, (uintptr_t)st});
+  if (it == g_ws.buf.end()) return HODLR_OK;
+  void* ptr = it->second.first;
+  g_ws.buf.erase(it);
+  if (ptr) HIP_TRY(hipFreeAsync(ptr, st));
+  return HODLR_OK;
+}
 
 int hodlr_profile_mask(unsigned mask) {
   std::lock_guard<std::mutex> lk(g_prof.mu);
@@ -872,7 +902,7 @@ int hodlr_grad(int64_t n, int leaf, int rank, int nrhs, const double* U, int64_t
   if (!dU && !dV) return HODLR_OK;
   const int64_t pe = (n / leaf + 1) * (int64_t)rank * nrhs;
   void* w = nullptr;
-  int rc = workspace(sizeof(double) * (size_t)(2 * pe), &w);
+  int rc = workspace(sizeof(double) * (size_t)(2 * pe), st, &w);
   if (rc) return rc;
   double* pw = static_cast<double*>(w);
   double* vx_part = pw;
@@ -894,15 +924,18 @@ int hodlr_grad(int64_t n, int leaf, int rank, int nrhs, const double* U, int64_t
   return HODLR_OK;
 }
 
-int hodlr_plan_status(hodlr_plan p, hipStream_t st, int* where) {
+int hodlr_plan_status(hodlr_plan p, hipStream_t st, int* where, int* level) {
   if (!p) return HODLR_ERR_ARG;
   DeviceGuard g(p->d.device);
   HIP_TRY(hipStreamSynchronize(st));
+  if (where) *where = -1;
+  if (level) *level = -1;
   std::vector<int> li(p->P), ci;
   HIP_TRY(hipMemcpy(li.data(), p->leaf_info, sizeof(int) * p->P, hipMemcpyDeviceToHost));
   for (int64_t i = 0; i < p->P; ++i)
     if (li[i]) {
       if (where) *where = (int)i;
+      if (level) *level = p->L;
       return HODLR_SINGULAR_LEAF;
     }
   for (int t = 0; t < p->L; ++t) {
@@ -912,11 +945,11 @@ int hodlr_plan_status(hodlr_plan p, hipStream_t st, int* where) {
                       hipMemcpyDeviceToHost));
     for (int k = 0; k < nn; ++k)
       if (ci[k]) {
-        if (where) *where = (t << 20) + k;
+        if (where) *where = k;
+        if (level) *level = t;
         return HODLR_SINGULAR_CORE;
       }
   }
-  if (where) *where = -1;
   return HODLR_OK;
 }
 
@@ -929,7 +962,7 @@ static int matvec_impl(bool trans, int64_t n, int leaf, int rank, int nrhs, cons
   if (ldk < n || ldu < n || ldv < n || ldx < n || ldy < n) return HODLR_ERR_ARG;
   if (nrhs == 0) return HODLR_OK;
   void* w = nullptr;
-  int rc = workspace(sizeof(double) * (size_t)hodlr::matvec_work_elems(n, leaf, rank, nrhs), &w);
+  int rc = workspace(sizeof(double) * (size_t)hodlr::matvec_work_elems(n, leaf, rank, nrhs), st, &w);
   if (rc) return rc;
   HIP_TRY(hodlr::launch_matvec(n, leaf, rank, nrhs, K, ldk, U, ldu, V, ldv, X, ldx, alpha, beta, Y,
                                ldy, static_cast<double*>(w), st, trans));
@@ -981,7 +1014,7 @@ int hodlr_hsolve_ref(int nrow, int nrhs, int rank, int nPart, int LD, double* K,
   // core INFO slots for every level (level t at offset 2^t - 1)
   const int64_t nint = (int64_t)nPart * m * 2 + nPart + ce * n2 * 2 + nPart + 64;
   void* w = nullptr;
-  int rc = workspace(sizeof(double) * (size_t)ndbl + sizeof(int) * (size_t)nint, &w);
+  int rc = workspace(sizeof(double) * (size_t)ndbl + sizeof(int) * (size_t)nint, st, &w);
   if (rc) return rc;
   double* dp = static_cast<double*>(w);
   double* leaf_dinv = dp; dp += dl;
@@ -1045,7 +1078,7 @@ int hodlr_gemm_red(int64_t rows, int nchunk, int nslot, double alpha, const doub
   const int64_t rc = blk < 512 ? blk : 512;
   const int64_t cpc = hodlr::reduce_chunks_per_child(blk, rc);
   void* w = nullptr;
-  int rc2 = workspace(sizeof(double) * (size_t)(nchunk * cpc * (int64_t)ka * kb), &w);
+  int rc2 = workspace(sizeof(double) * (size_t)(nchunk * cpc * (int64_t)ka * kb), st, &w);
   if (rc2) return rc2;
   double* part = static_cast<double*>(w);
   ColSeg x = hodlr::seg_single(const_cast<double*>(B), ldb);
@@ -1072,22 +1105,21 @@ int hodlr_gemm_bro(int64_t rows, int nchunk, int nslot, double alpha, const doub
 }
 
 int hodlr_node_solve(int nnode, int r, const double* VTu, int64_t ldu, double* VTd, int64_t ldd,
-                     int nrhs, hipStream_t st) {
-  if (nnode <= 0 || r <= 0 || nrhs < 0 || !VTu || !VTd) return HODLR_ERR_ARG;
+                     int nrhs, int* info, hipStream_t st) {
+  if (nnode <= 0 || r <= 0 || nrhs < 0 || !VTu || !VTd || !info) return HODLR_ERR_ARG;
   if (ldu < (int64_t)2 * r * nnode || ldd < (int64_t)2 * r * nnode) return HODLR_ERR_ARG;
   const int n2 = 2 * r;
   if (n2 > hodlr::kMaxBlock) return HODLR_ERR_UNSUPPORTED;
   const int64_t nS = (int64_t)nnode * n2 * n2, nR = (int64_t)nnode * n2 * std::max(nrhs, 1);
   const int64_t nD = (int64_t)nnode * hodlr::dinv_elems(n2);
   void* w = nullptr;
-  int rc = workspace(sizeof(double) * (size_t)(nS + nR + nD) + sizeof(int) * (size_t)(nnode * (2 * n2 + 1) + 16), &w);
+  int rc = workspace(sizeof(double) * (size_t)(nS + nR + nD) + sizeof(int) * (size_t)(nnode * 2 * n2 + 16), st, &w);
   if (rc) return rc;
   double* S = static_cast<double*>(w);
   double* R = S + nS;
   double* dinv = R + nR;
   int* ipiv = reinterpret_cast<int*>(dinv + nD);
   int* perm = ipiv + (int64_t)nnode * n2;
-  int* info = perm + (int64_t)nnode * n2;
   HIP_TRY(hodlr::launch_node_assemble(nnode, r, VTu, ldu, VTd, ldd, nrhs, S, R, st));
   CoreBufs c{S, R, ipiv, perm, info, dinv};
   rc = core_factor_solve(nnode, r, c, nrhs, st);
@@ -1096,11 +1128,6 @@ int hodlr_node_solve(int nnode, int r, const double* VTu, int64_t ldu, double* V
   for (int k = 0; k < nnode; ++k)
     HIP_TRY(hodlr::launch_copy_cols(n2, nrhs, R + (int64_t)k * n2 * nrhs, n2,
                                     VTd + (int64_t)k * n2, ldd, st));
-  HIP_TRY(hipStreamSynchronize(st));
-  std::vector<int> ci(nnode);
-  HIP_TRY(hipMemcpy(ci.data(), info, sizeof(int) * nnode, hipMemcpyDeviceToHost));
-  for (int k = 0; k < nnode; ++k)
-    if (ci[k]) return ci[k] + (k << 20);
   return HODLR_OK;
 }
 
@@ -1112,35 +1139,31 @@ int hodlr_dgetrf_batched(int n, double* A, int64_t lda, int64_t bstride, int bat
   if (n <= 0 || batch <= 0 || !A || !ipiv || !info || lda < n) return HODLR_ERR_ARG;
   if (n > hodlr::kMaxBlock) return HODLR_ERR_UNSUPPORTED;
   void* w = nullptr;
-  int rc = workspace(sizeof(int) * (size_t)batch * n, &w);
+  int rc = workspace(sizeof(int) * (size_t)batch * n, st, &w);
   if (rc) return rc;
   HIP_TRY(hodlr::launch_getrf(n, A, lda, bstride, batch, ipiv, static_cast<int*>(w), info, st));
   return HODLR_OK;
 }
 
 int hodlr_dgesv_batched(int n, int nrhs, double* A, int64_t lda, int64_t bstride, int batch,
-                        int* ipiv, double* B, int64_t ldb, int64_t bbstride, hipStream_t st) {
-  if (n <= 0 || batch <= 0 || nrhs < 0 || !A || !ipiv || lda < n || (nrhs && (!B || ldb < n)))
+                        int* ipiv, int* info, double* B, int64_t ldb, int64_t bbstride,
+                        hipStream_t st) {
+  if (n <= 0 || batch <= 0 || nrhs < 0 || !A || !ipiv || !info || lda < n ||
+      (nrhs && (!B || ldb < n)))
     return HODLR_ERR_ARG;
   if (n > hodlr::kMaxBlock) return HODLR_ERR_UNSUPPORTED;
   const int64_t nD = (int64_t)batch * hodlr::dinv_elems(n);
   void* w = nullptr;
-  int rc = workspace(sizeof(double) * (size_t)nD + sizeof(int) * (size_t)batch * (n + 1) + 64, &w);
+  int rc = workspace(sizeof(double) * (size_t)nD + sizeof(int) * (size_t)batch * n + 64, st, &w);
   if (rc) return rc;
   double* dinv = static_cast<double*>(w);
   int* perm = reinterpret_cast<int*>(dinv + nD);
-  int* info = perm + (int64_t)batch * n;
   HIP_TRY(hodlr::launch_getrf(n, A, lda, bstride, batch, ipiv, perm, info, st));
   HIP_TRY(hodlr::launch_trtri_diag(n, A, lda, bstride, batch, dinv, st));
   if (nrhs > 0) {
     ColSeg x = hodlr::seg_single(B, ldb);
     HIP_TRY(hodlr::launch_getrs(n, A, lda, bstride, batch, perm, dinv, x, bbstride, nrhs, st));
   }
-  HIP_TRY(hipStreamSynchronize(st));
-  std::vector<int> hi(batch);
-  HIP_TRY(hipMemcpy(hi.data(), info, sizeof(int) * batch, hipMemcpyDeviceToHost));
-  for (int b = 0; b < batch; ++b)
-    if (hi[b]) return hi[b] + (b << 20);
   return HODLR_OK;
 }
 

@@ -229,10 +229,18 @@ void Matrix::solve(Matrix& B, const Context& ctx) const {
   auto dA = upload(data_.data(), data_.size(), ctx);
   auto dB = upload(B.pointer(), (size_t)B.rows() * B.cols(), ctx);
   int* ipiv = nullptr;
-  hcheck(hipMalloc(&ipiv, sizeof(int) * mRows_), "hipMalloc ipiv");
-  int st = hodlr_dgesv_batched(mRows_, B.cols(), dA->p, mRows_, 0, 1, ipiv, dB->p, mRows_, 0, ctx.stream);
+  hcheck(hipMalloc(&ipiv, sizeof(int) * (mRows_ + 1)), "hipMalloc ipiv");
+  int* dinfo = ipiv + mRows_;
+  int st = hodlr_dgesv_batched(mRows_, B.cols(), dA->p, mRows_, 0, 1, ipiv, dinfo, dB->p, mRows_, 0,
+                               ctx.stream);
+  int info = 0;
+  if (st == HODLR_OK) {
+    hcheck(hipStreamSynchronize(ctx.stream), "Matrix::solve sync");
+    hcheck(hipMemcpy(&info, dinfo, sizeof(int), hipMemcpyDeviceToHost), "Matrix::solve info");
+  }
   (void)hipFree(ipiv);
   check(st, "Matrix::solve (dgesv, matrix.cc:340-350)");
+  check(info ? HODLR_SINGULAR_LEAF : HODLR_OK, "Matrix::solve: singular (matrix.cc:343)");
   hcheck(hipMemcpy(B.pointer(), dB->p, sizeof(double) * B.rows() * B.cols(), hipMemcpyDeviceToHost),
          "download");
 }
@@ -393,8 +401,20 @@ void LMatrix::node_solve(LMatrix& b, const Context& ctx) {
   const int rowBlk = rblock * plevel;
   require(rowBlk / 2 == mCols, "LMatrix::node_solve: rowBlk/2 == cols (lmatrix.cc:551)");
   const int nnode = nPart / plevel;
-  int st = hodlr_node_solve(nnode, mCols, pointer(), mRows, b.pointer(), b.mRows, b.cols(), ctx.stream);
+  int* dinfo = nullptr;
+  hcheck(hipMalloc(&dinfo, sizeof(int) * nnode), "hipMalloc info");
+  int st = hodlr_node_solve(nnode, mCols, pointer(), mRows, b.pointer(), b.mRows, b.cols(), dinfo,
+                            ctx.stream);
+  std::vector<int> info(nnode, 0);
+  if (st == HODLR_OK) {
+    hcheck(hipStreamSynchronize(ctx.stream), "LMatrix::node_solve sync");
+    hcheck(hipMemcpy(info.data(), dinfo, sizeof(int) * nnode, hipMemcpyDeviceToHost),
+           "LMatrix::node_solve info");
+  }
+  (void)hipFree(dinfo);
   check(st, "LMatrix::node_solve");
+  for (int k = 0; k < nnode; ++k)
+    check(info[k] ? HODLR_SINGULAR_CORE : HODLR_OK, "LMatrix::node_solve: singular core (node_solve.cc:78)");
 }
 // LMatrix::gemmRed (lmatrix.cc:713-761): only ('t','n', beta = 0) is used
 void LMatrix::gemmRed(char ta, char tb, double alpha, const LMatrix& A, const LMatrix& B, double beta,
@@ -555,7 +575,7 @@ Matrix HMatrix::solve_all(const Matrix& b, const Context& ctx) {
            "copy K");
     check(hodlr_factor(plan, dK->p, N, dU->p, N, dV->p, N, ctx.stream), "HMatrix factor");
     int where = -1;
-    check(hodlr_plan_status(plan, ctx.stream, &where), "HMatrix factor status");
+    check(hodlr_plan_status(plan, ctx.stream, &where, nullptr), "HMatrix factor status");
     factored = true;
   }
   auto dB = upload(b.pointer(), (size_t)N * b.cols(), ctx);

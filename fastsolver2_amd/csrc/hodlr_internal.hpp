@@ -5,6 +5,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+struct hodlr_plan_s;
+
 namespace hodlr {
 
 // Column view of an operand made of two column segments [A | B]: column j
@@ -82,6 +84,10 @@ void set_getrs_tile(int nct);
 int getrs_tile();
 // true while per-launch profiling is on (no stream capture then)
 bool profiling_enabled();
+// A plan's "factored" state, for the batch: a captured factorization holds
+// no factors until its graph has been launched.
+void plan_mark_unfactored(hodlr_plan_s* p);
+void plan_mark_factored(hodlr_plan_s* p);
 hipError_t launch_getrs_t(int n, const double* LU, int64_t lda, int64_t bstride, int batch,
                           const int* perm, const double* dinv, const ColSeg& X,
                           int64_t xbstride, int ncols, hipStream_t st);

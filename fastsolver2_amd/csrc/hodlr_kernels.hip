@@ -210,9 +210,11 @@ template <typename V> __device__ __forceinline__ V opaque(V x) {
   return x;
 }
 // Workgroup barrier that orders LDS only: it waits for this wave's LDS
-// operations, not for its global stores (which __syncthreads also drains).
-// For exchanges through LDS where no wave reads another's global writes
-// before the next full barrier.
+// operations, not for its global stores (which the workgroup fence of
+// __syncthreads also covers).  For exchanges through LDS where no wave reads
+// another's global writes before the next full barrier.  Neither barrier
+// waits for global_load_lds copies into LDS: the issuing wave waits for
+// those itself (s_waitcnt vmcnt) before the barrier that publishes them.
 __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
   __builtin_amdgcn_s_barrier();
@@ -1786,8 +1788,10 @@ k_getrs_reg(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
   const uint32_t kstep = TRANS ? 4 * (uint32_t)sizeof(T) : 4 * ldb;
 
   // Di of global step gs (sweep-major) into Ds[gs & 1]: waves w <
-  // kTri*kTri*sizeof(T)/1024 copy 1 KB each; the copy lands before the next
-  // full barrier (it waits for vmcnt)
+  // kTri*kTri*sizeof(T)/1024 copy 1 KB each.  A barrier does not wait for
+  // these copies (they count on the vector-memory counter, not the LDS
+  // one): the copying waves wait for them explicitly before the barrier
+  // that precedes the first read (step (a1))
   constexpr int DSW = kTri * kTri * (int)sizeof(T) / 1024;
   static_assert(DSW <= 8 && DSW * 1024 == kTri * kTri * (int)sizeof(T), "Di copy split");
   auto prefetch_di = [&](int gs) {
@@ -1821,7 +1825,6 @@ k_getrs_reg(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
 
   for (int sweep = 0; sweep < 2; ++sweep) {
     const bool fwd = (sweep == 0);
-    const int islot = TRANS ? (fwd ? 1 : 0) : (fwd ? 0 : 1);
     for (int step = 0; step < nblk; ++step) {
       const int ib = fwd ? step : nblk - 1 - step;
       const int i0 = ib * kTri;
@@ -1839,6 +1842,8 @@ k_getrs_reg(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
               xa[((rt & 1) * 16 + Elem<T>::crow(lane, v)) * (W + 1) + ct * 16 + lr] = acc[q][ct][v];
         }
       }
+      // this step's Di (copied into LDS during the previous step) has landed
+      if (wid < DSW) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       __syncthreads();
       // (a2) xs = D * xa, 2*NCT tiles spread over the waves
       for (int tt = wid; tt < 2 * NCT; tt += 8) {

@@ -18,6 +18,17 @@
  * synchronizing unless stated.  Nothing here aborts: every entry point
  * returns a status (0 = success, < 0 = bad argument / runtime error,
  * > 0 = numerically singular block, see hodlr_plan_status()).
+ *
+ * Threading and streams.  A plan (and a batch) is stream-ordered and not
+ * re-entrant: its entry points must not run concurrently with each other on
+ * different streams or threads; distinct plans may run concurrently on
+ * distinct streams.  Entry points that take no plan (matvec, gradients,
+ * operators, batched LAPACK, hsolve) use library scratch kept per (device,
+ * stream): calls on different streams never share it, calls on one stream
+ * are ordered by that stream.  Calls on ONE stream from several host threads
+ * must be serialized by the caller.  Scratch grows stream-ordered (no device
+ * synchronization); it cannot grow while its stream is being captured
+ * (HODLR_ERR_STATE: make one call outside capture first).
  */
 #ifndef HODLR_H
 #define HODLR_H
@@ -44,6 +55,11 @@ extern "C" {
 const char *hodlr_strerror(int status);
 /* Detail of the last HODLR_ERR_HIP / HODLR_ERR_NOMEM on this thread. */
 const char *hodlr_last_error_detail(void);
+/* Releases the library scratch of `stream` on the current device
+ * (stream-ordered free; for callers about to destroy the stream).  No
+ * reference counterpart (the reference's tasks leak their scratch,
+ * src/tasks/leaf_solve.cc:120-121). */
+int hodlr_workspace_release(hipStream_t stream);
 
 /* ---------------------------------------------------------------- plans */
 enum hodlr_precision { HODLR_FP64 = 0, HODLR_MIXED_IR = 1 };
@@ -160,8 +176,10 @@ int hodlr_batch_factor_solve(hodlr_batch batch, double *const *K, const int64_t 
                              const double *const *V, const int64_t *ldv,
                              double *const *B, const int64_t *ldb, const int *nrhs,
                              hipStream_t stream);
-/* Synchronizes; first failing system in *sys, its hodlr_plan_status where. */
-int hodlr_batch_status(hodlr_batch batch, hipStream_t stream, int *sys, int *where);
+/* Synchronizes; first failing system in *sys and its hodlr_plan_status
+ * (*where, *level).  Any of sys, where, level may be NULL. */
+int hodlr_batch_status(hodlr_batch batch, hipStream_t stream, int *sys, int *where,
+                       int *level);
 /* Launch shape knobs: lanes (1..16 concurrent streams) and the leaf-solve
  * column tile (0 = by width, else 1, 2 or 5 MFMA tiles per workgroup). */
 int hodlr_batch_set_tuning(hodlr_batch batch, int lanes, int tile);
@@ -176,10 +194,12 @@ int hodlr_batch_tune(hodlr_batch batch, double *const *K, const int64_t *ldk,
                      hipStream_t stream, double *best_ms);
 
 /* Synchronizes `stream` and reports the numerical status of the last factor:
- * HODLR_OK, HODLR_SINGULAR_LEAF (*where = leaf index) or HODLR_SINGULAR_CORE
- * (*where = level * 2^20 + node).  Replaces the reference's
- * `assert(INFO == 0)` (src/tasks/leaf_solve.cc:90,141). */
-int hodlr_plan_status(hodlr_plan plan, hipStream_t stream, int *where);
+ * HODLR_OK, HODLR_SINGULAR_LEAF (*where = leaf index, *level = levels) or
+ * HODLR_SINGULAR_CORE (*where = node index within tree level *level, the
+ * root being level 0).  `where` and `level` may be NULL; both are -1 on
+ * HODLR_OK.  Replaces the reference's `assert(INFO == 0)`
+ * (src/tasks/leaf_solve.cc:90,141). */
+int hodlr_plan_status(hodlr_plan plan, hipStream_t stream, int *where, int *level);
 
 /* FP64 HODLR matrix-vector product with the ORIGINAL K (not its LU):
  * Y = alpha * A X + beta * Y, per leaf y_i = K_i x_i + U_i (V^T x - V_i^T x_i).
@@ -228,10 +248,12 @@ int hodlr_gemm_bro(int64_t rows, int nchunk, int nslot, double alpha,
 
 /* LMatrix::node_solve (src/lmatrix.cc:534-581; node_solve.cc:60-81): for
  * each of nnode node slots of 2r rows, S = [I, VTu(r:2r,:); VTu(0:r,:), I],
- * the RHS halves of VTd swapped, VTd <- S^-1 VTd.  Synchronizes `stream` and
- * returns INFO. */
+ * the RHS halves of VTd swapped, VTd <- S^-1 VTd.  `info` is a device array
+ * of nnode ints receiving each core's LAPACK INFO (0, or the 1-based column
+ * of the first zero pivot) -- the reference asserts INFO == 0
+ * (node_solve.cc:78).  Asynchronous on `stream`. */
 int hodlr_node_solve(int nnode, int r, const double *VTu, int64_t ldu,
-                     double *VTd, int64_t ldd, int nrhs, hipStream_t stream);
+                     double *VTd, int64_t ldd, int nrhs, int *info, hipStream_t stream);
 
 /* ------------------------------------------------------ batched LAPACK */
 /* Batched LU with partial pivoting (dgetrf semantics, dgetrf.f) of `batch`
@@ -241,10 +263,11 @@ int hodlr_node_solve(int nnode, int r, const double *VTu, int64_t ldu,
 int hodlr_dgetrf_batched(int n, double *A, int64_t lda, int64_t bstride, int batch,
                          int *ipiv, int *info, hipStream_t stream);
 /* Batched dgesv_ (include/lapack_blas.hpp:45-50): A_b overwritten by LU,
- * B_b = B + b*bbstride (n x nrhs, ld ldb) by the solution.  Synchronizes and
- * returns the first nonzero INFO (+ batch index * 2^20). */
+ * B_b = B + b*bbstride (n x nrhs, ld ldb) by the solution.  ipiv (batch x n,
+ * 1-based) and info (batch ints, LAPACK INFO per item) are device arrays.
+ * Asynchronous on `stream`, like hodlr_dgetrf_batched. */
 int hodlr_dgesv_batched(int n, int nrhs, double *A, int64_t lda, int64_t bstride,
-                        int batch, int *ipiv, double *B, int64_t ldb,
+                        int batch, int *ipiv, int *info, double *B, int64_t ldb,
                         int64_t bbstride, hipStream_t stream);
 
 /* ------------------------------------------------ workload generators */

@@ -69,7 +69,7 @@ def test_operator_argument_checks(L):
     assert L.hodlr_gemm_red(100, 3, 1, 1.0, p, 100, 2, p, 100, 2, p, 2, nul) == -2
     assert L.hodlr_gemm_red(96, 6, 4, 1.0, p, 96, 2, p, 96, 2, p, 8, nul) == -2
     assert L.hodlr_gemm_bro(96, 6, 4, -1.0, p, 96, 2, p, 8, p, 96, 2, nul) == -2
-    assert L.hodlr_node_solve(0, 3, p, 6, p, 6, 1, nul) == -1
+    assert L.hodlr_node_solve(0, 3, p, 6, p, 6, 1, p, nul) == -1
     assert L.hodlr_matvec(100, 30, 2, 1, p, 100, p, 100, p, 100, p, 100, 1.0, 0.0, p, 100,
                           nul) == -1
     assert L.hodlr_generate(0, 0, 0, 7, 10, 1, 1.0, p, 10, nul) == -1

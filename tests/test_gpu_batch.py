@@ -118,3 +118,50 @@ def test_batch_autotuner(gpu, orc):
         ref = orc.hodlr_solve(Kh, Uh, Vh, Bh, sy["leaf"], nthreads=4)
         assert relerr(to_host(Bs[i]), ref) <= 1e-10, i
     b.close()
+
+
+def test_config3_full_batch(gpu, orc):
+    """The uneven batch at its stated size: 256 systems, N log-uniform in
+    [4096, 65536], leaf in {128, 256, 512}, rank in [16, 100], 1 RHS, seed 0.
+    Every system's relative residual through the device HODLR matvec is at
+    most 1e-12, and 32 systems -- the largest N, the largest rank, the
+    smallest system and 29 more spread over the batch -- match the oracle's
+    solve of the same arrays to 1e-10."""
+    from fastsolver2_amd.workloads import batch_operands, batch_systems
+    F = gpu
+    systems = batch_systems(256, seed=0)
+    say(f"case: config-3 batch of {len(systems)} systems, sum N {sum(s['n_real'] for s in systems)}")
+    Ks, Us, Vs, Bs = batch_operands(systems, seed=0)
+    K0 = [F.fortran_clone(k) for k in Ks]
+    B0 = [F.fortran_clone(x) for x in Bs]
+    b = F.HodlrBatch(systems)
+    side = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(side):
+        b.factor_solve(Ks, Us, Vs, Bs)
+        st = b.status()
+    assert st[0] == 0, st
+    worst = 0.0
+    for i, sy in enumerate(systems):
+        R = F.matvec(K0[i], Us[i], Vs[i], Bs[i], sy["leaf"], alpha=-1.0)
+        R.add_(B0[i])
+        res = (R.abs().max() / B0[i].abs().max()).item()
+        worst = max(worst, res)
+        assert res <= 1e-12, (i, res)
+    say(f"  worst residual {worst:.2e}")
+    pick = {max(range(256), key=lambda i: systems[i]["n_real"]),
+            max(range(256), key=lambda i: systems[i]["rank"]),
+            min(range(256), key=lambda i: systems[i]["n_real"])}
+    for i in range(0, 256, 9):
+        if len(pick) >= 32:
+            break
+        pick.add(i)
+    for i in sorted(pick):
+        sy = systems[i]
+        Kh, Uh, Vh, Bh = (to_host(t) for t in (K0[i], Us[i], Vs[i], B0[i]))
+        ref = orc.hodlr_solve(Kh, Uh, Vh, Bh, sy["leaf"], task_level=min(sy["levels"], 4),
+                              nthreads=16)
+        e = relerr(to_host(Bs[i]), ref)
+        assert e <= 1e-10, (i, sy, e)
+    say(f"  {len(pick)} systems match the oracle")
+    b.close()

@@ -98,7 +98,7 @@ def test_dgesv_batched_pivoting(gpu, orc, n):
     A = to_dev(F, np.hstack(As))
     B = to_dev(F, np.hstack(Bs))
     info, ipiv = F.dgesv_batched(A, B, n, batch)
-    assert info == 0
+    assert not info.any()
     X = to_host(B)
     LU = to_host(A)
     ipiv = ipiv.cpu().numpy().reshape(batch, n)
@@ -130,7 +130,7 @@ def test_dgesv_large_leading_dimension(gpu, n, lda):
     A.copy_(torch.from_numpy(np.asfortranarray(Ah)))
     B = to_dev(F, Bh)
     info, ipiv = F.dgesv_batched(A, B, n, 1)
-    assert info == 0
+    assert not info.any()
     X = to_host(B)
     ref = np.linalg.solve(Ah, Bh)
     assert relerr(X, ref) <= 1e-15 * np.linalg.cond(Ah) * n + 1e-13
@@ -149,7 +149,25 @@ def test_dgesv_batched_singular(gpu):
     A = to_dev(F, np.hstack([A0, A1]))
     B = to_dev(F, rng.standard_normal((n, 2)))
     info, _ = F.dgesv_batched(A, B, n, 2)
-    assert info == 10 + (1 << 20)          # INFO 10 in batch item 1
+    assert info.tolist() == [0, 10]        # INFO 10 in batch item 1
+
+
+def test_dgesv_batched_singular_item_past_2048(gpu):
+    """Per-item INFO for a large batch: a singular item at index 2050 and
+    one at the last index are reported in their own slots."""
+    say("case: dgesv_batched singular items 2050 and 2999 of 3000")
+    F = gpu
+    n, batch = 8, 3000
+    rng = np.random.default_rng(7)
+    As = rng.standard_normal((batch, n, n))
+    As[2050][:, 3] = 0.0
+    As[2999][:, 0] = 0.0
+    A = to_dev(F, np.hstack(list(As)))
+    B = to_dev(F, rng.standard_normal((n, batch)))
+    info, _ = F.dgesv_batched(A, B, n, batch)
+    info = info.cpu().numpy()
+    assert info[2050] == 4 and info[2999] == 1
+    assert np.count_nonzero(info) == 2
 
 
 # ------------------------------------------------------------------ full solve
@@ -187,6 +205,24 @@ def test_factor_solve_matches_oracle(gpu, orc, N, m, r, s):
     assert relerr(to_host(B2), xo2) <= RTOL
     plan.close()
 
+
+
+def test_config1_seed0_matches_oracle(gpu, orc):
+    """The CPU-runnable workload exactly as specified: N=16384, 64 leaves of
+    256, rank 30, 1 RHS U[0,1), seed 0."""
+    say("case: config 1 N=16384 m=256 r=30 s=1 seed 0")
+    F = gpu
+    N, m, r, s, L = 16384, 256, 30, 1, 6
+    K, U, V, B = F.workload_gaussian(N, m, r, s, seed=0)
+    Kh, Uh, Vh, Bh = to_host(K), to_host(U), to_host(V), to_host(B)
+    xo = orc.hodlr_solve(Kh, Uh, Vh, Bh, m, task_level=3, nthreads=8)
+    plan = F.HodlrPlan(N, m, L, r, nrhs_max=s)
+    plan.factor_solve(K, U, V, B)
+    assert plan.status()[0] == 0
+    e = relerr(to_host(B), xo)
+    say(f"  rel err vs oracle {e:.2e}")
+    assert e <= RTOL
+    plan.close()
 
 
 def test_padded_leading_dimension_is_bitwise_identical(gpu):
@@ -337,8 +373,47 @@ def test_gemm_red_bro_node_solve(gpu, orc):
     VTd_o = VTd.copy(order="F")
     assert orc.node_solve(VTu, VTd_o, r) == 0
     VTd_g = to_dev(F, VTd)
-    assert F.node_solve(to_dev(F, VTu), VTd_g, r) == 0
+    assert not F.node_solve(to_dev(F, VTu), VTd_g, r).any()
     assert relerr(to_host(VTd_g), VTd_o) <= 1e-12
+
+
+def test_operators_bitwise_on_integer_inputs(gpu, orc):
+    """gemm_red / gemm_bro on small integers (every product and partial sum
+    exact in FP64, so any summation order gives the same bits) and
+    node_solve on cores whose elimination is exact (S = [I, A; 0, I]:
+    x2 = R2, x1 = R1 - A x2 in integers) equal the oracle bit for bit --
+    the test.cc known-answer relations (:325-343, :353-371, :386-424)."""
+    say("case: integer-exact gemm_red / gemm_bro / node_solve")
+    F = gpu
+    rng = np.random.default_rng(11)
+    rows, ka, kb = 2048, 9, 5
+    A = np.asfortranarray(rng.integers(-4, 5, (rows, ka)).astype(np.float64))
+    Bm = np.asfortranarray(rng.integers(-4, 5, (rows, kb)).astype(np.float64))
+    for nchunk, nslot in [(8, 2), (16, 16), (4, 1)]:
+        Cg = to_host(F.gemm_red(to_dev(F, A), to_dev(F, Bm), nchunk, nslot, alpha=2.0))
+        Co = orc.gemm_red(A, Bm, nchunk, nslot, alpha=2.0)
+        assert np.array_equal(Cg, Co), (nchunk, nslot)
+    k, nd = 6, 7
+    u = np.asfortranarray(rng.integers(-3, 4, (rows, k)).astype(np.float64))
+    for nchunk, nslot in [(8, 2), (4, 4)]:
+        E = np.asfortranarray(rng.integers(-3, 4, (nslot * k, nd)).astype(np.float64))
+        D = np.asfortranarray(rng.integers(-9, 10, (rows, nd)).astype(np.float64))
+        Dg = to_dev(F, D)
+        F.gemm_bro(to_dev(F, u), to_dev(F, E), Dg, nchunk, nslot, alpha=-1.0)
+        Do = orc.gemm_bro(u, E, D.copy(order="F"), nchunk, nslot, alpha=-1.0)
+        assert np.array_equal(to_host(Dg), Do), (nchunk, nslot)
+    r, nnode, nrhs = 6, 8, 4
+    VTu = np.asfortranarray(rng.integers(-3, 4, (nnode * 2 * r, r)).astype(np.float64))
+    for k2 in range(nnode):
+        # S = [I, VTu(r:2r); VTu(0:r), I]: a zero lower-left block makes
+        # S = [I, A; 0, I], factored without pivoting or rounding
+        VTu[k2 * 2 * r:k2 * 2 * r + r, :] = 0.0
+    VTd = np.asfortranarray(rng.integers(-5, 6, (nnode * 2 * r, nrhs)).astype(np.float64))
+    VTd_o = VTd.copy(order="F")
+    assert orc.node_solve(VTu, VTd_o, r) == 0
+    VTd_g = to_dev(F, VTd)
+    assert not F.node_solve(to_dev(F, VTu), VTd_g, r).any()
+    assert np.array_equal(to_host(VTd_g), VTd_o)
 
 
 def test_matvec_matches_oracle(gpu, orc):

@@ -217,3 +217,30 @@ def test_ir_graph_replays_match_eager(gpu):
         torch.cuda.synchronize()
         del junk
     plan.close()
+
+
+def test_config4_full(gpu, orc):
+    """The mixed-precision workload at its stated size: N = 2^20, 2048 leaves
+    of 512, rank 64, 4 RHS iid N(0,1), leaf shift calibrated so that
+    cond_2(A) ~ 1e4 (seed 0).  Refinement reaches 1e-12, and the refined
+    solution matches the oracle's FP64 solve of the same arrays to 1e-10."""
+    from fastsolver2_amd.workloads import conditioned_shift_global
+    F = gpu
+    N, m, r, s, L = 1 << 20, 512, 64, 4, 11
+    shift, c = conditioned_shift_global(N, m, r, 1e4, seed=0)
+    say(f"case: config 4 N={N} m={m} r={r} s={s} shift={shift:.4f} cond~{c:.3g}")
+    assert 5e3 < c < 2e4
+    K, U, V, B = F.workload_gaussian(N, m, r, s, seed=0, shift=shift, rhs="normal")
+    plan = F.HodlrPlan(N, m, L, r, nrhs_max=s, precision=F.MIXED_IR)
+    X = F.fortran_clone(B)
+    plan.factor_solve(K, U, V, X)
+    assert plan.status()[0] == 0
+    X, st = plan.solve_ir(K, U, V, B, X=X, tol=1e-12, maxit=20, x_given=True)
+    say(f"  {st['iterations']} corrections, residual {st['residual']:.2e}")
+    assert st["converged"] and st["residual"] <= 1e-12
+    plan.close()
+    xo = orc.hodlr_solve(to_host(K), to_host(U), to_host(V), to_host(B), m, task_level=4,
+                         nthreads=16)
+    e = relerr(to_host(X), xo)
+    say(f"  rel err vs FP64 oracle {e:.2e}")
+    assert e <= 1e-10
