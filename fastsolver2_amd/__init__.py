@@ -102,6 +102,8 @@ def lib():
                         vp, i64, vp], ci),
         "hodlr_matvec": ([i64, ci, ci, ci, vp, i64, vp, i64, vp, i64, vp, i64, dbl, dbl, vp, i64,
                           vp], ci),
+        "hodlr_matvec_noalloc": ([i64, ci, ci, ci, vp, i64, vp, i64, vp, i64, vp, i64, dbl, dbl, vp,
+                                  i64, vp], ci),
         "hodlr_hsolve_ref": ([ci, ci, ci, ci, ci, vp, vp, vp, vp], ci),
         "hodlr_gemm_red": ([i64, ci, ci, dbl, vp, i64, ci, vp, i64, ci, vp, i64, vp], ci),
         "hodlr_gemm_bro": ([i64, ci, ci, dbl, vp, i64, ci, vp, i64, vp, i64, ci, vp], ci),

@@ -976,6 +976,29 @@ int hodlr_matvec(int64_t n, int leaf, int rank, int nrhs, const double* K, int64
                      ldy, st);
 }
 
+int hodlr_matvec_noalloc(int64_t n, int leaf, int rank, int nrhs, const double* K, int64_t ldk,
+                         const double* U, int64_t ldu, const double* V, int64_t ldv,
+                         const double* X, int64_t ldx, double alpha, double beta, double* Y,
+                         int64_t ldy, hipStream_t st) {
+  if (n <= 0 || leaf <= 0 || rank <= 0 || nrhs < 0 || n % leaf) return HODLR_ERR_ARG;
+  if (!K || !U || !V || !X || !Y) return HODLR_ERR_ARG;
+  if (ldk < n || ldu < n || ldv < n || ldx < n || ldy < n) return HODLR_ERR_ARG;
+  const int64_t need = hodlr::matvec_work_elems(n, leaf, rank, nrhs);
+  if (need > hodlr::kNoAllocScratchElems) {
+    snprintf(g_detail, sizeof g_detail,
+             "hodlr_matvec_noalloc needs (N/leaf + 1) * rank * nrhs = %lld doubles of scratch, "
+             "more than its fixed %lld",
+             (long long)need, (long long)hodlr::kNoAllocScratchElems);
+    return HODLR_ERR_UNSUPPORTED;
+  }
+  if (nrhs == 0) return HODLR_OK;
+  double* work = nullptr;
+  HIP_TRY(hodlr::noalloc_scratch(&work));
+  HIP_TRY(hodlr::launch_matvec(n, leaf, rank, nrhs, K, ldk, U, ldu, V, ldv, X, ldx, alpha, beta,
+                               Y, ldy, work, st, false));
+  return HODLR_OK;
+}
+
 int hodlr_matvec_t(int64_t n, int leaf, int rank, int nrhs, const double* K, int64_t ldk,
                    const double* U, int64_t ldu, const double* V, int64_t ldv, const double* X,
                    int64_t ldx, double alpha, double beta, double* Y, int64_t ldy,

@@ -133,6 +133,10 @@ hipError_t launch_matvec(int64_t n, int leaf, int rank, int nrhs, const double* 
                          double* Y, int64_t ldy, double* work, hipStream_t st,
                          bool trans = false);
 int64_t matvec_work_elems(int64_t n, int leaf, int rank, int nrhs);
+// The allocation-free matvec's fixed device scratch (a file-scope device
+// array): its size in doubles, and the current device's address of it.
+constexpr int64_t kNoAllocScratchElems = int64_t(1) << 20;
+hipError_t noalloc_scratch(double** out);
 // part (nleaf x r x nrhs) = V_i^T X_i per leaf, tot (r x nrhs) = their sum
 // in leaf order.
 hipError_t launch_leaf_projections(int64_t n, int leaf, int rank, int nrhs, const double* V,

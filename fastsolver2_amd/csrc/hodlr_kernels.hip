@@ -16,7 +16,9 @@
 // here are the batched, level-synchronous building blocks of that loop.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <mutex>
 #include <utility>
+#include <vector>
 
 #include <cfloat>
 #include <climits>
@@ -2646,6 +2648,36 @@ hipError_t launch_leaf_outer(int64_t n, int leaf, int nc, int k, double alpha, c
   hipLaunchKernelGGL(k_leaf_outer<4>, grid, dim3(256), 0, st, leaf, nc, k, alpha, A, lda, B, ldb,
                      part, tot, C, ldc, gpl);
   return hipGetLastError();
+}
+
+// Fixed scratch of the allocation-free matvec (hodlr_matvec_noalloc): the
+// per-leaf projections V_i^T x and their total, kNoAllocScratchElems
+// doubles.  Every device the library runs on holds its own copy until the
+// process ends.  In host code `scratch` is only a symbol: the device address
+// of the current device's copy comes from hipGetSymbolAddress, looked up
+// once per device and kept in g_scratch_ptr.
+static __device__ __align__(256) double scratch[kNoAllocScratchElems];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_ptr;
+}  // namespace
+
+hipError_t noalloc_scratch(double** out) {
+  *out = nullptr;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return hipSuccess;
 }
 
 hipError_t launch_matvec(int64_t n, int leaf, int rank, int nrhs, const double* K, int64_t ldk,

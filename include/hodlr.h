@@ -210,6 +210,21 @@ int hodlr_matvec(int64_t n, int leaf, int rank, int nrhs,
                  const double *V, int64_t ldv, const double *X, int64_t ldx,
                  double alpha, double beta, double *Y, int64_t ldy,
                  hipStream_t stream);
+/* hodlr_matvec without any device allocation, for callers that cannot
+ * call hipMalloc (code under graph capture, tight serving loops).  No
+ * reference counterpart.  Same arguments and result as hodlr_matvec; its
+ * scratch (the per-leaf V_i^T x partials, (N/leaf + 1) * rank * nrhs
+ * doubles) is a fixed device array of 2^20 doubles (8 MiB) that every GPU
+ * the library runs on holds until the process ends; larger problems are
+ * rejected with HODLR_ERR_UNSUPPORTED before anything is enqueued.  All
+ * calls on one GPU share that array: calls that overlap in time on one GPU
+ * (different streams, or threads) race and must be serialized by the
+ * caller.  No scratch content needs to start at zero. */
+int hodlr_matvec_noalloc(int64_t n, int leaf, int rank, int nrhs,
+                         const double *K, int64_t ldk, const double *U, int64_t ldu,
+                         const double *V, int64_t ldv, const double *X, int64_t ldx,
+                         double alpha, double beta, double *Y, int64_t ldy,
+                         hipStream_t stream);
 /* The same with A^T: y_i = K_i^T x_i + V_i (U^T x - U_i^T x_i). */
 int hodlr_matvec_t(int64_t n, int leaf, int rank, int nrhs,
                    const double *K, int64_t ldk, const double *U, int64_t ldu,
