@@ -109,6 +109,7 @@ def lib():
         "hodlr_gemm_bro": ([i64, ci, ci, dbl, vp, i64, ci, vp, i64, vp, i64, ci, vp], ci),
         "hodlr_node_solve": ([ci, ci, vp, i64, vp, i64, ci, vp, vp], ci),
         "hodlr_dgetrf_batched": ([ci, vp, i64, i64, ci, vp, vp, vp], ci),
+        "hodlr_dgetrf_batched_oop": ([ci, vp, i64, i64, vp, i64, i64, ci, vp, vp, vp], ci),
         "hodlr_dgesv_batched": ([ci, ci, vp, i64, i64, ci, vp, vp, vp, i64, i64, vp], ci),
         "hodlr_generate": ([C.c_uint64, C.c_uint32, C.c_uint32, ci, i64, ci, dbl, vp, i64, vp],
                            ci),
@@ -513,6 +514,22 @@ def dgesv_batched(A, B, n, batch):
     _call(lib().hodlr_dgesv_batched, n, nrhs, _ptr(A), _ld(A), _ld(A) * n, batch, _ptr(ipiv),
           _ptr(info), _ptr(B), _ld(B), _ld(B) * nrhs, _stream(), where="hodlr_dgesv_batched")
     return info, ipiv
+
+
+def dgetrf_batched_oop(K, n, batch, ldl=None):
+    """Out-of-place batched dgetrf of the n x n blocks stacked by rows in K
+    (K is (batch*n) x n, block b = rows b*n..); K is not modified.  Returns
+    (LU, ipiv, info): LU as a (batch, n, ldl) tensor whose [b, :, :n] is
+    block b's factor in column-major order (transpose to index [row, col]),
+    the 1-based pivots (batch x n) and the per-block INFO."""
+    _check(K, "K")
+    ldl = n + 8 if ldl is None else ldl
+    LU = torch.zeros(batch, n, ldl, dtype=torch.float64, device=K.device)
+    ipiv = torch.zeros(batch, n, dtype=torch.int32, device=K.device)
+    info = torch.zeros(batch, dtype=torch.int32, device=K.device)
+    _call(lib().hodlr_dgetrf_batched_oop, n, _ptr(K), _ld(K), n, _ptr(LU), ldl, n * ldl, batch,
+          _ptr(ipiv), _ptr(info), _stream(), where="hodlr_dgetrf_batched_oop")
+    return LU, ipiv, info
 
 
 def generate(seed, stream_id, system, kind, rows, cols, scale=1.0, out=None, device="cuda",

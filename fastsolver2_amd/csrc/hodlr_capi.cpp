@@ -213,7 +213,13 @@ struct hodlr_plan_s {
   int m = 0, L = 0, r = 0, smax = 0;
   int64_t P = 0;
   int64_t wst = 0;  // per-leaf stride of the r x r blocks W_t,i and Q_i (r*r)
-  // leaf factorization
+  // leaf factorization, out of place: leaf i's LU factors in the
+  // contiguous block LU + i*lbs (ld ldl = leaf + pad), FP64 or (mixed
+  // precision) FP32; the caller's K is only read
+  double* LU = nullptr;
+  float* LU32 = nullptr;
+  int64_t ldl = 0, lbs = 0;
+  int* leaf_retry = nullptr;  // blocks that left the no-interchange path
   int* leaf_ipiv = nullptr;
   int* leaf_perm = nullptr;
   int* leaf_info = nullptr;
@@ -236,11 +242,9 @@ struct hodlr_plan_s {
   double* G = nullptr;
   double* R = nullptr;
   int ncmax = 0;
-  // mixed precision: FP32 copy of the leaf blocks (factored in place) and
-  // its diagonal-block inverses; K itself is left untouched
+  // mixed precision: FP32 leaf factors (LU32) and their diagonal-block
+  // inverses
   bool mixed = false;
-  float* K32 = nullptr;
-  int64_t ldk32 = 0;
   float* leaf_dinv32 = nullptr;
   // refinement buffers (allocated on first use): residual block, matvec
   // workspace, column norms
@@ -307,14 +311,14 @@ void plan_free(hodlr_plan p) {
 int leaf_solve(hodlr_plan p, const ColSeg& x, int ncols, bool trans, hipStream_t st) {
   const int m = p->m;
   if (p->mixed) {
-    HIP_TRY(hodlr::launch_getrs_f32(m, p->K32, p->ldk32, m, (int)p->P, p->leaf_perm,
+    HIP_TRY(hodlr::launch_getrs_f32(m, p->LU32, p->ldl, p->lbs, (int)p->P, p->leaf_perm,
                                     p->leaf_dinv32, x, m, ncols, trans, st));
   } else if (trans) {
-    HIP_TRY(hodlr::launch_getrs_t(m, p->K, p->ldk, m, (int)p->P, p->leaf_perm, p->leaf_dinv, x, m,
-                                  ncols, st));
+    HIP_TRY(hodlr::launch_getrs_t(m, p->LU, p->ldl, p->lbs, (int)p->P, p->leaf_perm, p->leaf_dinv,
+                                  x, m, ncols, st));
   } else {
-    HIP_TRY(hodlr::launch_getrs(m, p->K, p->ldk, m, (int)p->P, p->leaf_perm, p->leaf_dinv, x, m,
-                                ncols, st));
+    HIP_TRY(hodlr::launch_getrs(m, p->LU, p->ldl, p->lbs, (int)p->P, p->leaf_perm, p->leaf_dinv, x,
+                                m, ncols, st));
   }
   return HODLR_OK;
 }
@@ -333,31 +337,22 @@ CoreBufs core_level(hodlr_plan p, int t) {
 
 int check_ld(int64_t ld, int64_t n) { return ld >= n ? HODLR_OK : HODLR_ERR_ARG; }
 
-// Leaf phase of factor (+ solve): LU of every leaf, then
-// [Y_{L-1} | B] = K^-1 [U | B].
-int leaf_phase(hodlr_plan p, double* K, int64_t ldk, const double* U, int64_t ldu, double* B,
-               int64_t ldb, int nrhs, hipStream_t st) {
+// Leaf phase of factor (+ solve): LU of every leaf (out of place, K is
+// only read), then [Y_{L-1} | B] = K^-1 [U | B].
+int leaf_phase(hodlr_plan p, const double* K, int64_t ldk, const double* U, int64_t ldu,
+               double* B, int64_t ldb, int nrhs, hipStream_t st) {
   const int m = p->m;
   const double kb = (p->mixed ? 4.0 : 8.0) * (double)p->N * m;
-  if (p->mixed) {
-    // FP32 copy of the leaf blocks, factored in place; K stays FP64 for the
-    // refinement residual
-    HIP_TRY(hodlr::launch_convert_f32(p->N, m, K, ldk, p->K32, p->ldk32, st));
-    {
-      ProfScope ps(HODLR_PROF_GETRF_LEAF, st, p->P * lu_flops(m), 2.0 * kb);
-      HIP_TRY(hodlr::launch_getrf_f32(m, p->K32, p->ldk32, m, (int)p->P, p->leaf_ipiv,
-                                      p->leaf_perm, p->leaf_info, st));
-    }
-    ProfScope ps(HODLR_PROF_TRTRI, st, 0.0, kb);
-    HIP_TRY(hodlr::launch_trtri_diag_f32(m, p->K32, p->ldk32, m, (int)p->P, p->leaf_dinv32, st));
-  } else {
-    {
-      ProfScope ps(HODLR_PROF_GETRF_LEAF, st, p->P * lu_flops(m), 2.0 * kb);
-      HIP_TRY(hodlr::launch_getrf(m, K, ldk, m, (int)p->P, p->leaf_ipiv, p->leaf_perm,
-                                  p->leaf_info, st));
-    }
-    ProfScope ps(HODLR_PROF_TRTRI, st, 0.0, kb);
-    HIP_TRY(hodlr::launch_trtri_diag(m, K, ldk, m, (int)p->P, p->leaf_dinv, st));
+  {
+    ProfScope ps(HODLR_PROF_GETRF_LEAF, st, p->P * lu_flops(m), 8.0 * (double)p->N * m + kb);
+    if (p->mixed)
+      HIP_TRY(hodlr::launch_getrf_leaf_f32(m, K, ldk, m, p->LU32, p->ldl, p->lbs, (int)p->P,
+                                           p->leaf_dinv32, p->leaf_ipiv, p->leaf_perm,
+                                           p->leaf_info, p->leaf_retry, st));
+    else
+      HIP_TRY(hodlr::launch_getrf_leaf(m, K, ldk, m, p->LU, p->ldl, p->lbs, (int)p->P,
+                                       p->leaf_dinv, p->leaf_ipiv, p->leaf_perm, p->leaf_info,
+                                       p->leaf_retry, st));
   }
   ColSeg x;
   int ncols;
@@ -575,11 +570,17 @@ int hodlr_plan_create(hodlr_plan* plan, const hodlr_desc* desc) {
   rc = rc ? rc : palloc(p, &p->leaf_perm, p->P * p->m);
   rc = rc ? rc : palloc(p, &p->leaf_info, p->P);
   p->mixed = d.precision == HODLR_MIXED_IR;
+  // leaf factors: contiguous blocks, the column stride padded off a power
+  // of two (a leaf's columns would otherwise alias in the HBM channel
+  // interleave)
+  p->ldl = p->m + (p->mixed ? 16 : 8);
+  p->lbs = p->ldl * p->m;
+  rc = rc ? rc : palloc(p, &p->leaf_retry, p->P);
   if (p->mixed) {
-    p->ldk32 = p->N + 32;  // keeps the column stride off a power of two
-    rc = rc ? rc : palloc(p, &p->K32, p->ldk32 * p->m);
+    rc = rc ? rc : palloc(p, &p->LU32, p->P * p->lbs);
     rc = rc ? rc : palloc(p, &p->leaf_dinv32, p->P * hodlr::dinv_elems(p->m));
   } else {
+    rc = rc ? rc : palloc(p, &p->LU, p->P * p->lbs);
     rc = rc ? rc : palloc(p, &p->leaf_dinv, p->P * hodlr::dinv_elems(p->m));
   }
   rc = rc ? rc : palloc(p, &p->Y, p->L > 0 ? p->N * r : 0);
@@ -626,7 +627,7 @@ int hodlr_plan_destroy(hodlr_plan plan) {
 
 int64_t hodlr_plan_device_bytes(hodlr_plan plan) { return plan ? plan->bytes : -1; }
 
-int hodlr_factor(hodlr_plan p, double* K, int64_t ldk, const double* U, int64_t ldu,
+int hodlr_factor(hodlr_plan p, const double* K, int64_t ldk, const double* U, int64_t ldu,
                  const double* V, int64_t ldv, hipStream_t st) {
   if (!p || !K || !U || !V) return HODLR_ERR_ARG;
   if (check_ld(ldk, p->N) || check_ld(ldu, p->N) || check_ld(ldv, p->N)) return HODLR_ERR_ARG;
@@ -640,7 +641,7 @@ int hodlr_factor(hodlr_plan p, double* K, int64_t ldk, const double* U, int64_t 
   return rc;
 }
 
-int hodlr_factor_solve(hodlr_plan p, double* K, int64_t ldk, const double* U, int64_t ldu,
+int hodlr_factor_solve(hodlr_plan p, const double* K, int64_t ldk, const double* U, int64_t ldu,
                        const double* V, int64_t ldv, double* B, int64_t ldb, int nrhs,
                        hipStream_t st) {
   if (!p || !K || !U || !V || (nrhs > 0 && !B) || nrhs < 0) return HODLR_ERR_ARG;
@@ -1165,6 +1166,23 @@ int hodlr_dgetrf_batched(int n, double* A, int64_t lda, int64_t bstride, int bat
   int rc = workspace(sizeof(int) * (size_t)batch * n, st, &w);
   if (rc) return rc;
   HIP_TRY(hodlr::launch_getrf(n, A, lda, bstride, batch, ipiv, static_cast<int*>(w), info, st));
+  return HODLR_OK;
+}
+
+int hodlr_dgetrf_batched_oop(int n, const double* K, int64_t ldk, int64_t kbs, double* LU,
+                             int64_t ldl, int64_t lbs, int batch, int* ipiv, int* info,
+                             hipStream_t st) {
+  if (n <= 0 || batch <= 0 || !K || !LU || !ipiv || !info || ldk < n || ldl < n) return HODLR_ERR_ARG;
+  if (n > hodlr::kMaxBlock) return HODLR_ERR_UNSUPPORTED;
+  const int64_t nD = (int64_t)batch * hodlr::dinv_elems(n);
+  void* w = nullptr;
+  int rc = workspace(sizeof(double) * (size_t)nD + sizeof(int) * (size_t)batch * (n + 1) + 64, st, &w);
+  if (rc) return rc;
+  double* dinv = static_cast<double*>(w);
+  int* perm = reinterpret_cast<int*>(dinv + nD);
+  int* retry = perm + (int64_t)batch * n;
+  HIP_TRY(hodlr::launch_getrf_leaf(n, K, ldk, kbs, LU, ldl, lbs, batch, dinv, ipiv, perm, info,
+                                   retry, st));
   return HODLR_OK;
 }
 

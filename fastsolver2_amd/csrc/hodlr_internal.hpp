@@ -45,6 +45,16 @@ __host__ __device__ inline int64_t dinv_elems(int n) { return (int64_t)ntri(n) *
 // 1-based LAPACK IPIV interchanges, info the first zero pivot (1-based).
 hipError_t launch_getrf(int n, double* A, int64_t lda, int64_t bstride, int batch,
                         int* ipiv, int* perm, int* info, hipStream_t st);
+// Leaf factorization out of place: LU_b (ld ldl, stride lbs) <- LU of K_b
+// (ld ldk, stride kbs) with pivots, INFO and the stored 32x32 diagonal
+// inverses (dinv).  retry holds batch ints of workspace (blocks that left
+// the no-interchange fast path).  K is not modified.
+hipError_t launch_getrf_leaf(int n, const double* K, int64_t ldk, int64_t kbs, double* LU,
+                             int64_t ldl, int64_t lbs, int batch, double* dinv, int* ipiv,
+                             int* perm, int* info, int* retry, hipStream_t st);
+hipError_t launch_getrf_leaf_f32(int n, const double* K, int64_t ldk, int64_t kbs, float* LU,
+                                 int64_t ldl, int64_t lbs, int batch, float* dinv, int* ipiv,
+                                 int* perm, int* info, int* retry, hipStream_t st);
 // Inverts the 32x32 diagonal blocks of L (unit) and U of each LU.
 // FP32 leaf factorization / solve for the mixed-precision path (n <= 512);
 // the solve reads and writes FP64 right-hand sides.

@@ -482,7 +482,12 @@ k_getrf(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
 template <int NB, int NT, int S, typename T>
 __global__ void __launch_bounds__(NT, 2)  // two waves per SIMD: 256 registers
 k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
-            int* __restrict__ ipiv0, int* __restrict__ perm0, int* __restrict__ info0) {
+            int* __restrict__ ipiv0, int* __restrict__ perm0, int* __restrict__ info0,
+            const int* __restrict__ only, const double* __restrict__ src, int64_t lds,
+            int64_t sbstride) {
+  // `only` (optional): factor block b only where only[b] != 0; `src`
+  // (optional): block b is first copied from src + b*sbstride (ld lds),
+  // converted to T, and factored in A
   // per-column candidates, T-buffered by column parity: each wave
   // publishes its best row (key, index and the whole panel row) so the
   // winner's row is readable right after the single barrier of the column
@@ -520,8 +525,15 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
   T* const Bs = pool + KC * (BR + UPAD);
   __shared__ int perm[NT];
   const int b = blockIdx.x;
-  const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+  if (only && !only[b]) return;
+  const int t = threadIdx.x;
   T* A = A0 + (int64_t)b * bstride;
+  if (src) {
+    const double* S0 = src + (int64_t)b * sbstride;
+    for (int c = 0; c < n; ++c)
+      if (t < n) A[(int64_t)t + (int64_t)c * lda] = (T)S0[(int64_t)t + (int64_t)c * lds];
+    __syncthreads();
+  }
   int info = 0;  // thread 0
   if (t < n) perm[t] = t;
 
@@ -1068,6 +1080,541 @@ k_getrf_reg(T* __restrict__ A0, int64_t lda, int64_t bstride, int n,
 }
 
 // =====================================================================
+// Batched leaf LU without row interchanges, verified against the partial
+// pivoting rule, out of place (K -> LU) -- the fast path of the leaf
+// factorization.  n = 128*RT; one workgroup of 8 waves per block.
+//
+// Left-looking over 64-column slabs.  A slab (all n rows) lives in MFMA
+// accumulators as transposed 16x16 tiles -- lane l holds rows l & 15 of
+// columns crow(l, v), so each 16-lane group moves 16 consecutive rows of a
+// column (a 128-byte line) -- wave w owning the 16-row tiles w, w+8, ...
+// (interleaved, so every wave keeps work as the factored rows move down).
+// For each 32-column chunk c left of the slab, in order:
+//     U_c = L_cc^-1 X_c      (X_c: the slab's rows of chunk c; MFMA with
+//                            the stored inverse of the unit-lower L_cc)
+//     X_r -= L_rc U_c        (every row tile r below the chunk; L_rc read
+//                            once per slab from the factors already written)
+// then the slab's two 32-column panels are factored: the 32x32 diagonal
+// block by one wave (dgetf2 order, no interchanges), its L and U inverted
+// (the stored 32x32 inverses the solves use), L21 = A21 U11^-1 and, in the
+// first panel, A22 -= L21 (L11^-1 A12) on MFMA.
+//
+// Partial pivoting (dgetrf2: the first row of largest |a| in the column)
+// keeps the diagonal at step j exactly when every multiplier of that step
+// has |l_ij| <= 1.  Every multiplier is checked against 1 - margin (the
+// margin absorbs rounding differences to the netlib operation order); a
+// block where any check fails -- or a zero or non-finite pivot -- stops and
+// sets retry[b], and the caller refactors it from K with the pivoted
+// kernel.  Accepted blocks are exactly the factorization dgetrf computes
+// with IPIV = identity, up to rounding.
+// =====================================================================
+template <typename T> struct PivotMargin;
+template <> struct PivotMargin<double> {
+  __device__ static double thr() { return 1.0 - 1.0 / 1048576.0; }  // 1 - 2^-20
+};
+template <> struct PivotMargin<float> {
+  __device__ static float thr() { return 1.0f - 1.0f / 4096.0f; }   // 1 - 2^-12
+};
+
+// Reciprocal of a pivot: the hardware estimate and one Newton-Raphson
+// step, r = r0 + r0 (1 - x r0), within an ulp of 1/x (exact for powers of
+// two, inf for x = 0, NaN for NaN) -- a short dependent chain where the
+// IEEE divide is a long one.
+__device__ __forceinline__ double recip_nr(double x) {
+  const double r0 = __builtin_amdgcn_rcp(x);
+  return fma(r0, fma(-x, r0, 1.0), r0);
+}
+__device__ __forceinline__ float recip_nr(float x) {
+  const float r0 = __builtin_amdgcn_rcpf(x);
+  return fmaf(r0, fmaf(-x, r0, 1.0f), r0);
+}
+
+// Orders one wave's LDS accesses at this point: in-order LDS execution
+// makes the values written before it visible to the wave's reads after it;
+// the empty asm with a memory clobber also keeps the compiler from moving
+// memory operations across it (and so from overlapping the phases of the
+// diagonal step, which would need more registers than there are).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_wave_barrier();
+  asm volatile("" ::: "memory");
+}
+
+// Unblocked LU without interchanges of a 16x16 block held by lanes 0..15
+// (lane i: row i in a[]), dgetf2's operation order; every multiplier is
+// checked against thr.  Row j is published through Prow (16-byte LDS
+// stores) once it is final and read back by every lane before use.
+template <typename T>
+__device__ __forceinline__ void ll_lu16(T (&a)[16], int lane, T* Prow, T thr, bool& fail) {
+  const bool own = lane < 16;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    if (lane == j) {
+#pragma unroll
+      for (int k = j; k < 16; ++k) Prow[k] = a[k];
+    }
+    wave_sync();
+    T u[16];
+#pragma unroll
+    for (int k = j; k < 16; ++k) u[k] = Prow[k];
+    const T r = recip_nr(u[j]);
+    if (own && lane > j) {
+      const T l = a[j] * r;
+      fail |= !(fabs(l) <= thr);  // NaN, inf (zero pivot) included
+      a[j] = l;
+#pragma unroll
+      for (int k = j + 1; k < 16; ++k) a[k] -= l * u[k];
+    }
+    wave_sync();
+  }
+}
+
+// Inverses of the unit-lower L and upper U of a 16x16 LU held in the
+// inversion tiles Mt[0] (L with its unit diagonal, [col][row]) and Mt[1]
+// (U index-reversed, also lower triangular): lanes 0..15 solve for the
+// columns of L^-1, lanes 16..31 for those of U^-1 (reversed), by
+// column-oriented substitution; written to Li / Ui at offset (o, o).
+template <typename T>
+__device__ __forceinline__ void ll_inv16(T (*Mt)[16][17], T (*Li)[33], T (*Ui)[33], int o,
+                                         int lane) {
+  const int h = (lane >> 4) & 1, c = lane & 15;
+  const int c0 = h ? 15 - c : c;
+  T x[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) x[q] = (q == c0) ? T(1) : T(0);
+  const T* const tile = &Mt[h][0][0];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const T* const col = tile + opaque(k * 17);
+    // the reciprocal is formed inside the step, from the step's own load,
+    // which keeps the steps in order (a multiply by a reciprocal stored in
+    // advance lets the scheduler interleave them, at many more registers
+    // than the accumulators leave)
+    const T d = opaque(x[k] * recip_nr(col[k]));
+    const T xk = opaque((k < c0) ? T(0) : d);
+    x[k] = xk;
+#pragma unroll
+    for (int q = k + 1; q < 16; ++q) x[q] -= col[q] * xk;
+  }
+  if (lane < 16) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) Li[o + q][o + c] = x[q];
+  } else if (lane < 32) {
+#pragma unroll
+    for (int q = 0; q < 16; ++q) Ui[o + q][o + c] = x[15 - q];
+  }
+}
+
+// The diagonal step of a panel, one wave: factor the 32x32 block Ds in
+// place without interchanges, check every multiplier, invert L11 (unit
+// lower) and U11 into Li / Ui ([row][col]) and the stored inverse `out`
+// ([Linv | Uinv], column-major), and with A12 given form U12 = L11^-1 A12
+// in place.  Blocked 2 x 2 on 16x16 blocks, so the serial chains are 16
+// steps long and the couplings are single-tile MFMA products:
+//   A11 = L1 U1;  L21 = A21 U1^-1;  U12 = L1^-1 A12;  A22 - L21 U12 = L2 U2;
+//   L^-1 = [L1^-1, 0; -L2^-1 (L21 L1^-1), L2^-1],
+//   U^-1 = [U1^-1, -(U1^-1 U12) U2^-1; 0, U2^-1].
+template <typename T>
+__device__ __forceinline__ void ll_diag_step(T (*Ds)[33], T (*As12)[33], T (*Li)[33],
+                                             T (*Ui)[33], T (*Ms)[32][33], T* Prow,
+                                             T* __restrict__ out, bool with_a12, T thr,
+                                             bool& fail, int lane_in) {
+  typedef typename Elem<T>::V4 V4;
+  // opaque: the lane-dependent addresses below are formed here, not
+  // hoisted out of the caller's slab loop into registers held throughout
+  const int lane = opaque(lane_in);
+  const int lr = lane & 15, lk = lane >> 4;
+  // the inversion tiles and a product scratch, in the space of Ms
+  T (*const Mt)[16][17] = reinterpret_cast<T (*)[16][17]>(&Ms[0][0][0]);
+  T (*const Pt)[16][17] = reinterpret_cast<T (*)[16][17]>(&Ms[0][0][0] + 2 * 16 * 17);
+  const int i = lane & 15;
+  auto factor16 = [&](int o) {
+    // LU of the diagonal 16x16 block at (o, o), then its inversion tiles
+    T a[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) a[k] = Ds[o + i][o + k];
+    ll_lu16<T>(a, lane, Prow, thr, fail);
+    if (lane < 16) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        Ds[o + i][o + k] = a[k];
+        Mt[0][k][i] = (i > k) ? a[k] : ((i == k) ? T(1) : T(0));
+        Mt[1][15 - k][15 - i] = (i <= k) ? a[k] : T(0);
+      }
+    }
+    wave_sync();
+    ll_inv16<T>(Mt, Li, Ui, o, lane);
+    wave_sync();
+  };
+  factor16(0);
+  {
+    // L21 = A21 U1^-1 (checked) and U12 = L1^-1 A12
+    V4 d = zero4<T>(), e = zero4<T>();
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      d = mfma(Ds[16 + lr][4 * s + lk], Ui[4 * s + lk][lr], d);
+      e = mfma(Li[lr][4 * s + lk], Ds[4 * s + lk][16 + lr], e);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) fail |= !(fabs(d[v]) <= thr);
+    wave_sync();
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      Ds[16 + Elem<T>::crow(lane, v)][lr] = d[v];
+      Ds[Elem<T>::crow(lane, v)][16 + lr] = e[v];
+    }
+    wave_sync();
+    // A22 -= L21 U12
+    V4 c;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) c[v] = Ds[16 + Elem<T>::crow(lane, v)][16 + lr];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) c = mfma(-Ds[16 + lr][4 * s + lk], Ds[4 * s + lk][16 + lr], c);
+    wave_sync();
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Ds[16 + Elem<T>::crow(lane, v)][16 + lr] = c[v];
+    wave_sync();
+  }
+  factor16(16);
+  {
+    // off-diagonal blocks of the inverses: P0 = L21 L1^-1, P1 = U1^-1 U12,
+    // then Linv21 = -L2^-1 P0 and Uinv12 = -P1 U2^-1
+    V4 p0 = zero4<T>(), p1 = zero4<T>();
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      p0 = mfma(Ds[16 + lr][4 * s + lk], Li[4 * s + lk][lr], p0);
+      p1 = mfma(Ui[lr][4 * s + lk], Ds[4 * s + lk][16 + lr], p1);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      Pt[0][Elem<T>::crow(lane, v)][lr] = p0[v];
+      Pt[1][Elem<T>::crow(lane, v)][lr] = p1[v];
+    }
+    wave_sync();
+    V4 l21 = zero4<T>(), u12 = zero4<T>();
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      l21 = mfma(Li[16 + lr][16 + 4 * s + lk], Pt[0][4 * s + lk][lr], l21);
+      u12 = mfma(Pt[1][lr][4 * s + lk], Ui[16 + 4 * s + lk][16 + lr], u12);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      Li[16 + Elem<T>::crow(lane, v)][lr] = -l21[v];
+      Ui[Elem<T>::crow(lane, v)][16 + lr] = -u12[v];
+      Li[Elem<T>::crow(lane, v)][16 + lr] = T(0);
+      Ui[16 + Elem<T>::crow(lane, v)][lr] = T(0);
+    }
+  }
+  wave_sync();
+  // the stored inverses, column-major, 512 consecutive bytes per store
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int idx = lane + 64 * q;
+    out[idx] = Li[idx % 32][idx / 32];
+    out[1024 + idx] = Ui[idx % 32][idx / 32];
+  }
+  if (with_a12) {
+    // U12 of the slab = L11^-1 A12: D(i = row, j = col), A(i, k) =
+    // Linv(row, k), B(k, j) = A12(k, col); Linv is lower triangular
+    V4 d[2][2];
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct) {
+        d[rt][ct] = zero4<T>();
+#pragma unroll
+        for (int s = 0; s < 4 * (rt + 1); ++s)
+          d[rt][ct] = mfma(Li[16 * rt + lr][4 * s + lk], As12[4 * s + lk][16 * ct + lr], d[rt][ct]);
+      }
+    wave_sync();
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < 2; ++ct)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) As12[16 * rt + Elem<T>::crow(lane, v)][16 * ct + lr] = d[rt][ct][v];
+  }
+}
+
+template <int RT, typename T>
+__global__ void __launch_bounds__(512, 1)
+k_getrf_ll(const double* __restrict__ K0, int64_t ldk, int64_t kbs, T* __restrict__ A0,
+           int64_t lda, int64_t abs_, T* __restrict__ dinv0, int* __restrict__ ipiv0,
+           int* __restrict__ perm0, int* __restrict__ info0, int* __restrict__ retry0) {
+  constexpr int n = 128 * RT;
+  constexpr int NS = n / 64;  // slabs
+  constexpr int DV = Elem<T>::DV;
+  typedef typename Elem<T>::V4 V4;
+  // X_c (the slab's rows of chunk c) and U_c = L_cc^-1 X_c in the chunk
+  // loop; in the panel phase the same space parks wave 0's accumulators
+  // while it runs the diagonal step
+  __shared__ T pool[2 * 32 * 65];
+  static_assert(RT * 16 * 64 <= 2 * 32 * 65, "accumulator parking fits");
+  T (*const Xs)[65] = reinterpret_cast<T (*)[65]>(pool);
+  T (*const Us)[65] = reinterpret_cast<T (*)[65]>(pool + 32 * 65);
+  __shared__ T Ds[32][33];     // diagonal block: A11 in, packed LU out
+  __shared__ T As12[32][33];   // A12 in, U12 out
+  __shared__ T Li[32][33];     // L11^-1 [row][col]
+  __shared__ T Ui[32][33];     // U11^-1 [row][col]
+  __shared__ T Ms[2][32][33];  // inversion tiles
+  __shared__ __align__(16) T Prow[32];  // pivot row broadcast
+  __shared__ int bad;
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x;
+  const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const double* K = K0 + (int64_t)b * kbs;
+  T* A = A0 + (int64_t)b * abs_;
+  T* dinv = dinv0 + (int64_t)b * dinv_elems(n);
+  const T thr = PivotMargin<T>::thr();
+  // byte strides of a column (32-bit offsets: the launcher checks
+  // 64 * ldk * 8 < 2^32 and n * lda * sizeof(T) < 2^32)
+  const uint32_t ldkb = (uint32_t)ldk * 8u;
+  const uint32_t ldab = (uint32_t)lda * (uint32_t)sizeof(T);
+  bool fail = false;
+  if (tid == 0) bad = 0;
+  V4 acc[RT][4];
+
+  for (int J = 0; J < NS; ++J) {
+    const int j0 = 64 * J;
+    // Barriers.  The chunk loop exchanges only LDS data (X_c, U_c), so its
+    // barriers order LDS alone and the slab's loads of K stay in flight
+    // under the first chunks.  Factors a chunk reads from global memory
+    // were stored by earlier slabs; those of the previous slab (chunks
+    // 2J-2, 2J-1, prefetched from the end of chunk 2J-3) become visible at
+    // one full barrier: here for J = 1, at chunk 2J-3 otherwise.
+    if (J == 1) __syncthreads();
+    // the lane indices again, opaque, so the lane parts of the addresses
+    // are formed here instead of being hoisted into registers for the
+    // whole kernel
+    const int lane = opaque((int)threadIdx.x) & 63;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int cr0 = Elem<T>::crow(lane, 0);  // column of element v: cr0 + v*DV
+    {
+      // the slab, all n rows, converted to T: element (q, ct, v) is row
+      // 16 (wid + 8q) + lr, column j0 + 16 ct + cr0 + v DV
+      const __amdgpu_buffer_rsrc_t rk = buf_rsrc(K + (int64_t)j0 * ldk);
+      const uint32_t vo = lr * 8u + (uint32_t)cr0 * ldkb;
+#pragma unroll
+      for (int q = 0; q < RT; ++q)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            acc[q][ct][v] = (T)buf_load<double>(rk, vo, 128u * (wid + 8 * q) + (16 * ct + v * DV) * ldkb);
+    }
+    if (J > 0 && wid < 2) {
+      // X_0 = rows 0..31 (tiles 0, 1: waves 0, 1, q = 0)
+#pragma unroll
+      for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Xs[16 * wid + lr][16 * ct + cr0 + v * DV] = acc[0][ct][v];
+    }
+    // Prefetches, issued where their registers are free so that the loads
+    // run under the next barrier: the stored L_cc^-1 fragments of a chunk's
+    // U step (B(k, j = row) = Linv(16 rl + row, k)), and the chunk's L
+    // fragments (B(k, j = row) = L(row, 32c + k)), four k-steps in flight:
+    // step s2 + 4 is fetched into the registers step s2 has just released.
+    const int rl = wid & 1, uct = wid >> 1;  // this wave's U tile
+    T lcv[8];
+    T lf[RT][8];
+    const uint32_t vl = opaque(lr * (uint32_t)sizeof(T) + lk * ldab);
+    auto fetch_lc = [&](int c) {
+      const T* Lc = dinv + (int64_t)c * 2 * kTri * kTri + 16 * rl + lr;
+#pragma unroll
+      for (int s2 = 0; s2 < 8; ++s2) lcv[s2] = Lc[(4 * s2 + lk) * 32];
+    };
+    auto fetch_l = [&](int c, int s2) {
+      const __amdgpu_buffer_rsrc_t ra = buf_rsrc(A + (int64_t)(32 * c) * lda);
+#pragma unroll
+      for (int q = 0; q < RT; ++q) {
+        const int rt = wid + 8 * q;
+        if (rt >= 2 * c + 2) lf[q][s2] = buf_load<T>(ra, vl, 16u * sizeof(T) * rt + 4 * s2 * ldab);
+      }
+    };
+    if (J > 0) {
+      fetch_lc(0);
+#pragma unroll
+      for (int s2 = 0; s2 < 4; ++s2) fetch_l(0, s2);
+    }
+    lds_barrier();  // Xs
+    for (int c = 0; c < 2 * J; ++c) {
+      {
+        // U_c tile (rl, uct) = sum_k Linv(16 rl + row, k) X(k, 16 uct + col),
+        // transposed: A(i = col, k) = X(k, col), B(k, j = row) = Linv(row, k)
+        V4 u = zero4<T>();
+#pragma unroll
+        for (int s2 = 0; s2 < 8; ++s2)
+          if (s2 < 4 * (rl + 1)) u = mfma(Xs[4 * s2 + lk][16 * uct + lr], lcv[s2], u);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Us[16 * rl + lr][16 * uct + cr0 + v * DV] = u[v];
+      }
+      // Us ready, Xs free; at chunk 2J-3 also the previous slab's factors
+      if (c == 2 * J - 3)
+        __syncthreads();
+      else
+        lds_barrier();
+#pragma unroll
+      for (int q = 0; q < RT; ++q) {
+        const int rt = wid + 8 * q;
+        if ((rt >> 1) == c) {
+          // this tile's rows are final U rows
+#pragma unroll
+          for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[q][ct][v] = Us[16 * (rt & 1) + lr][16 * ct + cr0 + v * DV];
+        }
+      }
+      // X_r -= L_rc U_c: A(i = col, k) = U(k, col), B(k, j = row) = L(row, k);
+      // k-step outermost, so only the step's four U operands are live
+#pragma unroll
+      for (int s2 = 0; s2 < 8; ++s2) {
+        T ua[4];
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) ua[ct] = -Us[4 * s2 + lk][16 * ct + lr];
+#pragma unroll
+        for (int q = 0; q < RT; ++q) {
+          const int rt = wid + 8 * q;
+          if (rt >= 2 * c + 2) {
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) acc[q][ct] = mfma(ua[ct], lf[q][s2], acc[q][ct]);
+          }
+        }
+        if (s2 + 4 < 8) fetch_l(c, s2 + 4);
+      }
+      if (c + 1 < 2 * J) {
+        // the next chunk's first loads, under the barrier and its U step
+        fetch_lc(c + 1);
+#pragma unroll
+        for (int s2 = 0; s2 < 4; ++s2) fetch_l(c + 1, s2);
+      }
+#pragma unroll
+      for (int q = 0; q < RT; ++q) {
+        const int rt = wid + 8 * q;
+        if ((rt >> 1) == c + 1 && c + 1 < 2 * J) {
+          // the next chunk's X
+#pragma unroll
+          for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Xs[16 * (rt & 1) + lr][16 * ct + cr0 + v * DV] = acc[q][ct][v];
+        }
+      }
+      lds_barrier();  // Xs (next chunk) ready, Us free
+    }
+    // the slab's two panels
+#pragma unroll
+    for (int p = 0; p < 2; ++p) {
+      const int ib = 2 * J + p;  // 32-row diagonal block
+      const int cp = 2 * p;      // its first column tile in the slab
+#pragma unroll
+      for (int q = 0; q < RT; ++q) {
+        const int rt = wid + 8 * q;
+        if ((rt >> 1) == ib) {
+#pragma unroll
+          for (int cc = 0; cc < 2; ++cc)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              Ds[16 * (rt & 1) + lr][16 * cc + cr0 + v * DV] = acc[q][cp + cc][v];
+              if (p == 0) As12[16 * (rt & 1) + lr][16 * cc + cr0 + v * DV] = acc[q][2 + cc][v];
+            }
+        }
+      }
+      __syncthreads();
+      if (wid == 0) {
+        // the accumulators wait in LDS, so the diagonal step has the registers
+        T* const park = pool + lane;
+#pragma unroll
+        for (int q = 0; q < RT; ++q)
+#pragma unroll
+          for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) park[((q * 4 + ct) * 4 + v) * 64] = acc[q][ct][v];
+        // compiler barrier: the parking stores stay here (sunk into the
+        // step they would keep the accumulators live)
+        asm volatile("" ::: "memory");
+        ll_diag_step<T>(Ds, As12, Li, Ui, Ms, Prow, dinv + (int64_t)ib * 2 * kTri * kTri, p == 0,
+                        thr, fail, lane);
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int q = 0; q < RT; ++q)
+#pragma unroll
+          for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[q][ct][v] = park[((q * 4 + ct) * 4 + v) * 64];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < RT; ++q) {
+        const int rt = wid + 8 * q;
+        if ((rt >> 1) == ib) {
+#pragma unroll
+          for (int cc = 0; cc < 2; ++cc)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              acc[q][cp + cc][v] = Ds[16 * (rt & 1) + lr][16 * cc + cr0 + v * DV];
+              if (p == 0) acc[q][2 + cc][v] = As12[16 * (rt & 1) + lr][16 * cc + cr0 + v * DV];
+            }
+        } else if (rt >= 2 * ib + 2) {
+          // L21 = A21 U11^-1: D(i = col, j = row) = sum_k Uinv(k, col) A21(row, k);
+          // k-step (cc, v) takes B from accumulator element v (k = 16 cc + crow)
+          V4 l0 = zero4<T>(), l1 = zero4<T>();
+#pragma unroll
+          for (int cc = 0; cc < 2; ++cc)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+              const int k = 16 * cc + cr0 + v * DV;
+              const T bb = acc[q][cp + cc][v];
+              if (cc == 0) l0 = mfma(Ui[k][lr], bb, l0);
+              l1 = mfma(Ui[k][16 + lr], bb, l1);
+            }
+#pragma unroll
+          for (int v = 0; v < 4; ++v) fail |= !(fabs(l0[v]) <= thr) || !(fabs(l1[v]) <= thr);
+          acc[q][cp] = l0;
+          acc[q][cp + 1] = l1;
+          if (p == 0) {
+            // A22 -= L21 U12: D(i = col, j = row) -= sum_k U12(k, col) L21(row, k)
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc)
+#pragma unroll
+              for (int v = 0; v < 4; ++v) {
+                const int k = 16 * cc + cr0 + v * DV;
+                const T bb = acc[q][cc][v];
+#pragma unroll
+                for (int oc = 0; oc < 2; ++oc) acc[q][2 + oc] = mfma(-As12[k][16 * oc + lr], bb, acc[q][2 + oc]);
+              }
+          }
+        }
+      }
+      if (p == 0) __syncthreads();  // Ds, As12 reused by the second panel
+    }
+    if (fail) bad = 1;
+    __syncthreads();
+    if (bad) {
+      if (tid == 0) retry0[b] = 1;
+      return;
+    }
+    {
+      // the slab's factors
+      const __amdgpu_buffer_rsrc_t ra = buf_rsrc(A + (int64_t)j0 * lda);
+      const uint32_t vo = lr * (uint32_t)sizeof(T) + (uint32_t)cr0 * ldab;
+#pragma unroll
+      for (int q = 0; q < RT; ++q)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            buf_store(acc[q][ct][v], ra, vo, 16u * sizeof(T) * (wid + 8 * q) + (16 * ct + v * DV) * ldab);
+    }
+  }
+  for (int t = tid; t < n; t += 512) {
+    ipiv0[(int64_t)b * n + t] = t + 1;
+    perm0[(int64_t)b * n + t] = t;
+  }
+  if (tid == 0) {
+    info0[b] = 0;
+    retry0[b] = 0;
+  }
+}
+
+// =====================================================================
 // Batched LU for n <= 128 (the 2r x 2r cores at rank <= 64): ONE wave per
 // block, each lane holding panel rows lane and lane + 64 in registers.  The
 // pivot search is a DPP reduction inside the wave and the pivot row is
@@ -1079,9 +1626,18 @@ template <int NB>
 __global__ void __launch_bounds__(64)
 k_getrf_w64(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
             int* __restrict__ ipiv0, int* __restrict__ perm0, int* __restrict__ info0,
-            const int* __restrict__ only) {
-  // `only` (optional): factor block b only where only[b] != 0
+            const int* __restrict__ only, const double* __restrict__ src = nullptr,
+            int64_t lds = 0, int64_t sbstride = 0) {
+  // `only` (optional): factor block b only where only[b] != 0; `src`
+  // (optional): block b is first copied from src + b*sbstride (ld lds)
   if (only && !only[blockIdx.x]) return;
+  if (src) {
+    const double* S0 = src + (int64_t)blockIdx.x * sbstride;
+    double* D0 = A0 + (int64_t)blockIdx.x * bstride;
+    for (int c = 0; c < n; ++c)
+      for (int r = threadIdx.x; r < n; r += 64) D0[(int64_t)r + (int64_t)c * lda] = S0[(int64_t)r + (int64_t)c * lds];
+    __syncthreads();
+  }
   constexpr int R = 2;  // rows per lane
   __shared__ double prow[NB];
   __shared__ int pivs[NB];
@@ -1344,12 +1900,12 @@ hipError_t launch_getrf(int n, double* A, int64_t lda, int64_t bstride, int batc
   }
   if (n <= 256) {
     hipLaunchKernelGGL((k_getrf_reg<GETRF_NB, 256, 2, double>), dim3(batch), dim3(256), 0, st, A, lda, bstride,
-                       n, ipiv, perm, info);
+                       n, ipiv, perm, info, (const int*)nullptr, (const double*)nullptr, (int64_t)0, (int64_t)0);
     return hipGetLastError();
   }
   if (n <= 512) {
     hipLaunchKernelGGL((k_getrf_reg<GETRF_NB, 512, 2, double>), dim3(batch), dim3(512), 0, st, A, lda, bstride,
-                       n, ipiv, perm, info);
+                       n, ipiv, perm, info, (const int*)nullptr, (const double*)nullptr, (int64_t)0, (int64_t)0);
     return hipGetLastError();
   }
   const size_t lim = 160 * 1024;
@@ -1546,7 +2102,7 @@ hipError_t launch_getrf_core(int n2, double* S, int64_t lda, int64_t bstride, in
 template <typename T>
 __global__ void __launch_bounds__(64)
 k_trtri_diag(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
-             T* __restrict__ dinv0) {
+             T* __restrict__ dinv0, const int* __restrict__ only = nullptr) {
   // Both triangles as lower-triangular tiles, [col][row]: Ms[0] = L with
   // its unit diagonal, Ms[1] = U index-reversed (Ms[1][31-c][31-r] =
   // U[r][c]), so lanes 0..31 (columns of Linv) and 32..63 (columns of
@@ -1554,6 +2110,7 @@ k_trtri_diag(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
   __shared__ T Ms[2][kTri][kTri + 1];
   const int nblk = ntri(n);
   const int b = blockIdx.x / nblk, ib = blockIdx.x % nblk;
+  if (only && !only[b]) return;  // (optional) blocks b with only[b] != 0
   const int i0 = ib * kTri;
   const int bs = min(kTri, n - i0);
   const T* LU = LU0 + (int64_t)b * bstride;
@@ -2027,10 +2584,12 @@ hipError_t launch_getrf_f32(int n, float* A, int64_t lda, int64_t bstride, int b
   if (n > 512) return hipErrorInvalidValue;
   if (n <= 256)
     hipLaunchKernelGGL((k_getrf_reg<GETRF_NB, 256, 2, float>), dim3(batch), dim3(256), 0, st, A,
-                       lda, bstride, n, ipiv, perm, info);
+                       lda, bstride, n, ipiv, perm, info, (const int*)nullptr, (const double*)nullptr,
+                       (int64_t)0, (int64_t)0);
   else
     hipLaunchKernelGGL((k_getrf_reg<GETRF_NB, 512, 2, float>), dim3(batch), dim3(512), 0, st, A,
-                       lda, bstride, n, ipiv, perm, info);
+                       lda, bstride, n, ipiv, perm, info, (const int*)nullptr, (const double*)nullptr,
+                       (int64_t)0, (int64_t)0);
   return hipGetLastError();
 }
 
@@ -2040,6 +2599,110 @@ hipError_t launch_trtri_diag_f32(int n, const float* LU, int64_t lda, int64_t bs
   hipLaunchKernelGGL(k_trtri_diag<float>, dim3(batch * ntri(n)), dim3(64), 0, st, LU, lda,
                      bstride, n, dinv);
   return hipGetLastError();
+}
+
+
+// Block copy with conversion: D_b (ld ldd, stride dbs) <- S_b (FP64, ld lds,
+// stride sbs), n x n per block; grid (batch, n / 64 column groups).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_copy_blocks(int n, const double* __restrict__ S0, int64_t lds, int64_t sbs, T* __restrict__ D0,
+              int64_t ldd, int64_t dbs) {
+  const double* S = S0 + (int64_t)blockIdx.x * sbs;
+  T* D = D0 + (int64_t)blockIdx.x * dbs;
+  const int c1 = min(n, (int)(blockIdx.y + 1) * 64);
+  for (int c = blockIdx.y * 64; c < c1; ++c)
+    for (int r = threadIdx.x; r < n; r += 256) D[r + (int64_t)c * ldd] = (T)S[r + (int64_t)c * lds];
+}
+
+// Leaf factorization out of place: LU_b (ld ldl, stride lbs, element type
+// T) <- LU of K_b (FP64, ld ldk, stride kbs) for every block b, with the
+// stored 32x32 diagonal inverses in dinv.  Blocks of n = 128, 256, 384 or
+// 512 take k_getrf_ll; the blocks it flags (retry[b] = 1, a multiplier
+// beyond the pivot margin) and blocks of any other n are copied from K and
+// factored by the pivoted kernels, then their diagonal blocks inverted.
+template <typename T>
+static hipError_t fallback_getrf(int n, const double* K, int64_t ldk, int64_t kbs, T* LU,
+                                 int64_t ldl, int64_t lbs, int batch, int* ipiv, int* perm,
+                                 int* info, const int* only, hipStream_t st);
+
+template <>
+hipError_t fallback_getrf<double>(int n, const double* K, int64_t ldk, int64_t kbs, double* LU,
+                                  int64_t ldl, int64_t lbs, int batch, int* ipiv, int* perm,
+                                  int* info, const int* only, hipStream_t st) {
+  if (n <= 128) {
+    hipLaunchKernelGGL(k_getrf_w64<32>, dim3(batch), dim3(64), 0, st, LU, ldl, lbs, n, ipiv, perm,
+                       info, only, K, ldk, kbs);
+  } else if (n <= 256) {
+    hipLaunchKernelGGL((k_getrf_reg<GETRF_NB, 256, 2, double>), dim3(batch), dim3(256), 0, st, LU,
+                       ldl, lbs, n, ipiv, perm, info, only, K, ldk, kbs);
+  } else if (n <= 512) {
+    hipLaunchKernelGGL((k_getrf_reg<GETRF_NB, 512, 2, double>), dim3(batch), dim3(512), 0, st, LU,
+                       ldl, lbs, n, ipiv, perm, info, only, K, ldk, kbs);
+  } else {
+    // larger blocks: every block (never masked), copied then factored in place
+    hipLaunchKernelGGL(k_copy_blocks<double>, dim3(batch, (n + 63) / 64), dim3(256), 0, st, n, K,
+                       ldk, kbs, LU, ldl, lbs);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_getrf(n, LU, ldl, lbs, batch, ipiv, perm, info, st);
+  }
+  return hipGetLastError();
+}
+
+template <>
+hipError_t fallback_getrf<float>(int n, const double* K, int64_t ldk, int64_t kbs, float* LU,
+                                 int64_t ldl, int64_t lbs, int batch, int* ipiv, int* perm,
+                                 int* info, const int* only, hipStream_t st) {
+  if (n > 512) return hipErrorInvalidValue;
+  if (n <= 256)
+    hipLaunchKernelGGL((k_getrf_reg<GETRF_NB, 256, 2, float>), dim3(batch), dim3(256), 0, st, LU,
+                       ldl, lbs, n, ipiv, perm, info, only, K, ldk, kbs);
+  else
+    hipLaunchKernelGGL((k_getrf_reg<GETRF_NB, 512, 2, float>), dim3(batch), dim3(512), 0, st, LU,
+                       ldl, lbs, n, ipiv, perm, info, only, K, ldk, kbs);
+  return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t getrf_leaf_impl(int n, const double* K, int64_t ldk, int64_t kbs, T* LU,
+                                  int64_t ldl, int64_t lbs, int batch, T* dinv, int* ipiv,
+                                  int* perm, int* info, int* retry, hipStream_t st) {
+  if (batch <= 0 || n <= 0) return hipSuccess;
+  // the fast kernel addresses a slab of K and a block of LU with 32-bit
+  // byte offsets
+  const bool fast = (n % 128 == 0) && n <= 512 && ldk * 8 * 64 < ((int64_t)1 << 32) &&
+                    ldl * (int64_t)sizeof(T) * n < ((int64_t)1 << 32);
+  if (fast) {
+    const dim3 g(batch), blk(512);
+    switch (n / 128) {
+      case 1: hipLaunchKernelGGL((k_getrf_ll<1, T>), g, blk, 0, st, K, ldk, kbs, LU, ldl, lbs, dinv, ipiv, perm, info, retry); break;
+      case 2: hipLaunchKernelGGL((k_getrf_ll<2, T>), g, blk, 0, st, K, ldk, kbs, LU, ldl, lbs, dinv, ipiv, perm, info, retry); break;
+      case 3: hipLaunchKernelGGL((k_getrf_ll<3, T>), g, blk, 0, st, K, ldk, kbs, LU, ldl, lbs, dinv, ipiv, perm, info, retry); break;
+      default: hipLaunchKernelGGL((k_getrf_ll<4, T>), g, blk, 0, st, K, ldk, kbs, LU, ldl, lbs, dinv, ipiv, perm, info, retry); break;
+    }
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  const int* only = fast ? retry : nullptr;
+  hipError_t e = fallback_getrf<T>(n, K, ldk, kbs, LU, ldl, lbs, batch, ipiv, perm, info, only, st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_trtri_diag<T>, dim3(batch * ntri(n)), dim3(64), 0, st, LU, ldl, lbs, n,
+                     dinv, only);
+  return hipGetLastError();
+}
+
+hipError_t launch_getrf_leaf(int n, const double* K, int64_t ldk, int64_t kbs, double* LU,
+                             int64_t ldl, int64_t lbs, int batch, double* dinv, int* ipiv,
+                             int* perm, int* info, int* retry, hipStream_t st) {
+  return getrf_leaf_impl<double>(n, K, ldk, kbs, LU, ldl, lbs, batch, dinv, ipiv, perm, info,
+                                 retry, st);
+}
+hipError_t launch_getrf_leaf_f32(int n, const double* K, int64_t ldk, int64_t kbs, float* LU,
+                                 int64_t ldl, int64_t lbs, int batch, float* dinv, int* ipiv,
+                                 int* perm, int* info, int* retry, hipStream_t st) {
+  return getrf_leaf_impl<float>(n, K, ldk, kbs, LU, ldl, lbs, batch, dinv, ipiv, perm, info,
+                                retry, st);
 }
 
 hipError_t launch_getrs_f32(int n, const float* LU, int64_t lda, int64_t bstride, int batch,

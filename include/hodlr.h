@@ -11,8 +11,10 @@
  *     off-diagonal block at every level equal to U(I,:) V(J,:)^T with global
  *     N x rank matrices U, V (src/tree.cc:59, :189-197);
  *   - the solution overwrites the right-hand side ("b will be overwritten by
- *     x", src/lmatrix.cc:447-448), and the leaf blocks K are overwritten by
- *     their LU factors, as dgesv does.
+ *     x", src/lmatrix.cc:447-448); the plan entry points only READ the leaf
+ *     blocks K (their LU factors are kept in the plan), while the
+ *     reference-layout hsolve and the batched LAPACK entry points overwrite
+ *     their blocks with the factors, as dgesv does.
  * All pointers passed to entry points taking a hipStream_t are DEVICE
  * pointers; work is enqueued on that stream and the call returns without
  * synchronizing unless stated.  Nothing here aborts: every entry point
@@ -91,9 +93,13 @@ int hodlr_plan_destroy(hodlr_plan plan);
 int64_t hodlr_plan_device_bytes(hodlr_plan plan);
 
 /* Factorization only (leaf LUs, the per-level Y_t blocks, core LUs).
- * K (N x leaf, ld ldk) is overwritten by the leaf LU factors -- except for a
- * HODLR_MIXED_IR plan, which factors an FP32 copy and leaves K untouched. */
-int hodlr_factor(hodlr_plan plan, double *K, int64_t ldk,
+ * K (N x leaf, ld ldk) is read, not modified: the leaf LU factors (FP64, or
+ * FP32 for a HODLR_MIXED_IR plan) are stored in the plan, so the same K can
+ * serve the refinement residuals.  Leaf blocks whose partial-pivoting LU
+ * keeps every diagonal pivot (|multiplier| <= 1 - 2^-20 at every step) are
+ * factored without interchanges on a fast path; the others with row
+ * interchanges, as dgetrf (the pivots equal dgetrf's either way). */
+int hodlr_factor(hodlr_plan plan, const double *K, int64_t ldk,
                  const double *U, int64_t ldu, const double *V, int64_t ldv,
                  hipStream_t stream);
 
@@ -106,7 +112,7 @@ int hodlr_solve(hodlr_plan plan, double *B, int64_t ldb, int nrhs, hipStream_t s
  * spmd driver level loop (spmd_driver/solver_irregular_part.cc:104-140),
  * which factors and solves together.  Leaves the factorization in the plan
  * for later hodlr_solve calls. */
-int hodlr_factor_solve(hodlr_plan plan, double *K, int64_t ldk,
+int hodlr_factor_solve(hodlr_plan plan, const double *K, int64_t ldk,
                        const double *U, int64_t ldu, const double *V, int64_t ldv,
                        double *B, int64_t ldb, int nrhs, hipStream_t stream);
 
@@ -115,7 +121,7 @@ int hodlr_factor_solve(hodlr_plan plan, double *K, int64_t ldk,
  *   X = solve(B); repeat { R = B - A X (FP64 HODLR matvec with the ORIGINAL
  *   K, U, V); stop if max_j |R_j|_inf / |B_j|_inf <= tol; X += solve(R) }
  * with solve() the plan's stored factorization (FP32 leaf LU for a
- * HODLR_MIXED_IR plan, whose hodlr_factor leaves K untouched).  At most
+ * HODLR_MIXED_IR plan).  At most
  * maxit corrections.  Synchronizes `stream` once per iteration.  Returns
  * HODLR_OK whether or not tol was reached (see stats->converged).  No
  * reference counterpart (the reference solves in FP64 only). */
@@ -277,6 +283,17 @@ int hodlr_node_solve(int nnode, int r, const double *VTu, int64_t ldu,
  * dgetrf_ calls behind dgesv_ (include/lapack_blas.hpp:45-57). */
 int hodlr_dgetrf_batched(int n, double *A, int64_t lda, int64_t bstride, int batch,
                          int *ipiv, int *info, hipStream_t stream);
+/* Out-of-place batched dgetrf (dgetrf.f): LU_b (n x n, ld ldl, LU + b*lbs)
+ * <- the LU factors with partial pivoting of K_b (ld ldk, K + b*kbs), K
+ * not modified.  ipiv (batch x n, 1-based) and info (batch) are device
+ * arrays with dgetrf's meaning.  The leaf factorization of the plans:
+ * blocks whose pivots are all diagonal take the no-interchange fast path
+ * (n = 128, 256, 384 or 512), the others the pivoted kernels.  Replaces the
+ * dgetrf_ call behind dgesv_ (include/lapack_blas.hpp:45-57) for callers
+ * that keep their matrix. */
+int hodlr_dgetrf_batched_oop(int n, const double *K, int64_t ldk, int64_t kbs, double *LU,
+                             int64_t ldl, int64_t lbs, int batch, int *ipiv, int *info,
+                             hipStream_t stream);
 /* Batched dgesv_ (include/lapack_blas.hpp:45-50): A_b overwritten by LU,
  * B_b = B + b*bbstride (n x nrhs, ld ldb) by the solution.  ipiv (batch x n,
  * 1-based) and info (batch ints, LAPACK INFO per item) are device arrays.

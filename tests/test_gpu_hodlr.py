@@ -114,6 +114,76 @@ def test_dgesv_batched_pivoting(gpu, orc, n):
             assert np.abs(LU[:, b * n:(b + 1) * n] - LUo).max() <= 1e-11 * np.abs(LUo).max()
 
 
+def _oop_vs_netlib(F, orc, blocks, n):
+    """Out-of-place batched LU of `blocks` against the restated netlib dgetrf:
+    identical IPIV and INFO, factors within 1e-13 (relative to the largest
+    factor entry), K untouched."""
+    batch = len(blocks)
+    Kh = np.vstack(blocks)
+    K = to_dev(F, Kh)
+    LU, ipiv, info = F.dgetrf_batched_oop(K, n, batch)
+    torch.cuda.synchronize()
+    assert np.array_equal(to_host(K), Kh)               # K is only read
+    LU = LU.cpu().numpy()
+    ipiv = ipiv.cpu().numpy()
+    info = info.cpu().numpy()
+    out = []
+    for b in range(batch):
+        LUo = np.asfortranarray(blocks[b].copy())
+        io, po = orc.dgetrf(LUo)
+        assert info[b] == io, (b, info[b], io)
+        assert np.array_equal(ipiv[b], po), (b, np.nonzero(ipiv[b] != po)[0][:5])
+        got = LU[b, :, :n].T
+        if io == 0:
+            assert np.abs(got - LUo).max() <= 1e-13 * np.abs(LUo).max(), b
+        out.append(po)
+    return out
+
+
+@pytest.mark.parametrize("n", [128, 256, 384, 512])
+def test_leaf_lu_fast_path_matches_netlib(gpu, orc, n):
+    """Diagonally dominant blocks G + n I: dgetrf keeps every diagonal pivot
+    and the no-interchange path runs; IPIV must be the identity exactly as
+    netlib's, the factors equal to rounding."""
+    say(f"case: leaf LU fast path n={n}")
+    F = gpu
+    rng = np.random.default_rng(n)
+    blocks = [rng.standard_normal((n, n)) + n * np.eye(n) for _ in range(5)]
+    for po in _oop_vs_netlib(F, orc, blocks, n):
+        assert np.array_equal(po, np.arange(1, n + 1))
+
+
+@pytest.mark.parametrize("n", [256, 512])
+def test_leaf_lu_late_pivot_and_ties(gpu, orc, n):
+    """Blocks that need an interchange only at a late column (rows j and
+    j + 5 of a dominant block swapped: dgetrf swaps them back at column j),
+    an exact tie |a_30| = |a_00| at column 0 (dgetrf keeps the diagonal: the
+    first of equal magnitudes), a near tie inside the pivot margin and one
+    outside it, a Gaussian block, and a block with a zero column: pivots and
+    INFO equal the restated netlib dgetrf's in every case."""
+    say(f"case: leaf LU pivots n={n}")
+    F = gpu
+    rng = np.random.default_rng(100 + n)
+    blocks = []
+    for j in (1, 70, n - 10, n - 6):
+        d = rng.standard_normal((n, n)) + n * np.eye(n)
+        d[[j, j + 5]] = d[[j + 5, j]]
+        blocks.append(d)
+    for f in (1.0, 1.0 - 1e-9, 1.0 - 1e-3):
+        d = rng.standard_normal((n, n)) + n * np.eye(n)
+        d[3, 0] = -f * d[0, 0]
+        blocks.append(d)
+    blocks.append(rng.standard_normal((n, n)))
+    d = rng.standard_normal((n, n)) + n * np.eye(n)
+    d[:, n // 2 + 3] = 0.0
+    blocks.append(d)
+    pos = _oop_vs_netlib(F, orc, blocks, n)
+    for k, j in enumerate((1, 70, n - 10, n - 6)):
+        want = np.arange(1, n + 1)
+        want[j] = j + 6
+        assert np.array_equal(pos[k], want), j
+
+
 @pytest.mark.parametrize("n,lda", [(300, (1 << 22) + 64), (40, (1 << 24) + 64)])
 def test_dgesv_large_leading_dimension(gpu, n, lda):
     """Leading dimensions past the 32-bit byte-offset limits of the fast
