@@ -20,7 +20,7 @@ import torch
 
 __all__ = [
     "lib", "lib_path", "available", "fortran_empty", "fortran", "HodlrPlan", "hsolve",
-    "gemm_red", "gemm_bro", "node_solve", "matvec", "generate", "workload_gaussian",
+    "gemm_red", "gemm_bro", "node_solve", "node_solve_region", "matvec", "generate", "workload_gaussian",
     "dgesv_batched", "HodlrError", "grad", "HodlrSolveFunction", "hodlr_solve", "fortran_clone",
     "FP64", "MIXED_IR", "IRStats", "HodlrBatch",
 ]
@@ -108,6 +108,8 @@ def lib():
         "hodlr_gemm_red": ([i64, ci, ci, dbl, vp, i64, ci, vp, i64, ci, vp, i64, vp], ci),
         "hodlr_gemm_bro": ([i64, ci, ci, dbl, vp, i64, ci, vp, i64, vp, i64, ci, vp], ci),
         "hodlr_node_solve": ([ci, ci, vp, i64, vp, i64, ci, vp, vp], ci),
+        "hodlr_node_solve_region": ([ci, ci, ci, ci, vp, i64, i64, vp, i64, i64, vp, i64, i64,
+                                     vp, i64, i64, vp, vp], ci),
         "hodlr_dgetrf_batched": ([ci, vp, i64, i64, ci, vp, vp, vp], ci),
         "hodlr_dgetrf_batched_oop": ([ci, vp, i64, i64, vp, i64, i64, ci, vp, vp, vp], ci),
         "hodlr_dgesv_batched": ([ci, ci, vp, i64, i64, ci, vp, vp, vp, i64, i64, vp], ci),
@@ -417,6 +419,42 @@ def node_solve(VTu, VTd, r):
     info = torch.zeros(nnode, dtype=torch.int32, device=VTu.device)
     _call(lib().hodlr_node_solve, nnode, r, _ptr(VTu), _ld(VTu), _ptr(VTd), _ld(VTd),
           VTd.shape[1], _ptr(info), _stream(), where="hodlr_node_solve")
+    return info
+
+
+def node_solve_region(VTu0, VTu1, VTd0, VTd1):
+    """Static LMatrix::node_solve(VTu0, VTu1, VTd0, VTd1) / NodeSolveRegionTask
+    (lmatrix.cc:583-636, node_solve_region.cc:35-108) for sibling ranks
+    r0 != r1: VTu0 is r1 x r0, VTu1 r0 x r1, VTd0 r1 x nrhs, VTd1 r0 x nrhs;
+    VTd1, VTd0 are overwritten by the two halves of the core solution.
+    3-D operands (nnode, rows, cols) solve nnode nodes in one call.
+    Returns the per-node LAPACK INFO as a device int32 tensor."""
+    ops = (VTu0, VTu1, VTd0, VTd1)
+    batched = VTu0.dim() == 3
+    if batched:
+        for t, nm in zip(ops, ("VTu0", "VTu1", "VTd0", "VTd1")):
+            if t.dim() != 3 or t.dtype != torch.float64 or not t.is_cuda or t.stride(1) != 1:
+                raise ValueError(f"{nm}: expected a CUDA float64 (nnode, rows, cols) tensor "
+                                 "with column-major matrices")
+        nnode = VTu0.shape[0]
+        mats = [(max(t.stride(2), t.shape[1]) if t.shape[2] > 1 else t.shape[1], t.stride(0))
+                for t in ops]
+    else:
+        for t, nm in zip(ops, ("VTu0", "VTu1", "VTd0", "VTd1")):
+            _check(t, nm)
+        nnode = 1
+        mats = [(_ld(t), 0) for t in ops]
+    r1, r0 = VTu0.shape[-2], VTu0.shape[-1]
+    if (VTu1.shape[-2], VTu1.shape[-1]) != (r0, r1) or VTd0.shape[-2] != r1 or \
+            VTd1.shape[-2] != r0 or VTd0.shape[-1] != VTd1.shape[-1]:
+        raise ValueError("node_solve_region: VTu0 r1 x r0, VTu1 r0 x r1, VTd0 r1 x nrhs, "
+                         "VTd1 r0 x nrhs expected")
+    info = torch.zeros(nnode, dtype=torch.int32, device=VTu0.device)
+    args = []
+    for t, (ld, st) in zip(ops, mats):
+        args += [_ptr(t), max(ld, 1), st]
+    _call(lib().hodlr_node_solve_region, nnode, r0, r1, VTd0.shape[-1], *args, _ptr(info),
+          _stream(), where="hodlr_node_solve_region")
     return info
 
 

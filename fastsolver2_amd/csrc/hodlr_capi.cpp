@@ -170,12 +170,11 @@ struct CoreBufs {
   int* general = nullptr;  // per-core flags of the structured LU (null: general LU)
 };
 
-// Core step shared by every level: S (already assembled) is factored in
-// place, R (2r x nh per node) overwritten by S^-1 R.
+// Core step shared by every level: S (already assembled, n2 x n2 per node)
+// is factored in place, R (n2 x nh per node) overwritten by S^-1 R.
 double lu_flops(double n) { return 2.0 / 3.0 * n * n * n; }
 
-int core_factor_solve(int nnode, int r, const CoreBufs& c, int nh, hipStream_t st) {
-  const int n2 = 2 * r;
+int core_factor_solve_n(int nnode, int n2, const CoreBufs& c, int nh, hipStream_t st) {
   const double blk = 8.0 * n2 * n2 * nnode;
   {
     ProfScope ps(HODLR_PROF_GETRF_CORE, st, nnode * lu_flops(n2), 2.0 * blk);
@@ -198,6 +197,11 @@ int core_factor_solve(int nnode, int r, const CoreBufs& c, int nh, hipStream_t s
                                 (int64_t)n2 * nh, nh, st));
   }
   return HODLR_OK;
+}
+
+// The HODLR cores: 2r x 2r.
+int core_factor_solve(int nnode, int r, const CoreBufs& c, int nh, hipStream_t st) {
+  return core_factor_solve_n(nnode, 2 * r, c, nh, st);
 }
 
 }  // namespace
@@ -1152,6 +1156,38 @@ int hodlr_node_solve(int nnode, int r, const double* VTu, int64_t ldu, double* V
   for (int k = 0; k < nnode; ++k)
     HIP_TRY(hodlr::launch_copy_cols(n2, nrhs, R + (int64_t)k * n2 * nrhs, n2,
                                     VTd + (int64_t)k * n2, ldd, st));
+  return HODLR_OK;
+}
+
+int hodlr_node_solve_region(int nnode, int r0, int r1, int nrhs, const double* VTu0, int64_t ldu0,
+                            int64_t su0, const double* VTu1, int64_t ldu1, int64_t su1,
+                            double* VTd0, int64_t ldd0, int64_t sd0, double* VTd1, int64_t ldd1,
+                            int64_t sd1, int* info, hipStream_t st) {
+  if (nnode <= 0 || r0 <= 0 || r1 <= 0 || nrhs < 0 || !VTu0 || !VTu1 || !info ||
+      (nrhs > 0 && (!VTd0 || !VTd1)))
+    return HODLR_ERR_ARG;
+  if (ldu0 < r1 || ldu1 < r0 || (nrhs > 0 && (ldd0 < r1 || ldd1 < r0))) return HODLR_ERR_ARG;
+  if (nnode > 1 && (su0 < 0 || su1 < 0 || sd0 < 0 || sd1 < 0)) return HODLR_ERR_ARG;
+  const int n = r0 + r1;
+  if (n > hodlr::kMaxBlock) return HODLR_ERR_UNSUPPORTED;
+  const int64_t nS = (int64_t)nnode * n * n, nR = (int64_t)nnode * n * std::max(nrhs, 1);
+  const int64_t nD = (int64_t)nnode * hodlr::dinv_elems(n);
+  void* w = nullptr;
+  int rc = workspace(sizeof(double) * (size_t)(nS + nR + nD) +
+                         sizeof(int) * (size_t)((int64_t)nnode * 2 * n + 16), st, &w);
+  if (rc) return rc;
+  double* S = static_cast<double*>(w);
+  double* R = S + nS;
+  double* dinv = R + nR;
+  int* ipiv = reinterpret_cast<int*>(dinv + nD);
+  int* perm = ipiv + (int64_t)nnode * n;
+  HIP_TRY(hodlr::launch_region_assemble(nnode, r0, r1, nrhs, VTu0, ldu0, su0, VTu1, ldu1, su1,
+                                        VTd0, ldd0, sd0, VTd1, ldd1, sd1, S, R, st));
+  CoreBufs c{S, R, ipiv, perm, info, dinv};
+  rc = core_factor_solve_n(nnode, n, c, nrhs, st);
+  if (rc) return rc;
+  HIP_TRY(hodlr::launch_region_scatter(nnode, r0, r1, nrhs, R, VTd0, ldd0, sd0, VTd1, ldd1, sd1,
+                                       st));
   return HODLR_OK;
 }
 

@@ -83,9 +83,29 @@ static long next_base_seed() {
   return g_master + v;
 }
 
+static bool g_clock_set = false;
+static long g_clock = 0;
+void set_rand_clock(long t) {
+  std::lock_guard<std::mutex> lk(g_seed_mu);
+  g_clock = t;
+  g_clock_set = true;
+}
+void clear_rand_clock() {
+  std::lock_guard<std::mutex> lk(g_seed_mu);
+  g_clock_set = false;
+}
+static long call_seed() {
+  {
+    std::lock_guard<std::mutex> lk(g_seed_mu);
+    if (g_clock_set) return g_clock;
+  }
+  return next_base_seed();
+}
+
+// matrix.cc:254-262: srand48_r(time) then one lrand48_r per partition.
 static std::vector<long> partition_seeds(int nPart) {
   struct drand48_data buf;
-  srand48_r(next_base_seed(), &buf);
+  srand48_r(call_seed(), &buf);
   std::vector<long> s(nPart);
   for (int i = 0; i < nPart; ++i) lrand48_r(&buf, &s[i]);
   return s;
@@ -471,10 +491,14 @@ static void build_views(LMatrix& U, int mLevel, int nRhs, int rank, std::vector<
     uv.push_back(u);
   }
 }
+// tree.cc:42-76: the region holds one u copy per matrix level
+// (UMat.levels()); the level views cover the top `level` levels, the leaf
+// tasks' hsolve the rest.
 void UTree::partition(int level, const Context& ctx) {
   require(UMat.rows() > 0 && UMat.cols() > 0, "UTree::partition");
+  require(level >= 0 && level <= UMat.levels(), "UTree::partition: level <= matrix levels");
   mLevel = level;
-  U.create(UMat.rows(), nRhs + rank * mLevel, ctx);
+  U.create(UMat.rows(), nRhs + rank * UMat.levels(), ctx);
   U.partition(mLevel, ctx);
   U.init_data(nRhs, U.cols(), UMat, ctx);
   build_views(U, mLevel, nRhs, rank, dMat_vec, uMat_vec);
@@ -491,6 +515,14 @@ LMatrix& UTree::uMat_level_new(int i) {
 LMatrix& UTree::dMat_level_new(int i) {
   require(0 <= i && i < mLevel, "UTree::dMat_level_new");
   return dMat_vec[i];
+}
+LMatrix& UTree::uMat_level(int i) {
+  require(0 < i && i <= mLevel, "UTree::uMat_level");
+  return uMat_vec[i - 1];
+}
+LMatrix& UTree::dMat_level(int i) {
+  require(0 < i && i <= mLevel, "UTree::dMat_level");
+  return dMat_vec[i - 1];
 }
 Matrix UTree::solution(const Context& ctx) { return U.to_matrix(0, nRhs, ctx); }
 

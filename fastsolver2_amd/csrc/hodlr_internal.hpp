@@ -131,6 +131,17 @@ hipError_t launch_update(const double* u, int64_t ldu, int r, const double* eta,
 hipError_t launch_node_assemble(int nnode, int r, const double* VTu, int64_t ldu,
                                 const double* VTd, int64_t ldd, int nrhs, double* S, double* R,
                                 hipStream_t st);
+// Cores of sibling ranks r0 != r1 ((r0+r1) x (r0+r1)) and R = [VTd1; VTd0]
+// for NodeSolveRegionTask; node k's operands at element offsets k*s*.
+hipError_t launch_region_assemble(int nnode, int r0, int r1, int nrhs, const double* VTu0,
+                                  int64_t ldu0, int64_t su0, const double* VTu1, int64_t ldu1,
+                                  int64_t su1, const double* VTd0, int64_t ldd0, int64_t sd0,
+                                  const double* VTd1, int64_t ldd1, int64_t sd1, double* S,
+                                  double* R, hipStream_t st);
+// R (node-major (r0+r1) x nrhs) back into VTd1 (rows [0, r0)) and VTd0.
+hipError_t launch_region_scatter(int nnode, int r0, int r1, int nrhs, const double* R,
+                                 double* VTd0, int64_t ldd0, int64_t sd0, double* VTd1,
+                                 int64_t ldd1, int64_t sd1, hipStream_t st);
 
 // Slot-mapped reduce / broadcast for the LMatrix operators.
 hipError_t launch_sum_slots(const double* part, int nchunk, int nslot, int ka, int kb,

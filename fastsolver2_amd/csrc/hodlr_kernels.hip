@@ -3113,6 +3113,81 @@ hipError_t launch_node_assemble(int nnode, int r, const double* VTu, int64_t ldu
   return hipGetLastError();
 }
 
+// Cores of sibling ranks r0 != r1 (NodeSolveRegionTask,
+// node_solve_region.cc:77-96), n = r0 + r1, node k's operands at element
+// offsets k*s*:  S = I_n with S(i, r0+j) = VTu1(i, j), S(r0+j, i) = VTu0(j, i)
+// (i < r0, j < r1), and R = [VTd1; VTd0] (n x nrhs, ld n).
+__global__ void k_region_assemble(int r0, int r1, int nrhs, const double* __restrict__ VTu0,
+                                  int64_t ldu0, int64_t su0, const double* __restrict__ VTu1,
+                                  int64_t ldu1, int64_t su1, const double* __restrict__ VTd0,
+                                  int64_t ldd0, int64_t sd0, const double* __restrict__ VTd1,
+                                  int64_t ldd1, int64_t sd1, double* __restrict__ S,
+                                  double* __restrict__ R) {
+  const int64_t node = blockIdx.y;
+  const int n = r0 + r1;
+  const int64_t nS = (int64_t)n * n;
+  const int64_t total = nS + (int64_t)n * nrhs;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * 256) {
+    if (e < nS) {
+      const int i = (int)(e % n), j = (int)(e / n);
+      double v;
+      if ((i < r0) == (j < r0)) v = (i == j) ? 1.0 : 0.0;
+      else if (i < r0) v = VTu1[node * su1 + i + (int64_t)(j - r0) * ldu1];
+      else v = VTu0[node * su0 + (i - r0) + (int64_t)j * ldu0];
+      S[node * nS + e] = v;
+    } else {
+      const int64_t f = e - nS;
+      const int i = (int)(f % n), j = (int)(f / n);
+      R[node * n * nrhs + f] = (i < r0) ? VTd1[node * sd1 + i + (int64_t)j * ldd1]
+                                        : VTd0[node * sd0 + (i - r0) + (int64_t)j * ldd0];
+    }
+  }
+}
+
+// Solution back into the slots (node_solve_region.cc:99-107): VTd1 <- rows
+// [0, r0) of R, VTd0 <- rows [r0, r0+r1).
+__global__ void k_region_scatter(int r0, int r1, int nrhs, const double* __restrict__ R,
+                                 double* __restrict__ VTd0, int64_t ldd0, int64_t sd0,
+                                 double* __restrict__ VTd1, int64_t ldd1, int64_t sd1) {
+  const int64_t node = blockIdx.y;
+  const int n = r0 + r1;
+  const int64_t total = (int64_t)n * nrhs;
+  for (int64_t f = (int64_t)blockIdx.x * 256 + threadIdx.x; f < total;
+       f += (int64_t)gridDim.x * 256) {
+    const int i = (int)(f % n), j = (int)(f / n);
+    const double v = R[node * total + f];
+    if (i < r0) VTd1[node * sd1 + i + (int64_t)j * ldd1] = v;
+    else VTd0[node * sd0 + (i - r0) + (int64_t)j * ldd0] = v;
+  }
+}
+
+hipError_t launch_region_assemble(int nnode, int r0, int r1, int nrhs, const double* VTu0,
+                                  int64_t ldu0, int64_t su0, const double* VTu1, int64_t ldu1,
+                                  int64_t su1, const double* VTd0, int64_t ldd0, int64_t sd0,
+                                  const double* VTd1, int64_t ldd1, int64_t sd1, double* S,
+                                  double* R, hipStream_t st) {
+  if (nnode <= 0) return hipSuccess;
+  const int64_t n = (int64_t)r0 + r1;
+  int64_t gx = (n * n + n * nrhs + 255) / 256;
+  if (gx > 64) gx = 64;
+  hipLaunchKernelGGL(k_region_assemble, dim3((unsigned)gx, nnode), dim3(256), 0, st, r0, r1, nrhs,
+                     VTu0, ldu0, su0, VTu1, ldu1, su1, VTd0, ldd0, sd0, VTd1, ldd1, sd1, S, R);
+  return hipGetLastError();
+}
+
+hipError_t launch_region_scatter(int nnode, int r0, int r1, int nrhs, const double* R,
+                                 double* VTd0, int64_t ldd0, int64_t sd0, double* VTd1,
+                                 int64_t ldd1, int64_t sd1, hipStream_t st) {
+  const int64_t total = ((int64_t)r0 + r1) * nrhs;
+  if (nnode <= 0 || total <= 0) return hipSuccess;
+  int64_t gx = (total + 255) / 256;
+  if (gx > 64) gx = 64;
+  hipLaunchKernelGGL(k_region_scatter, dim3((unsigned)gx, nnode), dim3(256), 0, st, r0, r1, nrhs,
+                     R, VTd0, ldd0, sd0, VTd1, ldd1, sd1);
+  return hipGetLastError();
+}
+
 // Slot sums for LMatrix::gemmRed: C[slot*ka + a, j] = alpha * sum of the
 // nper consecutive partials (ka x kb, ld ka) that belong to the slot.
 __global__ void k_sum_slots(const double* __restrict__ part, int nper, int nslot, int ka, int kb,

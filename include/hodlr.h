@@ -276,6 +276,25 @@ int hodlr_gemm_bro(int64_t rows, int nchunk, int nslot, double alpha,
 int hodlr_node_solve(int nnode, int r, const double *VTu, int64_t ldu,
                      double *VTd, int64_t ldd, int nrhs, int *info, hipStream_t stream);
 
+/* Static LMatrix::node_solve(VTu0, VTu1, VTd0, VTd1, ...) /
+ * NodeSolveRegionTask (src/lmatrix.cc:583-636, src/tasks/node_solve_region.cc:
+ * 35-108): the core solve of siblings with different ranks r0 != r1
+ * (spmd_driver/solver_irregular_ranks.cc:364-365).  VTu0 is r1 x r0,
+ * VTu1 r0 x r1, VTd0 r1 x nrhs, VTd1 r0 x nrhs; per node
+ *   S = I_{r0+r1},  S(i, r0+j) = VTu1(i, j),  S(r0+j, i) = VTu0(j, i),
+ *   B = [VTd1; VTd0],  B <- S^-1 B (partial pivoting, dgesv),
+ *   VTd1 <- B(0:r0, :),  VTd0 <- B(r0:r0+r1, :).
+ * Batched over nnode nodes: node k's operands start at element offsets
+ * k*su0, k*su1, k*sd0, k*sd1 (the reference solves one node per task).
+ * `info` is a device array of nnode ints (LAPACK INFO of each core).
+ * Asynchronous on `stream`. */
+int hodlr_node_solve_region(int nnode, int r0, int r1, int nrhs,
+                            const double *VTu0, int64_t ldu0, int64_t su0,
+                            const double *VTu1, int64_t ldu1, int64_t su1,
+                            double *VTd0, int64_t ldd0, int64_t sd0,
+                            double *VTd1, int64_t ldd1, int64_t sd1,
+                            int *info, hipStream_t stream);
+
 /* ------------------------------------------------------ batched LAPACK */
 /* Batched LU with partial pivoting (dgetrf semantics, dgetrf.f) of `batch`
  * n x n blocks A_b = A + b*bstride (ld lda) in place.  ipiv (batch x n,

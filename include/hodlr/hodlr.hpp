@@ -40,6 +40,13 @@ class Error : public std::runtime_error {
 // Master seed for the drand48-based generators (default 0).
 void set_master_seed(long seed);
 long master_seed();
+// The value the reference's generators take from time(NULL)
+// (src/matrix.cc:56,83,254,275): while set, every Matrix/Vector::rand call
+// seeds its partition stream with srand48_r(t) and draws one lrand48_r
+// seed per partition, exactly as the reference does within one second of
+// wall-clock time; clear_rand_clock() returns to the master-seed stream.
+void set_rand_clock(long t);
+void clear_rand_clock();
 
 class Matrix;
 
@@ -191,6 +198,9 @@ class UTree {
   Matrix solution(const Context& ctx);
   LMatrix& uMat_level_new(int i);
   LMatrix& dMat_level_new(int i);
+  // tree.cc:103-111: 1-based level accessors of the test drivers
+  LMatrix& uMat_level(int i);
+  LMatrix& dMat_level(int i);
   LMatrix& leaf() { return U; }
   void clear(const Context& ctx) { U.clear(ctx); }
   int num_rhs() const { return nRhs; }
@@ -209,6 +219,7 @@ class VTree {
   void partition(int level, const Context& ctx);
   void horizontal_partition(int task_level, const Context& ctx);
   LMatrix& leaf() { return V; }
+  LMatrix& level(int) { return V; }
   LMatrix& level_new(int) { return V; }
   void clear(const Context& ctx) { V.clear(ctx); }
 

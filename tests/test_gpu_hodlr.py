@@ -486,6 +486,91 @@ def test_operators_bitwise_on_integer_inputs(gpu, orc):
     assert np.array_equal(to_host(VTd_g), VTd_o)
 
 
+@pytest.mark.parametrize("r0,r1,nrhs,nnode,scale", [
+    (5, 9, 3, 1, 1.0),       # the reference's uniform [0,1) operands, one node
+    (64, 100, 17, 6, 0.5),   # Gaussian cores that pivot, batched
+    (30, 1, 4, 3, 2.0),
+    (1, 30, 2, 3, 2.0),
+    (300, 200, 9, 2, 0.1),   # largest case: a 500 x 500 core
+])
+def test_node_solve_region_matches_oracle(gpu, orc, r0, r1, nrhs, nnode, scale):
+    """hodlr_node_solve_region (NodeSolveRegionTask, node_solve_region.cc:
+    77-107: S(i, r0+j) = VTu1(i,j), S(r0+j, i) = VTu0(j,i), B = [VTd1; VTd0])
+    per node against orc_node_solve_region, and the core equations S x = B
+    themselves."""
+    say(f"case: node_solve_region r0={r0} r1={r1} nrhs={nrhs} nnode={nnode}")
+    F = gpu
+    rng = np.random.default_rng(r0 * 1000 + r1)
+    if scale == 1.0:
+        mk = lambda *s: rng.random(s)  # noqa: E731
+    else:
+        mk = lambda *s: scale * rng.standard_normal(s)  # noqa: E731
+    VTu0, VTu1 = mk(nnode, r1, r0), mk(nnode, r0, r1)
+    VTd0, VTd1 = mk(nnode, r1, nrhs), mk(nnode, r0, nrhs)
+
+    def dev3(a):
+        t = torch.empty((a.shape[0], a.shape[2], a.shape[1]), dtype=torch.float64,
+                        device="cuda").transpose(1, 2)
+        t.copy_(torch.from_numpy(a))
+        return t
+
+    g = [dev3(a) for a in (VTu0, VTu1, VTd0, VTd1)]
+    if nnode == 1:
+        g = [t[0] for t in g]
+    info = F.node_solve_region(*g).cpu().numpy()
+    assert not info.any(), info
+    torch.cuda.synchronize()
+    d0g = g[2].cpu().numpy().reshape(nnode, r1, nrhs)
+    d1g = g[3].cpu().numpy().reshape(nnode, r0, nrhs)
+    for k in range(nnode):
+        d0 = np.array(VTd0[k], order="F", copy=True)
+        d1 = np.array(VTd1[k], order="F", copy=True)
+        assert orc.node_solve_region(np.asfortranarray(VTu0[k]), np.asfortranarray(VTu1[k]),
+                                     d0, d1) == 0
+        S = np.eye(r0 + r1)
+        S[:r0, r0:] = VTu1[k]
+        S[r0:, :r0] = VTu0[k]
+        # same pivots as the oracle; the two differ by summation order only,
+        # amplified by at most the core's condition number
+        tol = max(1e-13, 2e-15 * np.linalg.cond(S))
+        assert relerr(d1g[k], d1) <= tol and relerr(d0g[k], d0) <= tol, k
+        x = np.vstack([d1g[k], d0g[k]])
+        rhs = np.vstack([VTd1[k], VTd0[k]])
+        assert np.abs(S @ x - rhs).max() <= 1e-12 * np.abs(S).max() * np.abs(x).max() * (r0 + r1)
+
+
+def test_node_solve_region_exact_and_singular(gpu, orc):
+    """Integer cores with VTu0 = 0 (S = [I, A; 0, I]: x = [R1 - A R2; R2]
+    in exact integer arithmetic, bit-equal to the oracle), and a singular
+    core (I - VTu0 VTu1 singular) reporting the oracle's LAPACK INFO."""
+    say("case: node_solve_region exact / singular")
+    F = gpu
+    rng = np.random.default_rng(21)
+    r0, r1, nrhs = 7, 4, 3
+    VTu0 = np.zeros((r1, r0))
+    VTu1 = rng.integers(-3, 4, (r0, r1)).astype(np.float64)
+    VTd0 = rng.integers(-5, 6, (r1, nrhs)).astype(np.float64)
+    VTd1 = rng.integers(-5, 6, (r0, nrhs)).astype(np.float64)
+    g = [to_dev(F, a) for a in (VTu0, VTu1, VTd0, VTd1)]
+    assert not F.node_solve_region(*g).cpu().numpy().any()
+    d0, d1 = VTd0.copy(order="F"), VTd1.copy(order="F")
+    assert orc.node_solve_region(F_(VTu0), F_(VTu1), d0, d1) == 0
+    assert np.array_equal(to_host(g[3]), d1) and np.array_equal(to_host(g[2]), d0)
+    assert np.array_equal(d0, VTd0) and np.array_equal(d1, VTd1 - VTu1 @ VTd0)
+    # singular: r0 = 1, r1 = 2, I - VTu0 VTu1 = diag(0, 1)
+    VTu0 = np.array([[1.0], [0.0]])
+    VTu1 = np.array([[1.0, 0.0]])
+    VTd0, VTd1 = np.ones((2, 1)), np.ones((1, 1))
+    g = [to_dev(F, a) for a in (VTu0, VTu1, VTd0, VTd1)]
+    info = int(F.node_solve_region(*g).cpu().numpy()[0])
+    info_o = orc.node_solve_region(F_(VTu0), F_(VTu1), F_(VTd0), F_(VTd1))
+    assert info > 0 and info == info_o, (info, info_o)
+
+
+def F_(a):
+    return np.asfortranarray(a, dtype=np.float64)
+
+
 def test_matvec_matches_oracle(gpu, orc):
     say("case: matvec N=4096 m=128 r=12 s=5")
     F = gpu
