@@ -34,11 +34,13 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def flops_alg(N, m, r, s):
+def flops_alg(N, m, r, s, L=None):
     """Algorithmic FLOPs of one factor+solve with the reference's column
-    layout (leading terms; L = log2(N/m) levels, t = 0 is the root)."""
+    layout (leading terms; L = log2(N/m) levels unless given, t = 0 is the
+    root; N/m leaves)."""
     P = N // m
-    L = int(round(math.log2(P)))
+    if L is None:
+        L = int(round(math.log2(P)))
     f = P * (2.0 / 3.0) * m ** 3 + 2.0 * P * m * m * r
     for t in range(L):
         f += 2 * N * r * r + 4 * N * r * r * t + 2 ** t * (16.0 / 3.0) * r ** 3 + 2 ** t * 8 * r ** 3 * t
@@ -144,7 +146,8 @@ def bench_config4(F, torch, W, log):
 def bench_batch(F, torch, W, K_, log, nsys=256):
     """256 independent systems (N log-uniform in [4096, 65536] rounded to the
     leaf, leaf in {128, 256, 512}, rank uniform in [16, 100], 1 RHS U[0,1),
-    seed 0), padded to balanced trees; autotuned, one graph per batch."""
+    seed 0), P leaves each in a tree of 2^ceil(log2 P) leaf slots (the
+    trailing slots empty); autotuned, one graph per batch."""
     from fastsolver2_amd.workloads import batch_operands, batch_systems
     systems = batch_systems(nsys, seed=0)
     Ks, Us, Vs, Bs = batch_operands(systems, seed=0, pad=LD_PAD)
@@ -178,9 +181,9 @@ def bench_batch(F, torch, W, K_, log, nsys=256):
         R.add_(B)
         worst = max(worst, (R.abs().max() / B.abs().max()).item())
     ms = statistics.median(ts)
-    fl = sum(flops_alg(sy["n"], sy["leaf"], sy["rank"], 1) for sy in systems)
-    out = {"workload": f"{nsys} systems, N log-uniform [4096, 65536] (sum {sum(s['n_real'] for s in systems)}"
-                       f" rows, {sum(s['n'] for s in systems)} padded), leaf {{128,256,512}}, "
+    fl = sum(flops_alg(sy["n"], sy["leaf"], sy["rank"], 1, sy["levels"]) for sy in systems)
+    out = {"workload": f"{nsys} systems, N log-uniform [4096, 65536] (sum {sum(s['n'] for s in systems)}"
+                       f" rows, no padding: empty trailing leaf slots), leaf {{128,256,512}}, "
                        "rank U[16,100], 1 RHS, seed 0, one HIP graph",
            "ms": round(ms, 3), "systems_per_s": round(nsys * 1000.0 / ms, 1),
            "batches_per_s": round(1000.0 / ms, 3), "gflops_alg": round(fl / ms / 1e6, 1),

@@ -22,7 +22,7 @@ __all__ = [
     "lib", "lib_path", "available", "fortran_empty", "fortran", "HodlrPlan", "hsolve",
     "gemm_red", "gemm_bro", "node_solve", "node_solve_region", "matvec", "generate", "workload_gaussian",
     "dgesv_batched", "HodlrError", "grad", "HodlrSolveFunction", "hodlr_solve", "fortran_clone",
-    "FP64", "MIXED_IR", "IRStats", "HodlrBatch",
+    "FP64", "MIXED_IR", "PLAN_TRAILING_EMPTY", "IRStats", "HodlrBatch",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -55,6 +55,7 @@ class IRStats(C.Structure):
 
 FP64 = 0
 MIXED_IR = 1
+PLAN_TRAILING_EMPTY = 1
 
 
 class _Desc(C.Structure):
@@ -77,6 +78,9 @@ def lib():
         "hodlr_strerror": ([ci], C.c_char_p),
         "hodlr_last_error_detail": ([], C.c_char_p),
         "hodlr_plan_create": ([C.POINTER(vp), C.POINTER(_Desc)], ci),
+        "hodlr_plan_create_uneven": ([C.POINTER(vp), C.POINTER(_Desc), C.POINTER(ci)], ci),
+        "hodlr_plan_matvec": ([vp, ci, ci, vp, i64, vp, i64, vp, i64, vp, i64, dbl, dbl, vp, i64,
+                               vp], ci),
         "hodlr_plan_destroy": ([vp], ci),
         "hodlr_plan_device_bytes": ([vp], i64),
         "hodlr_factor": ([vp, vp, i64, vp, i64, vp, i64, vp], ci),
@@ -200,18 +204,39 @@ def _call(fn, *args, where=""):
 # ---------------------------------------------------------------- plans
 class HodlrPlan:
     """Device factorization of one HODLR matrix (the working form of the
-    reference's HMatrix, include/hmatrix.hpp:8-34)."""
+    reference's HMatrix, include/hmatrix.hpp:8-34).
 
-    def __init__(self, n, leaf, levels, rank, nrhs_max=1, device=None, precision=FP64):
+    A balanced tree of 2^levels leaves of size `leaf` (n = leaf << levels);
+    with n < leaf << levels (a multiple of leaf) the last leaf slots are
+    empty (HODLR_PLAN_TRAILING_EMPTY); with `leaf_sizes` (2^levels entries,
+    0 for an empty slot) a tree of uneven leaves (hodlr_plan_create_uneven),
+    n their sum and `leaf` their maximum."""
+
+    def __init__(self, n, leaf, levels, rank, nrhs_max=1, device=None, precision=FP64,
+                 leaf_sizes=None):
         if not torch.cuda.is_available():
             raise RuntimeError("HodlrPlan needs a GPU")
         dev = torch.cuda.current_device() if device is None else int(device)
+        h = C.c_void_p()
+        if leaf_sizes is not None:
+            sizes = [int(x) for x in leaf_sizes]
+            if len(sizes) != 1 << levels:
+                raise ValueError("leaf_sizes needs 2^levels entries")
+            n, leaf = sum(sizes), max(sizes)
+            arr = (C.c_int * len(sizes))(*sizes)
+            d = _Desc(n, leaf, levels, rank, nrhs_max, precision, dev, 0)
+            _call(lib().hodlr_plan_create_uneven, C.byref(h), C.byref(d), arr,
+                  where="hodlr_plan_create_uneven")
+            self.leaf_sizes = sizes
+        else:
+            flags = PLAN_TRAILING_EMPTY if n < (leaf << levels) else 0
+            d = _Desc(n, leaf, levels, rank, nrhs_max, precision, dev, flags)
+            _call(lib().hodlr_plan_create, C.byref(h), C.byref(d), where="hodlr_plan_create")
+            nl = n // leaf
+            self.leaf_sizes = [leaf] * nl + [0] * ((1 << levels) - nl)
         self.n, self.leaf, self.levels, self.rank, self.nrhs_max = n, leaf, levels, rank, nrhs_max
         self.device = dev
         self.precision = precision
-        d = _Desc(n, leaf, levels, rank, nrhs_max, precision, dev, 0)
-        h = C.c_void_p()
-        _call(lib().hodlr_plan_create, C.byref(h), C.byref(d), where="hodlr_plan_create")
         self._h = h
         self._keep = ()
 
@@ -236,8 +261,24 @@ class HodlrPlan:
         if K.shape != (self.n, self.leaf) or U.shape != (self.n, self.rank) or V.shape != (self.n, self.rank):
             raise ValueError("K must be N x leaf, U and V N x rank")
 
+    def matvec(self, K, U, V, X, alpha=1.0, beta=0.0, Y=None, trans=False):
+        """Y = alpha * A X + beta * Y (A^T with trans) for the HODLR matrix
+        (K, U, V) on this plan's tree (hodlr_plan_matvec)."""
+        self._ops(K, U, V)
+        _check(X, "X")
+        X2 = X if X.dim() == 2 else X.view(-1, 1)
+        if Y is None:
+            Y = fortran_empty(X2.shape[0], X2.shape[1], device=X.device)
+            beta = 0.0
+        Y2 = Y if Y.dim() == 2 else Y.view(-1, 1)
+        _call(lib().hodlr_plan_matvec, self._h, 1 if trans else 0, X2.shape[1], _ptr(K), _ld(K),
+              _ptr(U), _ld(U), _ptr(V), _ld(V), _ptr(X2), _ld(X2), float(alpha), float(beta),
+              _ptr(Y2), _ld(Y2), _stream(), where="hodlr_plan_matvec")
+        return Y
+
     def factor(self, K, U, V):
-        """K is overwritten by the leaf LU factors (kept referenced)."""
+        """Factor A (K is only read; the leaf factors are kept in the plan,
+        and K, U, V stay referenced)."""
         self._ops(K, U, V)
         _call(lib().hodlr_factor, self._h, _ptr(K), _ld(K), _ptr(U), _ld(U), _ptr(V), _ld(V),
               _stream(), where="hodlr_factor")
@@ -299,7 +340,8 @@ class HodlrBatch:
     """Independent HODLR systems solved together (include/hodlr.h, batches):
     the batch is captured once as one HIP graph over concurrent lanes and
     replayed while the operand pointers stay the same.  `systems` is a list
-    of dicts with keys n, leaf, levels, rank and optionally nrhs_max."""
+    of dicts with keys n, leaf, levels, rank and optionally nrhs_max; a
+    system with n < leaf << levels has empty trailing leaf slots."""
 
     def __init__(self, systems, device=None, precision=FP64):
         if not torch.cuda.is_available():
@@ -308,8 +350,9 @@ class HodlrBatch:
         n = len(systems)
         descs = (_Desc * n)()
         for i, sy in enumerate(systems):
+            flags = PLAN_TRAILING_EMPTY if sy["n"] < (sy["leaf"] << sy["levels"]) else 0
             descs[i] = _Desc(sy["n"], sy["leaf"], sy["levels"], sy["rank"], sy.get("nrhs_max", 1),
-                             precision, dev, 0)
+                             precision, dev, flags)
         h = C.c_void_p()
         _call(lib().hodlr_batch_create, C.byref(h), n, C.cast(descs, C.c_void_p),
               where="hodlr_batch_create")

@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -214,8 +215,11 @@ bool hodlr::profiling_enabled() { return g_prof.on; }
 struct hodlr_plan_s {
   hodlr_desc d;
   int64_t N = 0;
-  int m = 0, L = 0, r = 0, smax = 0;
+  int m = 0, L = 0, r = 0, smax = 0;  // m: the largest leaf
   int64_t P = 0;
+  // leaves as runs of equal size (one run for a balanced tree of equal
+  // leaves; empty leaf slots form runs of size 0)
+  std::vector<hodlr::LeafRun> runs;
   int64_t wst = 0;  // per-leaf stride of the r x r blocks W_t,i and Q_i (r*r)
   // leaf factorization, out of place: leaf i's LU factors in the
   // contiguous block LU + i*lbs (ld ldl = leaf + pad), FP64 or (mixed
@@ -311,20 +315,42 @@ void plan_free(hodlr_plan p) {
   drop_graph(p);
 }
 
-// Leaf solve with the stored leaf factorization (FP64 or the FP32 copy).
+// The column view x with its rows shifted down by row0.
+ColSeg shift_rows(const ColSeg& x, int64_t row0) {
+  ColSeg s = x;
+  if (s.inA) s.inA += row0;
+  if (s.outA) s.outA += row0;
+  if (s.inB) s.inB += row0;
+  if (s.outB) s.outB += row0;
+  return s;
+}
+
+// Leaf solve with the stored leaf factorization (FP64 or the FP32 copy),
+// one launch per run of equal leaves.
 int leaf_solve(hodlr_plan p, const ColSeg& x, int ncols, bool trans, hipStream_t st) {
-  const int m = p->m;
-  if (p->mixed) {
-    HIP_TRY(hodlr::launch_getrs_f32(m, p->LU32, p->ldl, p->lbs, (int)p->P, p->leaf_perm,
-                                    p->leaf_dinv32, x, m, ncols, trans, st));
-  } else if (trans) {
-    HIP_TRY(hodlr::launch_getrs_t(m, p->LU, p->ldl, p->lbs, (int)p->P, p->leaf_perm, p->leaf_dinv,
-                                  x, m, ncols, st));
-  } else {
-    HIP_TRY(hodlr::launch_getrs(m, p->LU, p->ldl, p->lbs, (int)p->P, p->leaf_perm, p->leaf_dinv, x,
-                                m, ncols, st));
+  for (const hodlr::LeafRun& R : p->runs) {
+    if (R.m == 0) continue;
+    const ColSeg xr = shift_rows(x, R.row0);
+    const int* perm = p->leaf_perm + R.row0;
+    if (p->mixed) {
+      HIP_TRY(hodlr::launch_getrs_f32(R.m, p->LU32 + R.lu_off, R.ldl, R.lbs, (int)R.count, perm,
+                                      p->leaf_dinv32 + R.dinv_off, xr, R.m, ncols, trans, st));
+    } else if (trans) {
+      HIP_TRY(hodlr::launch_getrs_t(R.m, p->LU + R.lu_off, R.ldl, R.lbs, (int)R.count, perm,
+                                    p->leaf_dinv + R.dinv_off, xr, R.m, ncols, st));
+    } else {
+      HIP_TRY(hodlr::launch_getrs(R.m, p->LU + R.lu_off, R.ldl, R.lbs, (int)R.count, perm,
+                                  p->leaf_dinv + R.dinv_off, xr, R.m, ncols, st));
+    }
   }
   return HODLR_OK;
+}
+
+// sum over leaves of m^k
+double leaf_pow_sum(hodlr_plan p, int k) {
+  double s = 0.0;
+  for (const hodlr::LeafRun& R : p->runs) s += (double)R.count * std::pow((double)R.m, k);
+  return s;
 }
 
 CoreBufs core_level(hodlr_plan p, int t) {
@@ -345,18 +371,23 @@ int check_ld(int64_t ld, int64_t n) { return ld >= n ? HODLR_OK : HODLR_ERR_ARG;
 // only read), then [Y_{L-1} | B] = K^-1 [U | B].
 int leaf_phase(hodlr_plan p, const double* K, int64_t ldk, const double* U, int64_t ldu,
                double* B, int64_t ldb, int nrhs, hipStream_t st) {
-  const int m = p->m;
-  const double kb = (p->mixed ? 4.0 : 8.0) * (double)p->N * m;
-  {
-    ProfScope ps(HODLR_PROF_GETRF_LEAF, st, p->P * lu_flops(m), 8.0 * (double)p->N * m + kb);
+  const double m2 = leaf_pow_sum(p, 2);
+  const double kb = (p->mixed ? 4.0 : 8.0) * m2;
+  for (const hodlr::LeafRun& R : p->runs) {
+    if (R.m == 0) continue;
+    const double rm2 = (double)R.count * R.m * R.m;
+    ProfScope ps(HODLR_PROF_GETRF_LEAF, st, R.count * lu_flops(R.m),
+                 (8.0 + (p->mixed ? 4.0 : 8.0)) * rm2);
     if (p->mixed)
-      HIP_TRY(hodlr::launch_getrf_leaf_f32(m, K, ldk, m, p->LU32, p->ldl, p->lbs, (int)p->P,
-                                           p->leaf_dinv32, p->leaf_ipiv, p->leaf_perm,
-                                           p->leaf_info, p->leaf_retry, st));
+      HIP_TRY(hodlr::launch_getrf_leaf_f32(R.m, K + R.row0, ldk, R.m, p->LU32 + R.lu_off, R.ldl,
+                                           R.lbs, (int)R.count, p->leaf_dinv32 + R.dinv_off,
+                                           p->leaf_ipiv + R.row0, p->leaf_perm + R.row0,
+                                           p->leaf_info + R.first, p->leaf_retry + R.first, st));
     else
-      HIP_TRY(hodlr::launch_getrf_leaf(m, K, ldk, m, p->LU, p->ldl, p->lbs, (int)p->P,
-                                       p->leaf_dinv, p->leaf_ipiv, p->leaf_perm, p->leaf_info,
-                                       p->leaf_retry, st));
+      HIP_TRY(hodlr::launch_getrf_leaf(R.m, K + R.row0, ldk, R.m, p->LU + R.lu_off, R.ldl, R.lbs,
+                                       (int)R.count, p->leaf_dinv + R.dinv_off,
+                                       p->leaf_ipiv + R.row0, p->leaf_perm + R.row0,
+                                       p->leaf_info + R.first, p->leaf_retry + R.first, st));
   }
   ColSeg x;
   int ncols;
@@ -369,8 +400,7 @@ int leaf_phase(hodlr_plan p, const double* K, int64_t ldk, const double* U, int6
     ncols = nrhs;
   }
   if (ncols > 0) {
-    ProfScope ps(HODLR_PROF_GETRS_LEAF, st, 2.0 * (double)p->P * m * m * ncols,
-                 kb + 16.0 * (double)p->N * ncols);
+    ProfScope ps(HODLR_PROF_GETRS_LEAF, st, 2.0 * m2 * ncols, kb + 16.0 * (double)p->N * ncols);
     return leaf_solve(p, x, ncols, false, st);
   }
   return HODLR_OK;
@@ -383,24 +413,37 @@ int leaf_projections(hodlr_plan p, const double* Vm, int64_t ldv, const ColSeg& 
                      double* part, hipStream_t st) {
   ProfScope ps(HODLR_PROF_REDUCE, st, 2.0 * (double)p->N * p->r * nc,
                8.0 * (double)p->N * (p->r + nc));
-  int cpc = 0;
-  HIP_TRY(hodlr::launch_reduce(Vm, ldv, p->r, x, nc, p->m, (int)p->P, p->m, part, &cpc, st));
+  const int64_t pe = (int64_t)p->r * nc;
+  for (const hodlr::LeafRun& R : p->runs) {
+    if (R.m == 0) {  // empty leaf slots project to zero
+      HIP_TRY(hodlr::launch_fill_scale(R.count * pe, 1, part + R.first * pe, R.count * pe, 0, 0.0,
+                                       st));
+      continue;
+    }
+    int cpc = 0;
+    HIP_TRY(hodlr::launch_reduce(Vm + R.row0, ldv, p->r, shift_rows(x, R.row0), nc, R.m,
+                                 (int)R.count, R.m, part + R.first * pe, &cpc, st));
+  }
   return HODLR_OK;
 }
 
 // B_i -= Y_i G_i for every leaf (G_i: r x nc, ld r, stride r*nc).
 int final_update(hodlr_plan p, const double* Ym, int64_t ldy, const double* Gm, int nc, double* B,
                  int64_t ldb, hipStream_t st) {
-  hodlr::BGemmArgs g;
-  g.batch = (int)p->P; g.M = p->m; g.K = p->r; g.Nc = nc; g.csplit = nc;
-  g.A = Ym; g.lda = ldy; g.sa = p->m;
-  g.B = Gm; g.ldb = p->r; g.bshift = 0; g.bs1 = (int64_t)p->r * nc; g.bs2 = 2 * g.bs1;
-  g.alpha1 = -1.0; g.beta1 = 1.0;
-  g.D1 = B; g.ldd1 = ldb; g.sd1 = p->m;
-  g.C1 = B; g.ldc1 = ldb; g.sc1 = p->m;
   ProfScope ps(HODLR_PROF_UPDATE, st, 2.0 * (double)p->N * p->r * nc,
                8.0 * (double)p->N * (p->r + 2.0 * nc));
-  HIP_TRY(hodlr::launch_bgemm(g, st));
+  for (const hodlr::LeafRun& R : p->runs) {
+    if (R.m == 0) continue;
+    hodlr::BGemmArgs g;
+    g.batch = (int)R.count; g.M = R.m; g.K = p->r; g.Nc = nc; g.csplit = nc;
+    g.A = Ym + R.row0; g.lda = ldy; g.sa = R.m;
+    g.B = Gm + R.first * p->r * nc; g.ldb = p->r; g.bshift = 0; g.bs1 = (int64_t)p->r * nc;
+    g.bs2 = 2 * g.bs1;
+    g.alpha1 = -1.0; g.beta1 = 1.0;
+    g.D1 = B + R.row0; g.ldd1 = ldb; g.sd1 = R.m;
+    g.C1 = B + R.row0; g.ldc1 = ldb; g.sc1 = R.m;
+    HIP_TRY(hodlr::launch_bgemm(g, st));
+  }
   return HODLR_OK;
 }
 
@@ -551,41 +594,58 @@ int hodlr_profile_read(int kind, int64_t* count, double* total_ms, double* flops
   return HODLR_OK;
 }
 
-int hodlr_plan_create(hodlr_plan* plan, const hodlr_desc* desc) {
-  if (!plan || !desc) return HODLR_ERR_ARG;
-  *plan = nullptr;
-  const hodlr_desc& d = *desc;
-  if (d.leaf <= 0 || d.rank <= 0 || d.levels < 0 || d.levels > 24 || d.nrhs_max < 0 || d.flags)
-    return HODLR_ERR_ARG;
-  if (d.n != ((int64_t)d.leaf << d.levels)) return HODLR_ERR_SHAPE;
-  if (d.levels > 0 && d.leaf <= d.rank) return HODLR_ERR_SHAPE;  // matrix.cc:225
-  if (d.leaf > hodlr::kMaxBlock || 2 * d.rank > hodlr::kMaxBlock) return HODLR_ERR_UNSUPPORTED;
-  if (d.precision != HODLR_FP64 && d.precision != HODLR_MIXED_IR) return HODLR_ERR_ARG;
-  if (d.precision == HODLR_MIXED_IR && d.leaf > 512) return HODLR_ERR_UNSUPPORTED;
+// Plan of a balanced tree of 2^levels leaf slots with the given leaf sizes
+// (validated by the callers); d.n is their sum and d.leaf the largest.
+static int plan_create_sizes(hodlr_plan* plan, const hodlr_desc& d, const std::vector<int>& sizes) {
   DeviceGuard g(d.device);
   if (!g.ok) return HODLR_ERR_ARG;
   hodlr_plan p = new hodlr_plan_s();
   p->d = d;
   p->N = d.n; p->m = d.leaf; p->L = d.levels; p->r = d.rank; p->smax = d.nrhs_max;
   p->P = (int64_t)1 << d.levels;
-  const int r = p->r, n2 = 2 * r;
-  int rc = HODLR_OK;
-  rc = rc ? rc : palloc(p, &p->leaf_ipiv, p->P * p->m);
-  rc = rc ? rc : palloc(p, &p->leaf_perm, p->P * p->m);
-  rc = rc ? rc : palloc(p, &p->leaf_info, p->P);
   p->mixed = d.precision == HODLR_MIXED_IR;
-  // leaf factors: contiguous blocks, the column stride padded off a power
-  // of two (a leaf's columns would otherwise alias in the HBM channel
-  // interleave)
-  p->ldl = p->m + (p->mixed ? 16 : 8);
-  p->lbs = p->ldl * p->m;
+  const int r = p->r, n2 = 2 * r;
+  // runs of equal leaves; leaf factors in contiguous blocks, the column
+  // stride padded off a power of two (a leaf's columns would otherwise
+  // alias in the HBM channel interleave)
+  int64_t row0 = 0, lu = 0, dv = 0;
+  for (int64_t i = 0; i < p->P; ++i) {
+    const int m = sizes[(size_t)i];
+    if (p->runs.empty() || p->runs.back().m != m) {
+      hodlr::LeafRun R;
+      R.first = i; R.count = 0; R.m = m; R.row0 = row0; R.lu_off = lu; R.dinv_off = dv;
+      R.ldl = m > 0 ? m + (p->mixed ? 16 : 8) : 0;
+      R.lbs = R.ldl * m;
+      p->runs.push_back(R);
+    }
+    hodlr::LeafRun& R = p->runs.back();
+    R.count += 1;
+    row0 += m;
+    lu += R.lbs;
+    dv += m > 0 ? hodlr::dinv_elems(m) : 0;
+  }
+  if (p->runs.size() == 1) {
+    p->ldl = p->runs[0].ldl;
+    p->lbs = p->runs[0].lbs;
+  }
+  int rc = HODLR_OK;
+  rc = rc ? rc : palloc(p, &p->leaf_ipiv, std::max<int64_t>(p->N, 1));
+  rc = rc ? rc : palloc(p, &p->leaf_perm, std::max<int64_t>(p->N, 1));
+  rc = rc ? rc : palloc(p, &p->leaf_info, p->P);
   rc = rc ? rc : palloc(p, &p->leaf_retry, p->P);
   if (p->mixed) {
-    rc = rc ? rc : palloc(p, &p->LU32, p->P * p->lbs);
-    rc = rc ? rc : palloc(p, &p->leaf_dinv32, p->P * hodlr::dinv_elems(p->m));
+    rc = rc ? rc : palloc(p, &p->LU32, lu);
+    rc = rc ? rc : palloc(p, &p->leaf_dinv32, dv);
   } else {
-    rc = rc ? rc : palloc(p, &p->LU, p->P * p->lbs);
-    rc = rc ? rc : palloc(p, &p->leaf_dinv, p->P * hodlr::dinv_elems(p->m));
+    rc = rc ? rc : palloc(p, &p->LU, lu);
+    rc = rc ? rc : palloc(p, &p->leaf_dinv, dv);
+  }
+  // empty leaf slots never take a factorization: their INFO stays 0
+  if (!rc && p->runs.size() > 1) {
+    std::vector<int> zero((size_t)p->P, 0);
+    if (hipMemcpy(p->leaf_info, zero.data(), sizeof(int) * zero.size(), hipMemcpyHostToDevice) !=
+        hipSuccess)
+      rc = HODLR_ERR_HIP;
   }
   rc = rc ? rc : palloc(p, &p->Y, p->L > 0 ? p->N * r : 0);
   p->wst = (int64_t)r * r;
@@ -618,6 +678,66 @@ int hodlr_plan_create(hodlr_plan* plan, const hodlr_desc* desc) {
   }
   *plan = p;
   return HODLR_OK;
+}
+
+static int check_common(const hodlr_desc& d) {
+  if (d.rank <= 0 || d.levels < 0 || d.levels > 24 || d.nrhs_max < 0) return HODLR_ERR_ARG;
+  if (d.precision != HODLR_FP64 && d.precision != HODLR_MIXED_IR) return HODLR_ERR_ARG;
+  if (2 * d.rank > hodlr::kMaxBlock) return HODLR_ERR_UNSUPPORTED;
+  return HODLR_OK;
+}
+
+// One leaf size: 0 (empty slot) or above the rank (matrix.cc:225) and
+// within what the leaf kernels handle.
+static int check_leaf(const hodlr_desc& d, int m) {
+  if (m < 0) return HODLR_ERR_ARG;
+  if (m == 0) return HODLR_OK;
+  if (d.levels > 0 && m <= d.rank) return HODLR_ERR_SHAPE;
+  if (m > hodlr::kMaxBlock || (d.precision == HODLR_MIXED_IR && m > 512))
+    return HODLR_ERR_UNSUPPORTED;
+  return HODLR_OK;
+}
+
+int hodlr_plan_create(hodlr_plan* plan, const hodlr_desc* desc) {
+  if (!plan || !desc) return HODLR_ERR_ARG;
+  *plan = nullptr;
+  const hodlr_desc& d = *desc;
+  if (d.leaf <= 0 || (d.flags & ~HODLR_PLAN_TRAILING_EMPTY)) return HODLR_ERR_ARG;
+  int rc = check_common(d);
+  if (rc) return rc;
+  const int64_t slots = (int64_t)1 << d.levels;
+  int64_t nleaf = slots;
+  if (d.flags & HODLR_PLAN_TRAILING_EMPTY) {
+    if (d.n <= 0 || d.n % d.leaf || d.n / d.leaf > slots) return HODLR_ERR_SHAPE;
+    nleaf = d.n / d.leaf;
+  } else if (d.n != ((int64_t)d.leaf << d.levels)) {
+    return HODLR_ERR_SHAPE;
+  }
+  if ((rc = check_leaf(d, d.leaf))) return rc;
+  std::vector<int> sizes((size_t)slots, 0);
+  std::fill(sizes.begin(), sizes.begin() + nleaf, d.leaf);
+  return plan_create_sizes(plan, d, sizes);
+}
+
+int hodlr_plan_create_uneven(hodlr_plan* plan, const hodlr_desc* desc, const int* leaf_sizes) {
+  if (!plan || !desc || !leaf_sizes) return HODLR_ERR_ARG;
+  *plan = nullptr;
+  hodlr_desc d = *desc;
+  if (d.flags) return HODLR_ERR_ARG;
+  int rc = check_common(d);
+  if (rc) return rc;
+  const int64_t slots = (int64_t)1 << d.levels;
+  std::vector<int> sizes(leaf_sizes, leaf_sizes + slots);
+  int64_t n = 0;
+  int mmax = 0;
+  for (int m : sizes) {
+    if ((rc = check_leaf(d, m))) return rc;
+    n += m;
+    mmax = std::max(mmax, m);
+  }
+  if (mmax == 0 || d.n != n) return HODLR_ERR_SHAPE;
+  d.leaf = mmax;
+  return plan_create_sizes(plan, d, sizes);
 }
 
 int hodlr_plan_destroy(hodlr_plan plan) {
@@ -830,7 +950,7 @@ int hodlr_solve_ir(hodlr_plan p, const double* K, int64_t ldk, const double* U, 
     return HODLR_OK;
   }
   // residual block (N x nrhs), matvec workspace, norms of B and R
-  const int64_t mvw = hodlr::matvec_work_elems(N, p->m, p->r, nrhs);
+  const int64_t mvw = hodlr::matvec_work_elems(p->P, 1, p->r, nrhs);
   const int64_t need = N * nrhs + mvw + 2 * (int64_t)nrhs + (int64_t)hodlr::kColmaxParts * nrhs;
   if (p->ir_elems < need) {
     HIP_TRY(hipStreamSynchronize(st));
@@ -860,8 +980,8 @@ int hodlr_solve_ir(hodlr_plan p, const double* K, int64_t ldk, const double* U, 
   for (int it = 0;; ++it) {
     // R = B - A X
     HIP_TRY(hodlr::launch_copy_cols(N, nrhs, B, ldb, R, N, st));
-    HIP_TRY(hodlr::launch_matvec(N, p->m, p->r, nrhs, K, ldk, U, ldu, V, ldv, X, ldx, -1.0, 1.0, R,
-                                 N, work, st));
+    HIP_TRY(hodlr::launch_matvec_runs(p->runs.data(), (int)p->runs.size(), p->P, p->r, nrhs, K,
+                                      ldk, U, ldu, V, ldv, X, ldx, -1.0, 1.0, R, N, work, st));
     HIP_TRY(hodlr::launch_colmax(N, nrhs, R, N, nr, cpart, st));
     HIP_TRY(hipMemcpyAsync(hr.data(), nr, sizeof(double) * nrhs, hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -971,6 +1091,25 @@ static int matvec_impl(bool trans, int64_t n, int leaf, int rank, int nrhs, cons
   if (rc) return rc;
   HIP_TRY(hodlr::launch_matvec(n, leaf, rank, nrhs, K, ldk, U, ldu, V, ldv, X, ldx, alpha, beta, Y,
                                ldy, static_cast<double*>(w), st, trans));
+  return HODLR_OK;
+}
+
+int hodlr_plan_matvec(hodlr_plan p, int trans, int nrhs, const double* K, int64_t ldk,
+                      const double* U, int64_t ldu, const double* V, int64_t ldv, const double* X,
+                      int64_t ldx, double alpha, double beta, double* Y, int64_t ldy,
+                      hipStream_t st) {
+  if (!p || nrhs < 0 || !K || !U || !V || !X || !Y) return HODLR_ERR_ARG;
+  const int64_t n = p->N;
+  if (ldk < n || ldu < n || ldv < n || ldx < n || ldy < n) return HODLR_ERR_ARG;
+  if (nrhs == 0) return HODLR_OK;
+  DeviceGuard g(p->d.device);
+  void* w = nullptr;
+  int rc = workspace(sizeof(double) * (size_t)hodlr::matvec_work_elems(p->P, 1, p->r, nrhs), st,
+                     &w);
+  if (rc) return rc;
+  HIP_TRY(hodlr::launch_matvec_runs(p->runs.data(), (int)p->runs.size(), p->P, p->r, nrhs, K, ldk,
+                                    U, ldu, V, ldv, X, ldx, alpha, beta, Y, ldy,
+                                    static_cast<double*>(w), st, trans != 0));
   return HODLR_OK;
 }
 

@@ -147,7 +147,25 @@ hipError_t launch_region_scatter(int nnode, int r0, int r1, int nrhs, const doub
 hipError_t launch_sum_slots(const double* part, int nchunk, int nslot, int ka, int kb,
                             double alpha, double* C, int64_t ldc, hipStream_t st);
 
+// A run of consecutive leaves of one size m (m = 0: empty leaf slots) in a
+// tree of uneven leaves: leaves [first, first + count), rows starting at
+// row0, their LU factors at element offset lu_off of the plan's factor
+// buffer (leading dimension ldl, stride lbs per leaf) and their diagonal
+// inverses at dinv_off.
+struct LeafRun {
+  int64_t first = 0, count = 0;
+  int m = 0;
+  int64_t row0 = 0, lu_off = 0, dinv_off = 0, ldl = 0, lbs = 0;
+};
+
 // HODLR matvec pieces.
+// The matvec on a tree of uneven leaves (runs cover nleaf leaves; work as
+// matvec_work_elems(nleaf, 1, rank, nrhs)).
+hipError_t launch_matvec_runs(const LeafRun* runs, int nrun, int64_t nleaf, int rank, int nrhs,
+                              const double* K, int64_t ldk, const double* U, int64_t ldu,
+                              const double* V, int64_t ldv, const double* X, int64_t ldx,
+                              double alpha, double beta, double* Y, int64_t ldy, double* work,
+                              hipStream_t st, bool trans = false);
 hipError_t launch_matvec(int64_t n, int leaf, int rank, int nrhs, const double* K,
                          int64_t ldk, const double* U, int64_t ldu, const double* V,
                          int64_t ldv, const double* X, int64_t ldx, double alpha, double beta,

@@ -3418,35 +3418,66 @@ hipError_t noalloc_scratch(double** out) {
   return hipSuccess;
 }
 
-hipError_t launch_matvec(int64_t n, int leaf, int rank, int nrhs, const double* K, int64_t ldk,
-                         const double* U, int64_t ldu, const double* V, int64_t ldv,
-                         const double* X, int64_t ldx, double alpha, double beta, double* Y,
-                         int64_t ldy, double* work, hipStream_t st, bool trans) {
+hipError_t launch_matvec_runs(const LeafRun* runs, int nrun, int64_t nleaf, int rank, int nrhs,
+                              const double* K, int64_t ldk, const double* U, int64_t ldu,
+                              const double* V, int64_t ldv, const double* X, int64_t ldx,
+                              double alpha, double beta, double* Y, int64_t ldy, double* work,
+                              hipStream_t st, bool trans) {
   if (trans) {
     // A^T = blockdiag(K_i^T) + (V U^T off the leaf blocks): the same product
     // with U and V exchanged and the leaf blocks read transposed
     std::swap(U, V);
     std::swap(ldu, ldv);
   }
-  const int64_t nleaf = n / leaf;
-  double* part = work;
-  double* tot = work + nleaf * (int64_t)rank * nrhs;
-  ColSeg xs = seg_single(const_cast<double*>(X), ldx);
-  hipError_t e = launch_reduce(V, ldv, rank, xs, nrhs, leaf, (int)nleaf, leaf, part, nullptr, st);
-  if (e != hipSuccess) return e;
   const int64_t elems = (int64_t)rank * nrhs;
+  double* part = work;
+  double* tot = work + nleaf * elems;
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < nrun; ++k) {
+    const LeafRun& R = runs[k];
+    if (R.m == 0) {  // empty leaf slots project to zero
+      e = launch_fill_scale(R.count * elems, 1, part + R.first * elems, R.count * elems, 0, 0.0, st);
+    } else {
+      ColSeg xs = seg_single(const_cast<double*>(X) + R.row0, ldx);
+      e = launch_reduce(V + R.row0, ldv, rank, xs, nrhs, R.m, (int)R.count, R.m,
+                        part + R.first * elems, nullptr, st);
+    }
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(k_mv_total, dim3((unsigned)((elems + 15) / 16)), dim3(256), 0, st, part,
                      (int)nleaf, elems, tot);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  const int gpl = (leaf + 255) / 256;
-  dim3 grid((unsigned)(nleaf * gpl), (unsigned)((nrhs + 15) / 16));
-  if (trans)
-    hipLaunchKernelGGL((k_mv_apply<true>), grid, dim3(256), 0, st, leaf, rank, nrhs, K, ldk, U,
-                       ldu, X, ldx, part, tot, alpha, beta, Y, ldy, gpl);
-  else
-    hipLaunchKernelGGL((k_mv_apply<false>), grid, dim3(256), 0, st, leaf, rank, nrhs, K, ldk, U,
-                       ldu, X, ldx, part, tot, alpha, beta, Y, ldy, gpl);
-  return hipGetLastError();
+  for (int k = 0; k < nrun; ++k) {
+    const LeafRun& R = runs[k];
+    if (R.m == 0) continue;
+    const int gpl = (R.m + 255) / 256;
+    dim3 grid((unsigned)(R.count * gpl), (unsigned)((nrhs + 15) / 16));
+    const double* Kr = K + R.row0;
+    const double* Ur = U + R.row0;
+    const double* Xr = X + R.row0;
+    double* Yr = Y + R.row0;
+    const double* pr = part + R.first * elems;
+    if (trans)
+      hipLaunchKernelGGL((k_mv_apply<true>), grid, dim3(256), 0, st, R.m, rank, nrhs, Kr, ldk, Ur,
+                         ldu, Xr, ldx, pr, tot, alpha, beta, Yr, ldy, gpl);
+    else
+      hipLaunchKernelGGL((k_mv_apply<false>), grid, dim3(256), 0, st, R.m, rank, nrhs, Kr, ldk, Ur,
+                         ldu, Xr, ldx, pr, tot, alpha, beta, Yr, ldy, gpl);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_matvec(int64_t n, int leaf, int rank, int nrhs, const double* K, int64_t ldk,
+                         const double* U, int64_t ldu, const double* V, int64_t ldv,
+                         const double* X, int64_t ldx, double alpha, double beta, double* Y,
+                         int64_t ldy, double* work, hipStream_t st, bool trans) {
+  LeafRun run;
+  run.first = 0;
+  run.count = n / leaf;
+  run.m = leaf;
+  return launch_matvec_runs(&run, 1, run.count, rank, nrhs, K, ldk, U, ldu, V, ldv, X, ldx, alpha,
+                            beta, Y, ldy, work, st, trans);
 }
 
 // =====================================================================

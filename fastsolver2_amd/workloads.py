@@ -21,7 +21,47 @@ from . import (HodlrPlan, fortran_clone, fortran_empty, generate, lib, matvec, w
                _call, _ld, _ptr, _stream)
 
 __all__ = ["conditioned_shift", "condition_estimate", "conditioned_shift_global", "batch_systems",
-           "batch_operands"]
+           "batch_operands", "batch_leaf_sizes", "shard_leaf_sizes", "workload_uneven"]
+
+
+def shard_leaf_sizes(spmd_level, matrix_level, leaf, heavy, factor=10):
+    """Leaf sizes of the reference's irregular driver
+    (spmd_driver/solver_irregular_part.cc:73-75): 2^spmd_level shards, each a
+    balanced subtree of 2^matrix_level leaves; shards with index < heavy
+    have `factor` times the leaf size.  The tree has spmd_level +
+    matrix_level levels."""
+    out = []
+    for shard in range(1 << spmd_level):
+        m = leaf * factor if shard < heavy else leaf
+        out += [m] * (1 << matrix_level)
+    return out
+
+
+def workload_uneven(leaf_sizes, r, s, seed=0, system=0, shift=None, rhs="uniform"):
+    """Gaussian workload on a tree of uneven leaves: K (N x max m) holds leaf
+    i's block G_i + shift_i I in columns [0, m_i) of its rows (shift_i = m_i
+    unless given), U, V iid N(0,1)/sqrt(r), B iid U[0,1) (or N(0,1));
+    streams 0..3 of the device Philox generator as the balanced workload."""
+    sizes = [int(x) for x in leaf_sizes]
+    N, M = sum(sizes), max(sizes)
+    K = generate(seed, 0, system, 1, N, M, 1.0)
+    rows, cols, vals = [], [], []
+    row0 = 0
+    for m in sizes:
+        if m:
+            j = torch.arange(m, dtype=torch.int64)
+            rows.append(row0 + j)
+            cols.append(j)
+            vals.append(torch.full((m,), float(m if shift is None else shift), dtype=torch.float64))
+        row0 += m
+    ri = torch.cat(rows).to(K.device)
+    ci = torch.cat(cols).to(K.device)
+    K[ri, ci] += torch.cat(vals).to(K.device)
+    sc = 1.0 / float(np.sqrt(r))
+    U = generate(seed, 1, system, 1, N, r, sc)
+    V = generate(seed, 2, system, 1, N, r, sc)
+    B = generate(seed, 3, system, 0 if rhs == "uniform" else 1, N, s, 1.0)
+    return K, U, V, B
 
 
 def conditioned_shift(N, leaf, target_cond=1e4, seed=0, system=0, samples=2, iters=30):
@@ -140,9 +180,9 @@ def batch_systems(nsys=256, seed=0, n_range=(4096, 65536), leaves=(128, 256, 512
     """The uneven batch: per system N log-uniform in n_range rounded to a
     multiple of the leaf size, leaf size drawn from `leaves`, rank uniform in
     `ranks` (inclusive); numpy's Philox stream of `seed` draws them.  A
-    system of P leaves is padded to 2^ceil(log2 P) leaves (identity leaf
-    blocks, zero U, V and B rows) so that its tree is balanced; `n` is the
-    padded size, `n_real` the system's own."""
+    system of P leaves sits in a tree of 2^ceil(log2 P) leaf slots whose
+    last slots are empty (HODLR_PLAN_TRAILING_EMPTY): `n` = P * leaf rows,
+    no padding."""
     rng = np.random.Generator(np.random.Philox(seed))
     out = []
     for _ in range(nsys):
@@ -151,25 +191,23 @@ def batch_systems(nsys=256, seed=0, n_range=(4096, 65536), leaves=(128, 256, 512
         r = int(rng.integers(ranks[0], ranks[1] + 1))
         P = max(2, int(round(nraw / m)))
         L = int(np.ceil(np.log2(P)))
-        out.append(dict(n=m << L, n_real=P * m, leaf=m, levels=L, rank=r, nrhs_max=1))
+        out.append(dict(n=P * m, leaf=m, levels=L, rank=r, nrhs_max=1))
     return out
+
+
+def batch_leaf_sizes(sy):
+    """Leaf-slot sizes of a batch system (empty trailing slots as 0)."""
+    P = sy["n"] // sy["leaf"]
+    return [sy["leaf"]] * P + [0] * ((1 << sy["levels"]) - P)
 
 
 def batch_operands(systems, seed=0, pad=0):
     """Device operands of every system (generators of the Gaussian workload
     with the system index in the counter: K_i = G_i + m I, U, V iid
-    N(0,1)/sqrt(r), one right-hand side iid U[0,1)); padded rows carry
-    identity leaves and zero U, V, B."""
+    N(0,1)/sqrt(r), one right-hand side iid U[0,1))."""
     Ks, Us, Vs, Bs = [], [], [], []
     for k, sy in enumerate(systems):
-        n, m, r, nr = sy["n"], sy["leaf"], sy["rank"], sy["n_real"]
-        K, U, V, B = workload_gaussian(n, m, r, 1, seed=seed, system=k, pad=pad)
-        if nr < n:
-            K[nr:, :].zero_()
-            rows = torch.arange(nr, n, device=K.device)
-            K[rows, rows % m] = 1.0
-            U[nr:, :].zero_()
-            V[nr:, :].zero_()
-            B[nr:, :].zero_()
+        K, U, V, B = workload_gaussian(sy["n"], sy["leaf"], sy["rank"], 1, seed=seed, system=k,
+                                       pad=pad)
         Ks.append(K); Us.append(U); Vs.append(V); Bs.append(B)
     return Ks, Us, Vs, Bs

@@ -74,8 +74,13 @@ typedef struct hodlr_desc {
   int     nrhs_max;   /* largest number of right-hand sides solved at once   */
   int     precision;  /* enum hodlr_precision                                */
   int     device;     /* HIP device ordinal                                  */
-  int     flags;      /* reserved, must be 0                                 */
+  int     flags;      /* 0 or HODLR_PLAN_TRAILING_EMPTY                      */
 } hodlr_desc;
+
+/* hodlr_desc.flags: n = leaf * P with P <= 2^levels; the tree keeps 2^levels
+ * leaf slots and the last 2^levels - P are empty (no rows).  A system of P
+ * leaves, P not a power of two, then needs no padding rows. */
+#define HODLR_PLAN_TRAILING_EMPTY 1
 
 typedef struct hodlr_plan_s *hodlr_plan;
 
@@ -86,6 +91,18 @@ typedef struct hodlr_plan_s *hodlr_plan;
  * the K, U, V passed to hodlr_factor; they must stay allocated until the
  * plan is destroyed or factored again. */
 int hodlr_plan_create(hodlr_plan *plan, const hodlr_desc *desc);
+/* A tree of uneven leaves: 2^levels leaf slots of sizes leaf_sizes[i]
+ * (each 0 -- an empty slot -- or above the rank), desc->n their sum
+ * (desc->leaf and desc->flags are ignored / must be 0).  Leaf i covers rows
+ * [sum_{j<i} leaf_sizes[j], ... + leaf_sizes[i]), its block K_i sits in
+ * columns [0, leaf_sizes[i]) of those rows of K (N x max leaf), and the
+ * node of leaf range [a, b) has children [a, mid) and [mid, b) of the slot
+ * tree, however many rows they hold.  This is the shape of the reference's
+ * irregular drivers: shards of 2^k leaves each with their own leaf size
+ * (spmd_driver/solver_irregular_part.cc:73-75, heavy shards of 10x leaves),
+ * joined by the top levels.  Every entry point taking a plan accepts such
+ * a plan; the per-leaf launches run once per run of equal leaf sizes. */
+int hodlr_plan_create_uneven(hodlr_plan *plan, const hodlr_desc *desc, const int *leaf_sizes);
 /* Replaces HMatrix::destroy (include/hmatrix.hpp:23, declared but never
  * defined in the reference) / LMatrix::clear (src/lmatrix.cc:886-890). */
 int hodlr_plan_destroy(hodlr_plan plan);
@@ -231,6 +248,13 @@ int hodlr_matvec_noalloc(int64_t n, int leaf, int rank, int nrhs,
                          const double *V, int64_t ldv, const double *X, int64_t ldx,
                          double alpha, double beta, double *Y, int64_t ldy,
                          hipStream_t stream);
+/* hodlr_matvec for the tree of a plan (any leaf sizes, see
+ * hodlr_plan_create_uneven), on the given operands (not the plan's
+ * factors); trans != 0 applies A^T.  No reference counterpart. */
+int hodlr_plan_matvec(hodlr_plan plan, int trans, int nrhs, const double *K, int64_t ldk,
+                      const double *U, int64_t ldu, const double *V, int64_t ldv,
+                      const double *X, int64_t ldx, double alpha, double beta, double *Y,
+                      int64_t ldy, hipStream_t stream);
 /* The same with A^T: y_i = K_i^T x_i + V_i (U^T x - U_i^T x_i). */
 int hodlr_matvec_t(int64_t n, int leaf, int rank, int nrhs,
                    const double *K, int64_t ldk, const double *U, int64_t ldu,

@@ -274,6 +274,62 @@ def dense_assemble(K, U, V, leaf):
     return A
 
 
+# ---------------------------------------------------------- uneven trees
+# A tree of 2^L leaf slots of sizes leaf_sizes[i] (0 = empty slot): leaf i
+# covers rows [row0_i, row0_i + m_i), its block K_i is columns [0, m_i) of
+# those rows of K (N x max m), and every entry outside the leaf blocks is
+# U(i,:) V(j,:)^T (global U, V), as for balanced trees.  The reference
+# solves balanced trees only; the uneven tree is solved here by embedding it
+# in a balanced tree of 2^L leaves of size M = max m whose leaf i is
+# [K_i, 0; 0, I] with zero U, V and B rows below row m_i: that matrix is a
+# permutation of blockdiag(A, I), so its solution restricted to the real
+# rows is A^-1 B exactly.
+
+
+def uneven_rows(leaf_sizes):
+    return np.concatenate([[0], np.cumsum(leaf_sizes)]).astype(np.int64)
+
+
+def embed_uneven(K, U, V, B, leaf_sizes):
+    sizes = [int(x) for x in leaf_sizes]
+    M = max(sizes)
+    P = len(sizes)
+    row0 = uneven_rows(sizes)
+    r, s = U.shape[1], B.shape[1]
+    Kp = np.zeros((P * M, M), order="F")
+    Up = np.zeros((P * M, r), order="F")
+    Vp = np.zeros((P * M, r), order="F")
+    Bp = np.zeros((P * M, s), order="F")
+    idx = []
+    for i, m in enumerate(sizes):
+        a, b = i * M, i * M + m
+        Kp[a:b, :m] = K[row0[i]:row0[i] + m, :m]
+        for j in range(m, M):
+            Kp[a + j, j] = 1.0
+        Up[a:b] = U[row0[i]:row0[i] + m]
+        Vp[a:b] = V[row0[i]:row0[i] + m]
+        Bp[a:b] = B[row0[i]:row0[i] + m]
+        idx.append(np.arange(a, b))
+    return Kp, Up, Vp, Bp, M, np.concatenate(idx)
+
+
+def hodlr_solve_uneven(K, U, V, B, leaf_sizes, nthreads=1):
+    """The restated reference solve on the balanced embedding of an uneven
+    tree; returns the solution rows of the uneven system."""
+    Kp, Up, Vp, Bp, M, idx = embed_uneven(K, U, V, B, leaf_sizes)
+    xp = hodlr_solve(Kp, Up, Vp, Bp, M, nthreads=nthreads)
+    return np.asfortranarray(xp[idx])
+
+
+def dense_assemble_uneven(K, U, V, leaf_sizes):
+    A = U @ V.T
+    row0 = uneven_rows(leaf_sizes)
+    for i, m in enumerate(leaf_sizes):
+        a = row0[i]
+        A[a:a + m, a:a + m] = K[a:a + m, :m]
+    return np.asfortranarray(A)
+
+
 def gemm_red(A, B, nchunk, nslot, alpha=1.0):
     rows, ka = A.shape
     kb = B.shape[1]

@@ -1,7 +1,8 @@
 """Batched uneven systems under one HIP graph (gpu marker).
 
-Each system of the batch is checked against the oracle's solve of the same
-(padded) system; graph replays must reproduce the captured run and the
+Each system of the batch (P leaves in a tree of 2^ceil(log2 P) leaf slots,
+the trailing slots empty) is checked against the oracle's solve of the same
+arrays (oracle.hodlr_solve_uneven); graph replays must reproduce the captured run and the
 plain per-system plan bit for bit; the autotuner must return a valid launch
 shape whose batch solve is still exact.
 """
@@ -35,6 +36,11 @@ def _blas(orc):
     orc.use_netlib()
 
 
+def oracle_solve(orc, sy, Kh, Uh, Vh, Bh, nthreads=4):
+    from fastsolver2_amd.workloads import batch_leaf_sizes
+    return orc.hodlr_solve_uneven(Kh, Uh, Vh, Bh, batch_leaf_sizes(sy), nthreads=nthreads)
+
+
 def small_batch(F, nsys=10, seed=3):
     from fastsolver2_amd.workloads import batch_operands, batch_systems
     systems = batch_systems(nsys, seed=seed, n_range=(1024, 8192))
@@ -45,7 +51,8 @@ def test_batch_matches_oracle(gpu, orc):
     F = gpu
     systems, (Ks, Us, Vs, Bs) = small_batch(F)
     say("case: batch of", len(systems), "systems",
-        [(s["n_real"], s["leaf"], s["rank"]) for s in systems])
+        [(s["n"], s["leaf"], s["rank"]) for s in systems])
+    assert any(s["n"] < s["leaf"] << s["levels"] for s in systems)  # empty slots present
     host = [tuple(to_host(t) for t in ops) for ops in zip(Ks, Us, Vs, Bs)]
     b = F.HodlrBatch(systems)
     side = torch.cuda.Stream()
@@ -54,11 +61,10 @@ def test_batch_matches_oracle(gpu, orc):
         st = b.status()
     assert st[0] == 0, st
     for i, (sy, (Kh, Uh, Vh, Bh)) in enumerate(zip(systems, host)):
-        ref = orc.hodlr_solve(Kh, Uh, Vh, Bh, sy["leaf"], nthreads=4)
+        ref = oracle_solve(orc, sy, Kh, Uh, Vh, Bh)
         x = to_host(Bs[i])
         e = relerr(x, ref)
         assert e <= 1e-10, (i, e)
-        assert np.all(x[sy["n_real"]:] == 0.0)  # padding rows solve to zero
     b.close()
 
 
@@ -115,7 +121,7 @@ def test_batch_autotuner(gpu, orc):
         assert b.status()[0] == 0
     for i, sy in enumerate(systems):
         Kh, Uh, Vh, Bh = (to_host(t) for t in (K0[i], Us[i], Vs[i], B0[i]))
-        ref = orc.hodlr_solve(Kh, Uh, Vh, Bh, sy["leaf"], nthreads=4)
+        ref = oracle_solve(orc, sy, Kh, Uh, Vh, Bh)
         assert relerr(to_host(Bs[i]), ref) <= 1e-10, i
     b.close()
 
@@ -130,7 +136,7 @@ def test_config3_full_batch(gpu, orc):
     from fastsolver2_amd.workloads import batch_operands, batch_systems
     F = gpu
     systems = batch_systems(256, seed=0)
-    say(f"case: config-3 batch of {len(systems)} systems, sum N {sum(s['n_real'] for s in systems)}")
+    say(f"case: config-3 batch of {len(systems)} systems, sum N {sum(s['n'] for s in systems)}")
     Ks, Us, Vs, Bs = batch_operands(systems, seed=0)
     K0 = [F.fortran_clone(k) for k in Ks]
     B0 = [F.fortran_clone(x) for x in Bs]
@@ -149,9 +155,9 @@ def test_config3_full_batch(gpu, orc):
         worst = max(worst, res)
         assert res <= 1e-12, (i, res)
     say(f"  worst residual {worst:.2e}")
-    pick = {max(range(256), key=lambda i: systems[i]["n_real"]),
+    pick = {max(range(256), key=lambda i: systems[i]["n"]),
             max(range(256), key=lambda i: systems[i]["rank"]),
-            min(range(256), key=lambda i: systems[i]["n_real"])}
+            min(range(256), key=lambda i: systems[i]["n"])}
     for i in range(0, 256, 9):
         if len(pick) >= 32:
             break
@@ -159,8 +165,7 @@ def test_config3_full_batch(gpu, orc):
     for i in sorted(pick):
         sy = systems[i]
         Kh, Uh, Vh, Bh = (to_host(t) for t in (K0[i], Us[i], Vs[i], B0[i]))
-        ref = orc.hodlr_solve(Kh, Uh, Vh, Bh, sy["leaf"], task_level=min(sy["levels"], 4),
-                              nthreads=16)
+        ref = oracle_solve(orc, sy, Kh, Uh, Vh, Bh, nthreads=16)
         e = relerr(to_host(Bs[i]), ref)
         assert e <= 1e-10, (i, sy, e)
     say(f"  {len(pick)} systems match the oracle")

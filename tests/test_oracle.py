@@ -360,3 +360,29 @@ def test_openblas_backend_agrees(orc):
     finally:
         orc.use_netlib()
     assert np.abs(x1 - x2).max() <= 1e-12 * np.abs(x1).max(), name
+
+
+def test_uneven_embedding_matches_dense(orc):
+    """The balanced embedding of an uneven tree (hodlr_solve_uneven: every
+    leaf padded to the largest with an identity block and zero U, V, B rows)
+    solves the uneven system: against a dense solve of the assembled matrix,
+    including empty leaf slots; and the dense assembly of a uniform tree
+    equals the balanced one."""
+    rng = np.random.default_rng(17)
+    for sizes, r, s in (([40, 40, 400, 40], 6, 2), ([64, 0, 96, 32, 0, 0, 80, 48], 5, 3),
+                        ([50] * 5 + [0] * 3, 4, 1)):
+        N, M = sum(sizes), max(sizes)
+        K = np.asfortranarray(rng.standard_normal((N, M)))
+        row0 = orc.uneven_rows(sizes)
+        for i, m in enumerate(sizes):
+            K[row0[i] + np.arange(m), np.arange(m)] += m
+        U = np.asfortranarray(rng.standard_normal((N, r)) / np.sqrt(r))
+        V = np.asfortranarray(rng.standard_normal((N, r)) / np.sqrt(r))
+        B = np.asfortranarray(rng.random((N, s)))
+        A = orc.dense_assemble_uneven(K, U, V, sizes)
+        x = orc.hodlr_solve_uneven(K, U, V, B, sizes)
+        xd = np.linalg.solve(A, B)
+        assert np.abs(x - xd).max() <= 1e-12 * np.abs(xd).max(), sizes
+    K, U, V, B = _problem(orc, 512, 64, 6, 2, seed=19)
+    Au, Ab = orc.dense_assemble_uneven(K, U, V, [64] * 8), orc.dense_assemble(K, U, V, 64)
+    assert np.abs(Au - Ab).max() <= 1e-15 * np.abs(Ab).max()   # U V^T summation order only
