@@ -22,7 +22,7 @@ __all__ = [
     "lib", "lib_path", "available", "fortran_empty", "fortran", "HodlrPlan", "hsolve",
     "gemm_red", "gemm_bro", "node_solve", "node_solve_region", "matvec", "generate", "workload_gaussian",
     "dgesv_batched", "HodlrError", "grad", "HodlrSolveFunction", "hodlr_solve", "fortran_clone",
-    "FP64", "MIXED_IR", "PLAN_TRAILING_EMPTY", "IRStats", "HodlrBatch",
+    "FP64", "MIXED_IR", "PLAN_TRAILING_EMPTY", "IRStats", "HodlrBatch", "HodlrPair",
 ]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -81,6 +81,11 @@ def lib():
         "hodlr_plan_create_uneven": ([C.POINTER(vp), C.POINTER(_Desc), C.POINTER(ci)], ci),
         "hodlr_plan_matvec": ([vp, ci, ci, vp, i64, vp, i64, vp, i64, vp, i64, dbl, dbl, vp, i64,
                                vp], ci),
+        "hodlr_pair_create": ([C.POINTER(vp), C.POINTER(_Desc), vp, C.POINTER(_Desc), vp, ci], ci),
+        "hodlr_pair_destroy": ([vp], ci),
+        "hodlr_pair_plan": ([vp, ci], vp),
+        "hodlr_pair_factor_solve": ([vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, i64, ci, vp], ci),
+        "hodlr_pair_status": ([vp, vp, C.POINTER(ci), C.POINTER(ci), C.POINTER(ci)], ci),
         "hodlr_plan_destroy": ([vp], ci),
         "hodlr_plan_device_bytes": ([vp], i64),
         "hodlr_factor": ([vp, vp, i64, vp, i64, vp, i64, vp], ci),
@@ -334,6 +339,70 @@ class HodlrPlan:
         w, lv = C.c_int(-1), C.c_int(-1)
         st = lib().hodlr_plan_status(self._h, _stream(), C.byref(w), C.byref(lv))
         return (st, w.value, lv.value) if level else (st, w.value)
+
+
+class HodlrPair:
+    """Two sibling shards with different off-diagonal ranks (include/hodlr.h,
+    "pairs"; spmd_driver/solver_irregular_ranks.cc): A = [A_0, U_0 W_1^T;
+    U_1 W_0^T, A_1].  Each shard is given by its leaf-slot sizes
+    (2^levels entries) and rank."""
+
+    def __init__(self, leaf_sizes0, rank0, leaf_sizes1, rank1, nrhs_max=1, device=None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("HodlrPair needs a GPU")
+        dev = torch.cuda.current_device() if device is None else int(device)
+        descs, arrs = [], []
+        for sizes, r in ((leaf_sizes0, rank0), (leaf_sizes1, rank1)):
+            sizes = [int(x) for x in sizes]
+            L = int(round(math.log2(len(sizes))))
+            if 1 << L != len(sizes):
+                raise ValueError("a shard needs 2^levels leaf-slot sizes")
+            descs.append(_Desc(sum(sizes), max(sizes), L, r, 0, FP64, dev, 0))
+            arrs.append((C.c_int * len(sizes))(*sizes))
+        h = C.c_void_p()
+        _call(lib().hodlr_pair_create, C.byref(h), C.byref(descs[0]), C.cast(arrs[0], C.c_void_p),
+              C.byref(descs[1]), C.cast(arrs[1], C.c_void_p), int(nrhs_max),
+              where="hodlr_pair_create")
+        self._h = h
+        self.n = (descs[0].n, descs[1].n)
+        self.rank = (rank0, rank1)
+        self.leaf_sizes = ([int(x) for x in leaf_sizes0], [int(x) for x in leaf_sizes1])
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().hodlr_pair_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def factor_solve(self, K, U, V, W, B):
+        """K, U, V, W: pairs of device operands (shard 0, shard 1); B
+        ((n_0 + n_1) x s) is overwritten by the solution."""
+        ops = (K, U, V, W)
+        for pr, nm in zip(ops, "KUVW"):
+            for t in pr:
+                _check(t, nm)
+        _check(B, "B")
+        B2 = B if B.dim() == 2 else B.view(-1, 1)
+        ptrs = [(C.c_void_p * 2)(pr[0].data_ptr(), pr[1].data_ptr()) for pr in ops]
+        lds = [(C.c_int64 * 2)(_ld(pr[0]), _ld(pr[1])) for pr in ops]
+        args = []
+        for p_, l_ in zip(ptrs, lds):
+            args += [C.cast(p_, C.c_void_p), C.cast(l_, C.c_void_p)]
+        _call(lib().hodlr_pair_factor_solve, self._h, *args, _ptr(B2), _ld(B2), B2.shape[1],
+              _stream(), where="hodlr_pair_factor_solve")
+        self._keep = (K, U, V, W)
+        return B
+
+    def status(self):
+        """(status, shard, where, level) of the last factorization."""
+        sh, w, lv = C.c_int(-1), C.c_int(-1), C.c_int(-1)
+        st = lib().hodlr_pair_status(self._h, _stream(), C.byref(sh), C.byref(w), C.byref(lv))
+        return st, sh.value, w.value, lv.value
 
 
 class HodlrBatch:

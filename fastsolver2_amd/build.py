@@ -11,7 +11,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhodlr_amd.so")
-SOURCES = ["hodlr_kernels.hip", "hodlr_capi.cpp", "hodlr_batch.cpp", "hodlr_facade.cpp"]
+SOURCES = ["hodlr_kernels.hip", "hodlr_capi.cpp", "hodlr_batch.cpp", "hodlr_pair.cpp",
+           "hodlr_facade.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 

@@ -216,6 +216,39 @@ int hodlr_batch_tune(hodlr_batch batch, double *const *K, const int64_t *ldk,
                      double *const *B, const int64_t *ldb, const int *nrhs,
                      hipStream_t stream, double *best_ms);
 
+/* -------------------------------------------------------------- pairs */
+/* Two sibling shards with different off-diagonal ranks, the system of the
+ * reference's spmd_driver/solver_irregular_ranks.cc (rank0 != rank1):
+ *   A = [A_0, U_0 W_1^T; U_1 W_0^T, A_1]
+ * with A_c the HODLR matrix of shard c (its own leaves, levels and rank
+ * r_c; operands K_c, U_c, V_c as for a plan), U_c (n_c x r_c) the same U
+ * its own levels use, and W_c (n_c x r_{1-c}) its coupling V of the
+ * sibling's rank (the driver's VMatp, :113-117).  The root core is
+ * (r0+r1) x (r0+r1), solved as NodeSolveRegionTask (hodlr_node_solve_region).
+ * The shards are described as for hodlr_plan_create / _uneven
+ * (leaf_sizes_c NULL for a balanced shard); nrhs_max as for a plan.  No
+ * mixed precision.  Stream-ordered and not re-entrant, as a plan. */
+typedef struct hodlr_pair_s *hodlr_pair;
+int hodlr_pair_create(hodlr_pair *pair, const hodlr_desc *shard0, const int *leaf_sizes0,
+                      const hodlr_desc *shard1, const int *leaf_sizes1, int nrhs_max);
+int hodlr_pair_destroy(hodlr_pair pair);
+/* The plan of shard 0 or 1 (owned by the pair; it holds A_c's factors). */
+hodlr_plan hodlr_pair_plan(hodlr_pair pair, int shard);
+/* Factor and solve: K, U, V, W are two-element arrays (shard 0, shard 1) of
+ * device pointers with their leading dimensions; B ((n_0 + n_1) x nrhs, ld
+ * ldb, shard 0's rows first) is overwritten by the solution.  The driver's
+ * order (solver_irregular_ranks.cc:146-251): each shard solves [B_c | U_c],
+ * projects W_c^T [d_c | Y_c], the root core is solved, d_c -= Y_c eta_c. */
+int hodlr_pair_factor_solve(hodlr_pair pair, const double *const *K, const int64_t *ldk,
+                            const double *const *U, const int64_t *ldu,
+                            const double *const *V, const int64_t *ldv,
+                            const double *const *W, const int64_t *ldw,
+                            double *B, int64_t ldb, int nrhs, hipStream_t stream);
+/* Synchronizes; HODLR_OK, or the first failing shard's hodlr_plan_status
+ * (*shard = 0 or 1), or HODLR_SINGULAR_CORE for the root core (*shard = -1,
+ * *where = 0). */
+int hodlr_pair_status(hodlr_pair pair, hipStream_t stream, int *shard, int *where, int *level);
+
 /* Synchronizes `stream` and reports the numerical status of the last factor:
  * HODLR_OK, HODLR_SINGULAR_LEAF (*where = leaf index, *level = levels) or
  * HODLR_SINGULAR_CORE (*where = node index within tree level *level, the

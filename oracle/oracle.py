@@ -330,6 +330,43 @@ def dense_assemble_uneven(K, U, V, leaf_sizes):
     return np.asfortranarray(A)
 
 
+def pair_solve(K, U, V, W, B, leaf_sizes, nthreads=1):
+    """Two shards with ranks r0 != r1 (spmd_driver/solver_irregular_ranks.cc:
+    146-251): per shard c, [d_c | Y_c] = A_c^-1 [B_c | U_c] (the leaf task and
+    in-machine level loop, restated solve), T_c = W_c^T [d_c | Y_c]
+    (LMatrix::gemm, :181-182), the (r0+r1)^2 root core by NodeSolveRegionTask
+    (:218-220, node_solve_region), and x_c = d_c - Y_c eta_c
+    (LMatrix::gemm_inplace, :250-251).  K, U, V, W are pairs (shard 0,
+    shard 1), leaf_sizes the shards' leaf-slot sizes; returns x."""
+    s = B.shape[1]
+    n0 = U[0].shape[0]
+    rows = (slice(0, n0), slice(n0, n0 + U[1].shape[0]))
+    dY, T = [], []
+    for c in (0, 1):
+        Xc = np.asfortranarray(np.hstack([B[rows[c]], U[c]]))
+        Xc = hodlr_solve_uneven(K[c], U[c], V[c], Xc, leaf_sizes[c], nthreads=nthreads)
+        dY.append(Xc)
+        T.append(gemm_red(np.asfortranarray(W[c]), Xc, 1, 1))
+    VTd0, VTu0 = np.asfortranarray(T[0][:, :s]), np.asfortranarray(T[0][:, s:])
+    VTd1, VTu1 = np.asfortranarray(T[1][:, :s]), np.asfortranarray(T[1][:, s:])
+    info = node_solve_region(VTu0, VTu1, VTd0, VTd1)
+    if info:
+        raise RuntimeError(f"node_solve_region INFO={info}")
+    eta = (VTd1, VTd0)
+    out = []
+    for c in (0, 1):
+        d = np.asfortranarray(dY[c][:, :s])
+        Y = np.asfortranarray(dY[c][:, s:])
+        out.append(gemm_bro(Y, eta[c], d, 1, 1, alpha=-1.0))
+    return np.asfortranarray(np.vstack(out))
+
+
+def dense_assemble_pair(K, U, V, W, leaf_sizes):
+    A0 = dense_assemble_uneven(K[0], U[0], V[0], leaf_sizes[0])
+    A1 = dense_assemble_uneven(K[1], U[1], V[1], leaf_sizes[1])
+    return np.asfortranarray(np.block([[A0, U[0] @ W[1].T], [U[1] @ W[0].T, A1]]))
+
+
 def gemm_red(A, B, nchunk, nslot, alpha=1.0):
     rows, ka = A.shape
     kb = B.shape[1]

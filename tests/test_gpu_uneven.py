@@ -132,3 +132,43 @@ def test_uneven_shapes_rejected(gpu):
         F.HodlrPlan(0, 0, 1, 8, leaf_sizes=[0, 0])
     with pytest.raises(F.HodlrError):   # more leaves than slots
         F.HodlrPlan(9 * 64, 64, 3, 8)
+
+
+PAIRS = [
+    # (shard-0 sizes, rank0, shard-1 sizes, rank1, nrhs): solver_irregular_ranks.cc
+    ([96] * 8, 12, [96] * 8, 30, 2),                  # equal shards, rank0 != rank1
+    ([400, 40, 40, 40], 9, [64] * 4, 20, 3),           # a heavy leaf in shard 0
+    ([256] * 4, 64, [128, 128, 0, 0], 100, 1),         # empty slots in shard 1
+]
+
+
+@pytest.mark.parametrize("sz0,r0,sz1,r1,s", PAIRS)
+def test_pair_matches_oracle(gpu, orc, sz0, r0, sz1, r1, s):
+    """Two shards with ranks r0 != r1 (the reference's irregular-ranks
+    driver): the GPU pair solve against the oracle's restatement of that
+    driver (oracle.pair_solve, pinned by a dense solve) at 1e-10, and
+    against a dense solve of the assembled matrix."""
+    F = gpu
+    from fastsolver2_amd.workloads import workload_uneven
+    say(f"case: pair r0={r0} r1={r1} sizes={sz0}/{sz1} nrhs={s}")
+    K, U, V, W, Bs = [], [], [], [], []
+    for c, (sz, r, rp) in enumerate(((sz0, r0, r1), (sz1, r1, r0))):
+        Kc, Uc, Vc, Bc = workload_uneven(sz, r, s, seed=11, system=c)
+        K.append(Kc); U.append(Uc); V.append(Vc); Bs.append(Bc)
+        W.append(F.generate(11, 4, c, 1, sum(sz), rp, 1.0 / np.sqrt(rp)))
+    n0, n1 = sum(sz0), sum(sz1)
+    B = F.fortran_empty(n0 + n1, s)
+    B[:n0].copy_(Bs[0])
+    B[n0:].copy_(Bs[1])
+    host = [[to_host(t) for t in pr] for pr in (K, U, V, W)]
+    Bh = to_host(B)
+    pair = F.HodlrPair(sz0, r0, sz1, r1, nrhs_max=s)
+    X = F.fortran_clone(B)
+    pair.factor_solve(K, U, V, W, X)
+    assert pair.status()[0] == 0
+    x = to_host(X)
+    xo = orc.pair_solve(*host, Bh, (sz0, sz1), nthreads=4)
+    assert relerr(x, xo) <= RTOL, relerr(x, xo)
+    A = orc.dense_assemble_pair(*host, (sz0, sz1))
+    assert relerr(x, np.linalg.solve(A, Bh)) <= RTOL
+    pair.close()

@@ -386,3 +386,30 @@ def test_uneven_embedding_matches_dense(orc):
     K, U, V, B = _problem(orc, 512, 64, 6, 2, seed=19)
     Au, Ab = orc.dense_assemble_uneven(K, U, V, [64] * 8), orc.dense_assemble(K, U, V, 64)
     assert np.abs(Au - Ab).max() <= 1e-15 * np.abs(Ab).max()   # U V^T summation order only
+
+
+def test_pair_solve_matches_dense(orc):
+    """The restated two-shard driver with ranks r0 != r1
+    (solver_irregular_ranks.cc:146-251) solves A = [A_0, U_0 W_1^T;
+    U_1 W_0^T, A_1]: against a dense solve, shards balanced and uneven."""
+    rng = np.random.default_rng(23)
+    for sizes, ranks, s in ((([48] * 4, [48] * 4), (6, 11), 2),
+                            (([40, 400, 40, 40], [64, 0, 64, 64]), (9, 4), 3)):
+        K, U, V, W, Bs = [], [], [], [], []
+        for c in (0, 1):
+            sz, r = sizes[c], ranks[c]
+            N, M = sum(sz), max(sz)
+            Kc = np.asfortranarray(rng.standard_normal((N, M)))
+            row0 = orc.uneven_rows(sz)
+            for i, m in enumerate(sz):
+                Kc[row0[i] + np.arange(m), np.arange(m)] += m
+            K.append(Kc)
+            U.append(np.asfortranarray(rng.standard_normal((N, r)) / np.sqrt(r)))
+            V.append(np.asfortranarray(rng.standard_normal((N, r)) / np.sqrt(r)))
+            W.append(np.asfortranarray(rng.standard_normal((N, ranks[1 - c])) / np.sqrt(r)))
+            Bs.append(rng.random((N, s)))
+        B = np.asfortranarray(np.vstack(Bs))
+        x = orc.pair_solve(K, U, V, W, B, sizes)
+        A = orc.dense_assemble_pair(K, U, V, W, sizes)
+        xd = np.linalg.solve(A, B)
+        assert np.abs(x - xd).max() <= 1e-12 * np.abs(xd).max(), ranks
