@@ -15,6 +15,9 @@ Prints ONE JSON line on stdout; diagnostics go to stderr.
 from __future__ import annotations
 
 import argparse
+import csv
+import glob
+import hashlib
 import json
 import math
 import os
@@ -61,6 +64,60 @@ def flops_exec(N, m, r, s):
         f += 2 ** t * ((2.0 / 3.0) * (2 * r) ** 3 + 2.0 * (2 * r) ** 2 * nc)  # cores
         f += 2.0 * N * r * (nc if t > 0 else s)            # update
     return f
+
+
+KERNELS_SRC = os.path.join(ROOT, "fastsolver2_amd", "csrc", "hodlr_kernels.hip")
+TRAFFIC_JSON = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+
+
+def kernels_source_sha256():
+    with open(KERNELS_SRC, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+def pmc_mean_kib(d, counter, kernel_substr):
+    """Mean per-dispatch value of a rocprofv3 --pmc counter over the
+    dispatches of kernels whose name contains kernel_substr, from the
+    counter_collection CSVs under d (values summed per dispatch)."""
+    per = {}
+    for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(path, newline="") as f:
+            for row in csv.DictReader(f):
+                if row.get("Counter_Name") != counter or kernel_substr not in row.get("Kernel_Name", ""):
+                    continue
+                key = (path, row.get("Dispatch_Id"))
+                per[key] = per.get(key, 0.0) + float(row["Counter_Value"])
+    if not per:
+        raise SystemExit(f"no {counter} rows for '{kernel_substr}' under {d}")
+    return sum(per.values()) / len(per), len(per)
+
+
+def write_traffic(dirs, kernel_substr, workload, copy_bytes):
+    """profiles/pmc_traffic.json from four rocprofv3 --pmc runs: FETCH_SIZE
+    and WRITE_SIZE of the kernel, and of a plain device copy of copy_bytes
+    (the calibration: the copy reads and writes copy_bytes exactly).  The
+    JSON carries the SHA-256 of the kernel source it was measured on; the
+    bench reports the traffic only while the source is unchanged."""
+    fdir, wdir, cfdir, cwdir = dirs
+    fk, nf = pmc_mean_kib(fdir, "FETCH_SIZE", kernel_substr)
+    wk, nw = pmc_mean_kib(wdir, "WRITE_SIZE", kernel_substr)
+    cf, _ = pmc_mean_kib(cfdir, "FETCH_SIZE", "copyBuffer")
+    cw, _ = pmc_mean_kib(cwdir, "WRITE_SIZE", "copyBuffer")
+    fscale = copy_bytes / 1024.0 / cf
+    wscale = copy_bytes / 1024.0 / cw
+    rd, wr = fk * 1024.0 * fscale, wk * 1024.0 * wscale
+    N, m = workload["n"], workload["leaf"]
+    out = {"kernel": "getrf_leaf", "kernel_match": kernel_substr,
+           "workload": f"N={N},leaf={m},rank={workload['rank']},nrhs={workload['nrhs']}",
+           "source_sha256": kernels_source_sha256(), "dispatches": [nf, nw],
+           "fetch_kib_raw": fk, "write_kib_raw": wk, "copy_probe_bytes": copy_bytes,
+           "copy_fetch_kib": cf, "copy_write_kib": cw, "fetch_scale": fscale,
+           "write_scale": wscale, "hbm_read_bytes_per_launch": rd,
+           "hbm_write_bytes_per_launch": wr, "hbm_bytes_per_launch": rd + wr,
+           "algorithmic_bytes_per_launch": 16 * N * m}
+    with open(TRAFFIC_JSON, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out, indent=1))
 
 
 def host_threads():
@@ -211,6 +268,11 @@ def main():
     ap.add_argument("--config4", action=argparse.BooleanOptionalAction, default=True,
                     help="mixed precision at N=2^20, cond~1e4 (default on)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--write-traffic", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "COPYF_DIR",
+                                                         "COPYW_DIR"),
+                    help="write profiles/pmc_traffic.json from rocprofv3 --pmc output dirs "
+                         "(no GPU work)")
+    ap.add_argument("--traffic-kernel", default="k_getrf_ll<4, double>")
     ap.add_argument("--n", type=int, default=262144)
     ap.add_argument("--leaf", type=int, default=512)
     ap.add_argument("--rank", type=int, default=64)
@@ -222,6 +284,12 @@ def main():
               "built and measured on a single MI355X, multi-GPU sharding is design-only.",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.write_traffic:
+        write_traffic(args.write_traffic, args.traffic_kernel,
+                      {"n": args.n, "leaf": args.leaf, "rank": args.rank, "nrhs": args.nrhs},
+                      args.copy_probe or (1 << 30))
+        return
 
     import numpy as np
     import torch
@@ -350,12 +418,15 @@ def main():
     roof["avg_launch_us"] = round(avg_s * 1e6, 2)
     roof["algorithmic_flops_per_launch"] = fl_per
     roof["algorithmic_bytes_per_launch"] = by_per
+    # HBM traffic per launch from the calibrated --pmc passes, valid only
+    # for the kernel source it was measured on (null otherwise)
     traffic = None
-    tf_path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
-    if os.path.exists(tf_path):
+    if os.path.exists(TRAFFIC_JSON):
         try:
-            tj = json.load(open(tf_path))
-            if tj.get("kernel") == dom and tj.get("workload") == f"N={N},leaf={m},rank={r},nrhs={s}":
+            tj = json.load(open(TRAFFIC_JSON))
+            if (tj.get("kernel") == dom
+                    and tj.get("workload") == f"N={N},leaf={m},rank={r},nrhs={s}"
+                    and tj.get("source_sha256") == kernels_source_sha256()):
                 traffic = tj.get("hbm_bytes_per_launch")
         except Exception:
             traffic = None

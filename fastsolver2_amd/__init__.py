@@ -20,7 +20,8 @@ import torch
 
 __all__ = [
     "lib", "lib_path", "available", "fortran_empty", "fortran", "HodlrPlan", "hsolve",
-    "gemm_red", "gemm_bro", "node_solve", "node_solve_region", "matvec", "generate", "workload_gaussian",
+    "gemm_red", "gemm_bro", "node_solve", "node_solve_region", "matvec", "generate_ref",
+    "dense_blocks", "generate", "workload_gaussian",
     "dgesv_batched", "HodlrError", "grad", "HodlrSolveFunction", "hodlr_solve", "fortran_clone",
     "FP64", "MIXED_IR", "PLAN_TRAILING_EMPTY", "IRStats", "HodlrBatch", "HodlrPair",
 ]
@@ -125,6 +126,7 @@ def lib():
         "hodlr_generate": ([C.c_uint64, C.c_uint32, C.c_uint32, ci, i64, ci, dbl, vp, i64, vp],
                            ci),
         "hodlr_shift_leaf_diagonal": ([i64, ci, dbl, vp, i64, vp], ci),
+        "hodlr_generate_ref": ([C.c_long, ci, i64, ci, dbl, vp, i64, vp], ci),
         "hodlr_selftest_mfma": ([vp, vp], ci),
         "hodlr_selftest_mfma_f32": ([vp, vp], ci),
         "hodlr_dense_blocks": ([i64, ci, ci, vp, i64, vp, i64, vp, dbl, vp, i64, vp], ci),
@@ -690,6 +692,30 @@ def generate(seed, stream_id, system, kind, rows, cols, scale=1.0, out=None, dev
     _call(lib().hodlr_generate, seed, stream_id, system, kind, rows, cols, scale, _ptr(out),
           _ld(out), _stream(), where="hodlr_generate")
     return out
+
+
+def generate_ref(master, npart, rows_per_part, cols, offset=0.0, out=None, device="cuda"):
+    """The reference's drand48 generators on the device (hodlr_generate_ref):
+    Matrix::rand (cols > 1) / Vector::rand (cols = 1) with `master` in place
+    of time(NULL)."""
+    if out is None:
+        out = fortran_empty(npart * rows_per_part, cols, device=device)
+    _call(lib().hodlr_generate_ref, int(master), npart, rows_per_part, cols, float(offset),
+          _ptr(out), _ld(out), _stream(), where="hodlr_generate_ref")
+    return out
+
+
+def dense_blocks(U, V, D, leaf, offset=0.0, K=None):
+    """DenseBlockTask / LMatrix::init_dense_blocks: K_i = U_i V_i^T +
+    diag(D_i + offset) (hodlr_dense_blocks)."""
+    for t, nm in ((U, "U"), (V, "V"), (D, "D")):
+        _check(t, nm)
+    n, r = U.shape
+    if K is None:
+        K = fortran_empty(n, leaf, device=U.device)
+    _call(lib().hodlr_dense_blocks, n, leaf, r, _ptr(U), _ld(U), _ptr(V), _ld(V), _ptr(D),
+          float(offset), _ptr(K), _ld(K), _stream(), where="hodlr_dense_blocks")
+    return K
 
 
 def workload_gaussian(N, leaf, r, s, seed=0, system=0, shift=None, rhs="uniform",

@@ -1395,6 +1395,15 @@ int hodlr_generate(uint64_t seed, uint32_t stream_id, uint32_t system, int kind,
   return HODLR_OK;
 }
 
+int hodlr_generate_ref(long master, int npart, int64_t rows_per_part, int cols, double offset,
+                       double* out, int64_t ld, hipStream_t st) {
+  if (npart <= 0 || rows_per_part <= 0 || cols <= 0 || !out) return HODLR_ERR_ARG;
+  if (npart > 65535 || ld < (int64_t)npart * rows_per_part) return HODLR_ERR_ARG;
+  HIP_TRY(hodlr::launch_ref_fill((uint64_t)master, npart, rows_per_part, cols, offset, out, ld,
+                                 st));
+  return HODLR_OK;
+}
+
 int hodlr_selftest_mfma(double* out, hipStream_t st) {
   if (!out) return HODLR_ERR_ARG;
   HIP_TRY(hodlr::launch_selftest_mfma(out, st));

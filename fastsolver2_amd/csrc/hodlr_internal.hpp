@@ -194,6 +194,10 @@ hipError_t launch_leaf_outer(int64_t n, int leaf, int nc, int k, double alpha, c
 hipError_t launch_generate(uint64_t seed, uint32_t stream_id, uint32_t system, int kind,
                            int64_t rows, int cols, double scale, double* out, int64_t ld,
                            hipStream_t st);
+// Reference-mode generator (drand48 streams per partition from a master
+// seed), rows_per_part x cols per partition, stacked by rows.
+hipError_t launch_ref_fill(uint64_t master, int npart, int64_t rows, int cols, double offset,
+                           double* out, int64_t ld, hipStream_t st);
 hipError_t launch_shift_diag(int64_t n, int leaf, double shift, double* K, int64_t ldk,
                              hipStream_t st);
 hipError_t launch_dense_blocks(int64_t n, int leaf, int rank, const double* U, int64_t ldu,

@@ -3561,15 +3561,23 @@ __global__ void k_dense_blocks(int64_t n, int leaf, int rank, const double* __re
                                int64_t ldu, const double* __restrict__ V, int64_t ldv,
                                const double* __restrict__ D, double offset,
                                double* __restrict__ K, int64_t ldk) {
+#pragma clang fp contract(off)
   const int64_t total = n * (int64_t)leaf;
   for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
        e += (int64_t)gridDim.x * 256) {
     const int64_t row = e % n;
     const int col = (int)(e / n);
     const int64_t vrow = (row / leaf) * leaf + col;
+    // unfused multiply and add in k order: the products and sums of the
+    // reference's dgemm('N','T') (dgemm.f, alpha = 1, beta = 0), so K is
+    // the reference generator's bit for bit
+    // (the pragma at the top of the kernel keeps these operations unfused)
     double acc = 0.0;
-    for (int k = 0; k < rank; ++k) acc += U[row + (int64_t)k * ldu] * V[vrow + (int64_t)k * ldv];
-    if ((int)(row % leaf) == col) acc += D[row] + offset;
+    for (int k = 0; k < rank; ++k) {
+      const double prod = U[row + (int64_t)k * ldu] * V[vrow + (int64_t)k * ldv];
+      acc = acc + prod;
+    }
+    if ((int)(row % leaf) == col) acc = acc + (D[row] + offset);
     K[row + (int64_t)col * ldk] = acc;
   }
 }
@@ -3583,6 +3591,61 @@ hipError_t launch_dense_blocks(int64_t n, int leaf, int rank, const double* U, i
   if (g > 65536) g = 65536;
   hipLaunchKernelGGL(k_dense_blocks, dim3((unsigned)g), dim3(256), 0, st, n, leaf, rank, U, ldu,
                      V, ldv, D, offset, K, ldk);
+  return hipGetLastError();
+}
+
+// The reference generators (Matrix::rand / Vector::rand, matrix.cc:250-297
+// and :79-102, with PtrMatrix::rand, ptr_matrix.cc:62-71) on the device:
+// partition k's seed is the k-th lrand48 of srand48(master) (the master
+// standing for the reference's time(NULL)), and its rows_per_part x cols
+// block is that seed's drand48 stream visited rows outer, columns inner,
+// plus offset.  The POSIX LCG x <- (0x5DEECE66D x + 0xB) mod 2^48 is jumped
+// ahead in O(log n) by composing affine maps, so every thread starts at its
+// own element; drand48 = x * 2^-48 exactly, lrand48 = x >> 17.
+__device__ __forceinline__ uint64_t lcg48_jump(uint64_t x, uint64_t n) {
+  const uint64_t M = (1ULL << 48) - 1;
+  uint64_t A = 1, C = 0, a = 0x5DEECE66DULL, c = 0xBULL;
+  while (n) {
+    if (n & 1) {
+      A = (a * A) & M;
+      C = (a * C + c) & M;
+    }
+    c = ((a + 1) * c) & M;
+    a = (a * a) & M;
+    n >>= 1;
+  }
+  return (A * x + C) & M;
+}
+__device__ __forceinline__ uint64_t lcg48_seed(uint64_t seed) {
+  return ((seed & 0xffffffffULL) << 16) | 0x330EULL;
+}
+
+constexpr int kRefChunk = 16;  // consecutive stream elements per thread
+
+__global__ void k_ref_fill(uint64_t master, int64_t rows, int cols, double offset,
+                           double* __restrict__ out, int64_t ld) {
+  const int64_t part = blockIdx.y;
+  const uint64_t seed = lcg48_jump(lcg48_seed(master), (uint64_t)part + 1) >> 17;
+  const int64_t total = rows * (int64_t)cols;
+  const int64_t e0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * kRefChunk;
+  if (e0 >= total) return;
+  uint64_t x = lcg48_jump(lcg48_seed(seed), (uint64_t)e0);
+  double* blk = out + part * rows;
+  const int64_t e1 = e0 + kRefChunk < total ? e0 + kRefChunk : total;
+  for (int64_t e = e0; e < e1; ++e) {
+    x = (0x5DEECE66DULL * x + 0xBULL) & ((1ULL << 48) - 1);
+    const double v = (double)x * (1.0 / 281474976710656.0);
+    blk[e / cols + (e % cols) * ld] = v + offset;
+  }
+}
+
+hipError_t launch_ref_fill(uint64_t master, int npart, int64_t rows, int cols, double offset,
+                           double* out, int64_t ld, hipStream_t st) {
+  const int64_t total = rows * (int64_t)cols;
+  if (npart <= 0 || total <= 0) return hipSuccess;
+  const int64_t threads = (total + kRefChunk - 1) / kRefChunk;
+  hipLaunchKernelGGL(k_ref_fill, dim3((unsigned)((threads + 255) / 256), (unsigned)npart),
+                     dim3(256), 0, st, master, rows, cols, offset, out, ld);
   return hipGetLastError();
 }
 

@@ -387,9 +387,23 @@ int hodlr_dgesv_batched(int n, int nrhs, double *A, int64_t lda, int64_t bstride
 int hodlr_generate(uint64_t seed, uint32_t stream_id, uint32_t system, int kind,
                    int64_t rows, int cols, double scale, double *out, int64_t ld,
                    hipStream_t stream);
+/* The reference's generators on the device: Matrix::rand / Vector::rand
+ * (src/matrix.cc:250-297, :79-102) with PtrMatrix::rand (src/ptr_matrix.cc:
+ * 62-71).  Partition k's seed is the k-th lrand48_r of srand48_r(master)
+ * (master standing for the reference's time(NULL)); rows [k*rows_per_part,
+ * (k+1)*rows_per_part) of out (npart*rows_per_part x cols, ld) are that
+ * seed's drand48_r stream visited rows outer, columns inner, plus offset --
+ * bit for bit glibc's values.  With cols = 1 it is Vector::rand(offset).
+ * Together with hodlr_dense_blocks it builds the reference-mode K, U, V, D
+ * without host data (the reference's has_entry = false leaf generation,
+ * src/tasks/dense_block.cc:54-75). */
+int hodlr_generate_ref(long master, int npart, int64_t rows_per_part, int cols, double offset,
+                       double *out, int64_t ld, hipStream_t stream);
 /* DenseBlockTask::cpu_task + LMatrix::init_dense_blocks (src/tasks/
  * dense_block.cc:54-75, src/lmatrix.cc:273-297): K_i = U_i V_i^T +
- * diag(D_i + offset) for every leaf i of K (N x leaf); D has N entries. */
+ * diag(D_i + offset) for every leaf i of K (N x leaf); D has N entries.
+ * Products and sums unfused, in dgemm's order: the reference's K bit for
+ * bit. */
 int hodlr_dense_blocks(int64_t n, int leaf, int rank, const double *U, int64_t ldu,
                        const double *V, int64_t ldv, const double *D, double offset,
                        double *K, int64_t ldk, hipStream_t stream);

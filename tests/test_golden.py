@@ -66,3 +66,38 @@ def test_device_solve_matches_fixture(gpu, solutions, f):
     ref = solutions[f["name"]]
     assert np.abs(x - ref).max() <= 1e-10 * np.abs(ref).max()
     plan.close()
+
+
+REF_FIXTURES = [f for f in FIXTURES if f["mode"] == "reference"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("f", REF_FIXTURES, ids=[f["name"] for f in REF_FIXTURES])
+def test_device_reference_generator_reproduces_fixture(gpu, solutions, f):
+    """The reference-mode fixtures generated on the device from their seeds
+    alone (hodlr_generate_ref: drand48 streams, hodlr_dense_blocks:
+    K_i = U_i V_i^T + diag(D_i)): the same bytes as the fixture -- its
+    SHA-256 over (K, U, V, B) -- then solved to the fixture's solution."""
+    import torch
+    F = gpu
+    print(f"case: device reference generator {f['name']}", flush=True)
+    N, m, r, s, seed = f["n"], f["leaf"], f["rank"], f["nrhs"], f["seed"]
+    npart = N // m
+    U = F.generate_ref(seed, npart, m, r)
+    V = F.generate_ref(seed + 1, npart, m, r)
+    D = F.generate_ref(seed + 2, npart, m, 1, offset=10 * m)
+    B = F.generate_ref(seed + 3, npart, m, s)
+    K = F.dense_blocks(U, V, D, m)
+    torch.cuda.synchronize()
+    host = [np.asfortranarray(t.cpu().numpy()) for t in (K, U, V, B)]
+    want = MF.inputs("reference", N, m, r, s, seed)
+    for name, got, exp in zip("KUVB", host, want):
+        assert np.array_equal(got, exp), (name, float(np.abs(got - exp).max()))
+    assert MF.digest(*host) == f["input_sha256"]
+    plan = F.HodlrPlan(N, m, int(round(np.log2(npart))), r, nrhs_max=s)
+    plan.factor_solve(K, U, V, B)
+    assert plan.status()[0] == 0
+    torch.cuda.synchronize()
+    ref = solutions[f["name"]]
+    assert np.abs(B.cpu().numpy() - ref).max() <= 1e-10 * np.abs(ref).max()
+    plan.close()
