@@ -2014,50 +2014,101 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
   }
   __syncthreads();
   if (wid != 0) return;
-  // (3) LU of C with implicit pivoting: lane i keeps row i
-  double p[64];
-#pragma unroll
-  for (int c = 0; c < 64; ++c) p[c] = (lane < r && c < r) ? Cs[lane * 65 + c] : 0.0;
+  // (3) LU of C with implicit pivoting (lane i keeps row i of C; rows
+  // never move, pos is a row's LAPACK position), blocked like dgetrf: a
+  // 16-column panel is factored with rank-1 updates inside it (the pivot
+  // row's panel elements broadcast by readlane), then the panel's pivot
+  // rows form U12 = L11^-1 A12 (lane = trailing column) and the live rows
+  // take A22 -= L21 U12 (lane = row), both through Cs.  Per column only the
+  // panel's <= 16 elements cross lanes, where the unblocked form moved all
+  // 64.  Cs ends up holding every row of the LU.
   int pos = lane;
   bool live = lane < r;
-  int info = 0;  // lane 0
+  int info = 0;   // lane 0
   int mypiv = 0;  // lane jj: pivot position chosen at step jj
+  double* const crow_ = Cs + lane * 65;
+  // one panel's code, looped: the unrolled form ran to 150 KB of code
+#pragma unroll 1
+  for (int k0 = 0; k0 < 64; k0 += 16) {
+    if (k0 < r) {
+      double p[16];
 #pragma unroll
-  for (int jj = 0; jj < 64; ++jj) {
-    if (jj < r) {
-      const unsigned long long key = live ? Elem<double>::key(p[jj]) : 0ULL;
-      const int cpos = key ? pos : INT_MAX;
-      unsigned long long wmax;
-      const int wl = wave_argmax_key(key, Elem<double>::key32(key), cpos, wmax);
-      const int bpos = __builtin_amdgcn_readlane(pos, wl);
-      if (lane == jj) mypiv = bpos;
-      // the pivot row is broadcast lane-to-SGPR (readlane), no LDS
-      const double pv = readlane_f64(p[jj], wl);
-      if (lane == wl) {
-        live = false;
-        pos = jj;
-      } else if (pos == jj) {
-        pos = bpos;
-      }
-      if (pv != 0.0) {
-        double l;
-        if (fabs(pv) >= DBL_MIN)
-          l = p[jj] * (1.0 / pv);
-        else
-          l = p[jj] / pv;
-        // retired rows (the winner among them) are left out by the exec
-        // mask; readlane still reads the winner's inactive lane
-        if (live) {
-          p[jj] = l;
+      for (int c = 0; c < 16; ++c) p[c] = (lane < r && k0 + c < r) ? crow_[k0 + c] : 0.0;
+      int pl[16];  // pivot lane of each panel step (wave-uniform)
 #pragma unroll
-          for (int c = 0; c < 64; ++c)
-            if (c > jj) p[c] -= l * readlane_f64(p[c], wl);
+      for (int jj = 0; jj < 16; ++jj) {
+        pl[jj] = 0;
+        const int j = k0 + jj;
+        if (j < r) {
+          const unsigned long long key = live ? Elem<double>::key(p[jj]) : 0ULL;
+          const int cpos = key ? pos : INT_MAX;
+          unsigned long long wmax;
+          const int wl = wave_argmax_key(key, Elem<double>::key32(key), cpos, wmax);
+          pl[jj] = wl;
+          const int bpos = __builtin_amdgcn_readlane(pos, wl);
+          if (lane == j) mypiv = bpos;
+          const double pv = readlane_f64(p[jj], wl);
+          if (lane == wl) {
+            live = false;
+            pos = j;
+          } else if (pos == j) {
+            pos = bpos;
+          }
+          if (pv != 0.0) {
+            double l;
+            if (fabs(pv) >= DBL_MIN)
+              l = p[jj] * (1.0 / pv);
+            else
+              l = p[jj] / pv;
+            if (live) {
+              p[jj] = l;
+#pragma unroll
+              for (int c = 0; c < 16; ++c)
+                if (c > jj) p[c] -= l * readlane_f64(p[c], wl);
+            }
+          } else if (lane == 0 && info == 0) {
+            info = r + j + 1;
+          }
         }
-      } else if (lane == 0 && info == 0) {
-        info = r + jj + 1;
+      }
+      if (lane < r) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c)
+          if (k0 + c < r) crow_[k0 + c] = p[c];
+      }
+      if (k0 + 16 < r) {
+        wave_sync();
+        // U12: lane t takes trailing column k0 + 16 + t through the
+        // panel's pivot rows, in pivot order
+        const int c = k0 + 16 + lane;
+        if (c < r) {
+          double u[16];
+#pragma unroll
+          for (int k = 0; k < 16; ++k) {
+            double* prow = Cs + pl[k] * 65;
+            double v = prow[c];
+#pragma unroll
+            for (int m = 0; m < 16; ++m)
+              if (m < k) v -= prow[k0 + m] * u[m];
+            u[k] = v;
+            prow[c] = v;
+          }
+        }
+        wave_sync();
+        // A22 -= L21 U12 on the live rows
+        if (live) {
+          for (int cc = k0 + 16; cc < r; ++cc) {
+            double a = crow_[cc];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) a -= p[k] * Cs[pl[k] * 65 + cc];
+            crow_[cc] = a;
+          }
+        }
+        wave_sync();
       }
     }
   }
+  wave_sync();
   // (4) S's LU in LAPACK layout: rows r + pos get this lane's L2\U2 row and
   // its B row (the interchanges apply to the first r columns too)
   if (lane < r) {
@@ -2066,7 +2117,7 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
     for (int c = 0; c < 64; ++c) {
       if (c < r) {
         row[(int64_t)c * lda] = Bs[lane * 65 + c];
-        row[(int64_t)(r + c) * lda] = p[c];
+        row[(int64_t)(r + c) * lda] = crow_[c];
       }
     }
   }

@@ -337,6 +337,53 @@ def test_core_structured_and_general_lu(gpu, orc, scale):
     assert e <= RTOL
     plan.close()
 
+@pytest.mark.parametrize("r", [16, 30, 64])
+def test_core_schur_complement_pivots(gpu, orc, r):
+    """A root core whose structured LU pivots inside the Schur complement:
+    identity leaves (Y = U), V_c = [I; 0], U_0 = [B; 0] with |B| <= 1 (the
+    structured path) and U_1 = [A; 0] with large A, so S = [I, A; B, I] and
+    C = I - B A needs row interchanges in every panel.  The solution must
+    match the oracle (restated dgetrf pivoting) and a dense solve to the
+    core's condition number times rounding."""
+    F = gpu
+    m = 160
+    N = 2 * m
+    say(f"case: Schur-complement pivoting r={r}")
+    rng = np.random.default_rng(300 + r)
+    Bm = rng.uniform(-1.0, 1.0, (r, r))
+    Am = 5.0 * rng.standard_normal((r, r))
+    K = np.zeros((N, m))
+    K[:m] = np.eye(m)
+    K[m:] = np.eye(m)
+    U = np.zeros((N, r))
+    V = np.zeros((N, r))
+    U[:r] = Bm
+    U[m:m + r] = Am
+    V[:r] = np.eye(r)
+    V[m:m + r] = np.eye(r)
+    b = rng.standard_normal((N, 3))
+    K, U, V, b = (np.asfortranarray(a) for a in (K, U, V, b))
+    S = np.block([[np.eye(r), Am], [Bm, np.eye(r)]])
+    tol = max(1e-13, 4e-16 * np.linalg.cond(S) * r)
+    xo = orc.hodlr_solve(K, U, V, b, m)
+    A = orc.dense_assemble(K, U, V, m)
+    xd = np.linalg.solve(A, b)
+    assert relerr(xo, xd) <= tol
+    Kd, Ud, Vd, Bd = to_dev(F, K), to_dev(F, U), to_dev(F, V), to_dev(F, b)
+    plan = F.HodlrPlan(N, m, 1, r, nrhs_max=3)
+    plan.factor_solve(Kd, Ud, Vd, Bd)
+    assert plan.status()[0] == 0
+    x = to_host(Bd)
+    assert relerr(x, xo) <= tol, (relerr(x, xo), tol)
+    assert relerr(x, xd) <= tol
+    # factor reuse: the stored core factorization solves a new right-hand side
+    b2 = np.asfortranarray(rng.standard_normal((N, 2)))
+    B2 = to_dev(F, b2)
+    plan.solve(B2)
+    assert relerr(to_host(B2), np.linalg.solve(A, b2)) <= tol
+    plan.close()
+
+
 def test_factor_then_solve_many_rhs(gpu, orc):
     """hodlr_factor + hodlr_solve with more columns than nrhs_max (column
     batches), against the oracle."""
