@@ -180,8 +180,10 @@ int hodlr_grad(int64_t n, int leaf, int rank, int nrhs, const double *U, int64_t
 /* ------------------------------------------------------------- batches */
 /* Many independent HODLR systems (uneven N, leaf size, rank, nrhs), one
  * plan each.  hodlr_batch_factor_solve enqueues every system's fused factor
- * + solve (hodlr_factor_solve semantics, per system i: K[i] overwritten by
- * its leaf LU, B[i] by the solution).  On a capturable stream (not the
+ * + solve (hodlr_factor_solve semantics, per system i: K[i] only read, its
+ * leaf LU kept in system i's plan, B[i] overwritten by the solution; a desc
+ * may carry HODLR_PLAN_TRAILING_EMPTY for P leaves with P not a power of
+ * two).  On a capturable stream (not the
  * legacy null stream) the whole batch is captured once as ONE HIP graph --
  * the stream forks into `lanes` internal streams that run systems
  * concurrently and joins back -- and calls with the same operand pointers
