@@ -169,6 +169,7 @@ struct CoreBufs {
   int* info;
   double* dinv;
   int* general = nullptr;  // per-core flags of the structured LU (null: general LU)
+  int* tmask = nullptr;    // cores whose diagonal inverses are still to be formed
 };
 
 // Core step shared by every level: S (already assembled, n2 x n2 per node)
@@ -181,14 +182,18 @@ int core_factor_solve_n(int nnode, int n2, const CoreBufs& c, int nh, hipStream_
     ProfScope ps(HODLR_PROF_GETRF_CORE, st, nnode * lu_flops(n2), 2.0 * blk);
     if (c.general)
       HIP_TRY(hodlr::launch_getrf_core(n2, c.S, n2, (int64_t)n2 * n2, nnode, c.ipiv, c.perm, c.info,
-                                       c.general, st));
+                                       c.general, c.dinv, c.tmask, st));
     else
       HIP_TRY(hodlr::launch_getrf(n2, c.S, n2, (int64_t)n2 * n2, nnode, c.ipiv, c.perm, c.info,
                                   st));
   }
   {
     ProfScope ps(HODLR_PROF_TRTRI, st, 0.0, blk);
-    HIP_TRY(hodlr::launch_trtri_diag(n2, c.S, n2, (int64_t)n2 * n2, nnode, c.dinv, st));
+    if (c.general)  // only the cores the structured LU left without inverses
+      HIP_TRY(hodlr::launch_trtri_diag_masked(n2, c.S, n2, (int64_t)n2 * n2, nnode, c.dinv, c.tmask,
+                                              st));
+    else
+      HIP_TRY(hodlr::launch_trtri_diag(n2, c.S, n2, (int64_t)n2 * n2, nnode, c.dinv, st));
   }
   if (nh > 0) {
     ProfScope ps(HODLR_PROF_GETRS_CORE, st, 2.0 * n2 * n2 * nh * nnode,
@@ -244,6 +249,7 @@ struct hodlr_plan_s {
   int* core_info = nullptr;
   double* core_dinv = nullptr;
   int* core_general = nullptr;  // flags of the structured core LU (largest level)
+  int* core_tmask = nullptr;    // cores left without diagonal inverses (largest level)
   std::vector<int64_t> s_off, p_off, d_off, i_off;
   // workspace: per-leaf projections (two, ping-pong), accumulators, core RHS
   double* Pb[3] = {nullptr, nullptr, nullptr};
@@ -362,6 +368,7 @@ CoreBufs core_level(hodlr_plan p, int t) {
   c.info = p->core_info + p->i_off[t];
   c.dinv = p->core_dinv + p->d_off[t];
   c.general = p->core_general;
+  c.tmask = p->core_tmask;
   return c;
 }
 
@@ -663,6 +670,7 @@ static int plan_create_sizes(hodlr_plan* plan, const hodlr_desc& d, const std::v
   rc = rc ? rc : palloc(p, &p->core_info, io > 0 ? io : 1);
   rc = rc ? rc : palloc(p, &p->core_dinv, dof);
   rc = rc ? rc : palloc(p, &p->core_general, p->L > 0 ? ((int64_t)1 << (p->L - 1)) : 0);
+  rc = rc ? rc : palloc(p, &p->core_tmask, p->L > 0 ? ((int64_t)1 << (p->L - 1)) : 0);
   // per-leaf projections / accumulators and core RHS for the widest
   // operand: [Y | B] with nrhs_max columns, or a solve chunk of that width
   p->ncmax = r + (p->smax > 0 ? p->smax : 1);

@@ -77,8 +77,16 @@ hipError_t launch_add_cols(int64_t rows, int cols, const double* X, int64_t ldx,
 // LU of HODLR cores S = [I, A; B, I] (n2 = 2r): the block structure when
 // max|B| <= 1, else the general kernel (per core; `general` is batch ints of
 // workspace).  Same factors, pivots and INFO as launch_getrf.
+// Cores whose Schur complement keeps every diagonal pivot are factored
+// without interchanges and, when r is a multiple of 32, get their 32x32
+// diagonal inverses (dinv, dinv_elems(n2) per core) written here; tmask[b]
+// is set for every core whose inverses still have to be formed.
 hipError_t launch_getrf_core(int n2, double* S, int64_t lda, int64_t bstride, int batch, int* ipiv,
-                             int* perm, int* info, int* general, hipStream_t st);
+                             int* perm, int* info, int* general, double* dinv, int* tmask,
+                             hipStream_t st);
+// launch_trtri_diag for the items b with only[b] != 0.
+hipError_t launch_trtri_diag_masked(int n, const double* LU, int64_t lda, int64_t bstride,
+                                    int batch, double* dinv, const int* only, hipStream_t st);
 hipError_t launch_trtri_diag(int n, const double* LU, int64_t lda, int64_t bstride,
                              int batch, double* dinv, hipStream_t st);
 // X_b <- U^-1 L^-1 P X_b for batch item b, columns [0, ncols) of X, where

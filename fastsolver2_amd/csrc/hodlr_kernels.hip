@@ -1306,11 +1306,14 @@ __device__ __forceinline__ void ll_diag_step(T (*Ds)[33], T (*As12)[33], T (*Li)
   }
   wave_sync();
   // the stored inverses, column-major, 512 consecutive bytes per store
+  // (none when out is null)
+  if (out) {
 #pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int idx = lane + 64 * q;
-    out[idx] = Li[idx % 32][idx / 32];
-    out[1024 + idx] = Ui[idx % 32][idx / 32];
+    for (int q = 0; q < 16; ++q) {
+      const int idx = lane + 64 * q;
+      out[idx] = Li[idx % 32][idx / 32];
+      out[1024 + idx] = Ui[idx % 32][idx / 32];
+    }
   }
   if (with_a12) {
     // U12 of the slab = L11^-1 A12: D(i = row, j = col), A(i, k) =
@@ -1934,6 +1937,11 @@ hipError_t launch_getrf(int n, double* A, int64_t lda, int64_t bstride, int batc
   return hipGetLastError();
 }
 
+__global__ void k_fill_int(int* __restrict__ a, int n, int v) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
 // =====================================================================
 // LU of the HODLR cores S = [I, A; B, I] (2r x 2r, r <= 64), one workgroup
 // of four waves each.  With max|B| <= 1 partial pivoting takes the diagonal
@@ -1950,10 +1958,13 @@ hipError_t launch_getrf(int n, double* A, int64_t lda, int64_t bstride, int batc
 __global__ void __launch_bounds__(256)
 k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
              int* __restrict__ ipiv0, int* __restrict__ perm0, int* __restrict__ info0,
-             int* __restrict__ general) {
+             int* __restrict__ general, double* __restrict__ dinv0, int* __restrict__ tmask) {
   __shared__ double Cs[64 * 65];  // C = I - B A, [row*65 + col]
   __shared__ double Bs[64 * 65];  // B, [row*65 + k]
-  __shared__ double As[64 * 65];  // A, [k*65 + col]
+  __shared__ double As[64 * 65];  // A, [k*65 + col]; then the fast path's packed LU
+  // the no-interchange attempt's 32x32 tiles (ll_diag_step)
+  __shared__ double Dt[32][33], A12t[32][33], Lit[32][33], Uit[32][33], Mst[2][32][33];
+  __shared__ __align__(16) double Prow[32];
   __shared__ unsigned long long wmk[4];
   const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
   const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -1983,7 +1994,10 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
   __syncthreads();
   mk = umax64(umax64(wmk[0], wmk[1]), umax64(wmk[2], wmk[3]));
   const bool fallback = mk > (unsigned long long)__double_as_longlong(1.0);
-  if (tid == 0) general[b] = fallback ? 1 : 0;
+  if (tid == 0) {
+    general[b] = fallback ? 1 : 0;
+    tmask[b] = 1;  // cleared below when this kernel writes the inverses
+  }
   if (fallback) return;  // uniform over the workgroup
   // (2) C = I - B A on MFMA from LDS: wave w forms 16-row block w of C,
   // its four column tiles side by side (independent MFMA chains)
@@ -2014,6 +2028,117 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
   }
   __syncthreads();
   if (wid != 0) return;
+  // (3a) No interchanges when partial pivoting keeps every diagonal: C,
+  // padded with the identity to rp = 32 or 64 rows, factored in 32x32
+  // blocks by the leaf LU's diagonal step (every multiplier checked
+  // against 1 - 2^-20, so an accepted factorization is the one dgetrf
+  // computes with P2 = I), L21 = C21 U11^-1 (checked) and
+  // A22 = C22 - L21 U12 on MFMA.  The packed LU goes to As (A is no longer
+  // needed: S keeps it); Cs is left intact for the pivoted path.  With r a
+  // multiple of 32 the blocks of S's diagonal inverses are the identity
+  // (rows < r) and the diagonal steps' L^-1 / U^-1, written here.
+  {
+    const int rp = r <= 32 ? 32 : 64;
+    const bool inv = (r & 31) == 0;
+    double* dv = dinv0 + (int64_t)b * dinv_elems(n2);
+    const double thr = PivotMargin<double>::thr();
+    bool fail = false;
+    double (*const LUp)[65] = reinterpret_cast<double (*)[65]>(As);
+    auto cpad = [&](int i, int j) -> double {
+      return (i < r && j < r) ? Cs[i * 65 + j] : (i == j ? 1.0 : 0.0);
+    };
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int idx = lane + 64 * q, i = idx >> 5, j = idx & 31;
+      Dt[i][j] = cpad(i, j);
+      if (rp == 64) A12t[i][j] = cpad(i, 32 + j);
+    }
+    wave_sync();
+    ll_diag_step<double>(Dt, A12t, Lit, Uit, Mst, Prow, inv ? dv + (int64_t)(r / 32) * 2 * kTri * kTri : nullptr,
+                         rp == 64, thr, fail, lane);
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int idx = lane + 64 * q, i = idx >> 5, j = idx & 31;
+      LUp[i][j] = Dt[i][j];
+      if (rp == 64) LUp[i][32 + j] = A12t[i][j];
+    }
+    if (rp == 64) {
+      const int lr = lane & 15, lk = lane >> 4;
+      // L21 = C21 U11^-1 (Uit holds U11^-1), checked
+#pragma unroll
+      for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+          d4 d = d4zero();
+#pragma unroll
+          for (int s2 = 0; s2 < 8; ++s2)
+            d = mfma(cpad(32 + 16 * rt + lr, 4 * s2 + lk), Uit[4 * s2 + lk][16 * ct + lr], d);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+            fail |= !(fabs(d[v]) <= thr);
+            LUp[32 + 16 * rt + Elem<double>::crow(lane, v)][16 * ct + lr] = d[v];
+          }
+        }
+      wave_sync();
+      // A22 = C22 - L21 U12 into the diagonal tile
+#pragma unroll
+      for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 2; ++ct) {
+          d4 c;
+#pragma unroll
+          for (int v = 0; v < 4; ++v)
+            c[v] = cpad(32 + 16 * rt + Elem<double>::crow(lane, v), 32 + 16 * ct + lr);
+#pragma unroll
+          for (int s2 = 0; s2 < 8; ++s2)
+            c = mfma(-LUp[32 + 16 * rt + lr][4 * s2 + lk], A12t[4 * s2 + lk][16 * ct + lr], c);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) Dt[16 * rt + Elem<double>::crow(lane, v)][16 * ct + lr] = c[v];
+        }
+      wave_sync();
+      ll_diag_step<double>(Dt, A12t, Lit, Uit, Mst, Prow,
+                           inv ? dv + (int64_t)(r / 32 + 1) * 2 * kTri * kTri : nullptr, false,
+                           thr, fail, lane);
+      wave_sync();
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int idx = lane + 64 * q, i = idx >> 5, j = idx & 31;
+        LUp[32 + i][32 + j] = Dt[i][j];
+      }
+    }
+    wave_sync();
+    if (__ballot(fail) == 0ULL) {
+      // S's LU: rows 0..r-1 ([I | A]) and the B block stay as they are;
+      // rows r..2r-1, columns r..2r-1 take L2\U2
+      if (lane < r) {
+        double* row = S + (int64_t)(r + lane);
+        for (int c = 0; c < r; ++c) row[(int64_t)(r + c) * lda] = LUp[lane][c];
+      }
+      int* ipiv = ipiv0 + (int64_t)b * n2;
+      int* perm = perm0 + (int64_t)b * n2;
+      for (int i = lane; i < n2; i += 64) {
+        ipiv[i] = i + 1;
+        perm[i] = i;
+      }
+      if (inv) {
+        // identity inverses of the blocks of rows < r
+        for (int q = 0; q < r / 32; ++q)
+#pragma unroll
+          for (int e = 0; e < 16; ++e) {
+            const int idx = lane + 64 * e;
+            const double one = ((idx & 31) == (idx >> 5)) ? 1.0 : 0.0;
+            dv[(int64_t)q * 2 * kTri * kTri + idx] = one;
+            dv[(int64_t)q * 2 * kTri * kTri + 1024 + idx] = one;
+          }
+      }
+      if (lane == 0) {
+        info0[b] = 0;
+        if (inv) tmask[b] = 0;
+      }
+      return;
+    }
+  }
   // (3) LU of C with implicit pivoting (lane i keeps row i of C; rows
   // never move, pos is a row's LAPACK position), blocked like dgetrf: a
   // 16-column panel is factored with rank-1 updates inside it (the pivot
@@ -2133,11 +2258,16 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
 }
 
 hipError_t launch_getrf_core(int n2, double* S, int64_t lda, int64_t bstride, int batch, int* ipiv,
-                             int* perm, int* info, int* general, hipStream_t st) {
+                             int* perm, int* info, int* general, double* dinv, int* tmask,
+                             hipStream_t st) {
   if (batch <= 0 || n2 <= 0) return hipSuccess;
-  if (n2 > 128 || (n2 & 1)) return launch_getrf(n2, S, lda, bstride, batch, ipiv, perm, info, st);
+  if (n2 > 128 || (n2 & 1)) {
+    // every core takes the general LU; all need their inverses formed
+    hipLaunchKernelGGL(k_fill_int, dim3((batch + 255) / 256), dim3(256), 0, st, tmask, batch, 1);
+    return launch_getrf(n2, S, lda, bstride, batch, ipiv, perm, info, st);
+  }
   hipLaunchKernelGGL(k_getrf_core, dim3(batch), dim3(256), 0, st, S, lda, bstride, n2, ipiv, perm,
-                     info, general);
+                     info, general, dinv, tmask);
   hipLaunchKernelGGL(k_getrf_w64<32>, dim3(batch), dim3(64), 0, st, S, lda, bstride, n2, ipiv, perm,
                      info, (const int*)general);
   return hipGetLastError();
@@ -2228,6 +2358,14 @@ hipError_t launch_trtri_diag(int n, const double* LU, int64_t lda, int64_t bstri
   if (batch <= 0 || n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_trtri_diag<double>, dim3(batch * ntri(n)), dim3(64), 0, st, LU, lda,
                      bstride, n, dinv);
+  return hipGetLastError();
+}
+
+hipError_t launch_trtri_diag_masked(int n, const double* LU, int64_t lda, int64_t bstride,
+                                    int batch, double* dinv, const int* only, hipStream_t st) {
+  if (batch <= 0 || n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_trtri_diag<double>, dim3(batch * ntri(n)), dim3(64), 0, st, LU, lda,
+                     bstride, n, dinv, only);
   return hipGetLastError();
 }
 
