@@ -170,6 +170,7 @@ struct CoreBufs {
   double* dinv;
   int* general = nullptr;  // per-core flags of the structured LU (null: general LU)
   int* tmask = nullptr;    // cores whose diagonal inverses are still to be formed
+  const hodlr::CoreSchur* schur = nullptr;  // scratch for ranks 64 < r <= 128
 };
 
 // Core step shared by every level: S (already assembled, n2 x n2 per node)
@@ -182,7 +183,7 @@ int core_factor_solve_n(int nnode, int n2, const CoreBufs& c, int nh, hipStream_
     ProfScope ps(HODLR_PROF_GETRF_CORE, st, nnode * lu_flops(n2), 2.0 * blk);
     if (c.general)
       HIP_TRY(hodlr::launch_getrf_core(n2, c.S, n2, (int64_t)n2 * n2, nnode, c.ipiv, c.perm, c.info,
-                                       c.general, c.dinv, c.tmask, st));
+                                       c.general, c.dinv, c.tmask, c.schur, st));
     else
       HIP_TRY(hodlr::launch_getrf(n2, c.S, n2, (int64_t)n2 * n2, nnode, c.ipiv, c.perm, c.info,
                                   st));
@@ -250,6 +251,7 @@ struct hodlr_plan_s {
   double* core_dinv = nullptr;
   int* core_general = nullptr;  // flags of the structured core LU (largest level)
   int* core_tmask = nullptr;    // cores left without diagonal inverses (largest level)
+  hodlr::CoreSchur schur;       // Schur-complement scratch for ranks 64 < r <= 128
   std::vector<int64_t> s_off, p_off, d_off, i_off;
   // workspace: per-leaf projections (two, ping-pong), accumulators, core RHS
   double* Pb[3] = {nullptr, nullptr, nullptr};
@@ -369,6 +371,7 @@ CoreBufs core_level(hodlr_plan p, int t) {
   c.dinv = p->core_dinv + p->d_off[t];
   c.general = p->core_general;
   c.tmask = p->core_tmask;
+  c.schur = p->schur.C ? &p->schur : nullptr;
   return c;
 }
 
@@ -671,6 +674,16 @@ static int plan_create_sizes(hodlr_plan* plan, const hodlr_desc& d, const std::v
   rc = rc ? rc : palloc(p, &p->core_dinv, dof);
   rc = rc ? rc : palloc(p, &p->core_general, p->L > 0 ? ((int64_t)1 << (p->L - 1)) : 0);
   rc = rc ? rc : palloc(p, &p->core_tmask, p->L > 0 ? ((int64_t)1 << (p->L - 1)) : 0);
+  if (p->L > 0 && r > 64 && r <= 128) {
+    const int64_t nn = (int64_t)1 << (p->L - 1);
+    rc = rc ? rc : palloc(p, &p->schur.C, nn * hodlr::kCoreSchurElems);
+    rc = rc ? rc : palloc(p, &p->schur.LU, nn * hodlr::kCoreSchurElems);
+    rc = rc ? rc : palloc(p, &p->schur.dinv, nn * hodlr::dinv_elems(128));
+    rc = rc ? rc : palloc(p, &p->schur.ipiv, nn * 128);
+    rc = rc ? rc : palloc(p, &p->schur.perm, nn * 128);
+    rc = rc ? rc : palloc(p, &p->schur.info, nn);
+    rc = rc ? rc : palloc(p, &p->schur.retry, nn);
+  }
   // per-leaf projections / accumulators and core RHS for the widest
   // operand: [Y | B] with nrhs_max columns, or a solve chunk of that width
   p->ncmax = r + (p->smax > 0 ? p->smax : 1);

@@ -81,9 +81,22 @@ hipError_t launch_add_cols(int64_t rows, int cols, const double* X, int64_t ldx,
 // without interchanges and, when r is a multiple of 32, get their 32x32
 // diagonal inverses (dinv, dinv_elems(n2) per core) written here; tmask[b]
 // is set for every core whose inverses still have to be formed.
+// Scratch for cores of rank 64 < r <= 128 (per core: the padded 128 x 128
+// Schur complement and its LU, ld 136, their inverses, pivots, INFO and
+// retry flag); null for other ranks (then the general LU runs).
+struct CoreSchur {
+  double* C = nullptr;
+  double* LU = nullptr;
+  double* dinv = nullptr;
+  int* ipiv = nullptr;
+  int* perm = nullptr;
+  int* info = nullptr;
+  int* retry = nullptr;
+};
+constexpr int64_t kCoreSchurElems = 136 * 128;  // doubles of C (and of LU) per core
 hipError_t launch_getrf_core(int n2, double* S, int64_t lda, int64_t bstride, int batch, int* ipiv,
                              int* perm, int* info, int* general, double* dinv, int* tmask,
-                             hipStream_t st);
+                             const CoreSchur* cs, hipStream_t st);
 // launch_trtri_diag for the items b with only[b] != 0.
 hipError_t launch_trtri_diag_masked(int n, const double* LU, int64_t lda, int64_t bstride,
                                     int batch, double* dinv, const int* only, hipStream_t st);

@@ -2257,14 +2257,132 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
   if (lane == 0) info0[b] = info;
 }
 
+// Cores of rank 64 < r <= 128 (S = [I, A; B, I], n2 = 2r <= 256).  The
+// same structure as k_getrf_core, with the Schur complement too large for
+// one wave: k_core_schur128 forms C = I - B A per core on MFMA (operands
+// read from S; padded with the identity to 128 x 128, ld kC) and flags
+// cores with max|B| > 1 or a NaN in B as general; C is then factored by
+// the 128-row leaf LU (its fast path, else the pivoted kernel: P2 C =
+// L2 U2 with dgetrf's pivots -- the identity padding is never chosen as a
+// pivot of the first r columns, where it holds zeros); k_core_compose
+// writes S's LU, L = [I, 0; P2 B, L2], U = [I, A; 0, U2], its pivots
+// [1..r, r + ipiv2] and INFO.  General cores are factored by the pivoted
+// kernel in a masked launch.
+constexpr int kC = 136;  // leading dimension of the padded Schur complements
+
+__global__ void __launch_bounds__(256)
+k_core_schur128(const double* __restrict__ S0, int64_t lda, int64_t bstride, int r,
+                double* __restrict__ C0, int* __restrict__ general) {
+  __shared__ unsigned long long wmk[4];
+  const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const double* S = S0 + (int64_t)b * bstride;
+  double* C = C0 + (int64_t)b * kC * 128;
+  unsigned long long mk = 0ULL;
+  for (int c = wid; c < r; c += 4)
+    for (int i = lane; i < r; i += 64) {
+      const double v = S[(int64_t)(r + i) + (int64_t)c * lda];
+      const unsigned long long k = (v == v) ? (unsigned long long)__double_as_longlong(fabs(v)) : ~0ULL;
+      mk = k > mk ? k : mk;
+    }
+  mk = wave_max_u64(mk);
+  if (lane == 0) wmk[wid] = mk;
+  __syncthreads();
+  mk = umax64(umax64(wmk[0], wmk[1]), umax64(wmk[2], wmk[3]));
+  const bool gen = mk > (unsigned long long)__double_as_longlong(1.0);
+  if (tid == 0) general[b] = gen ? 1 : 0;
+  // 8 x 8 tiles of 16 x 16; wave w takes tiles w, w + 4, ...
+  for (int t = wid; t < 64; t += 4) {
+    const int tm = t & 7, tn = t >> 3;
+    d4 acc = d4zero();
+    if (!gen && tm * 16 < r && tn * 16 < r) {
+      const int ar = tm * 16 + lr, bc = tn * 16 + lr;
+      for (int k0 = 0; k0 < r; k0 += 4) {
+        const int k = k0 + lk;
+        const double af = (ar < r && k < r) ? S[(int64_t)(r + ar) + (int64_t)k * lda] : 0.0;
+        const double bf = (bc < r && k < r) ? S[(int64_t)k + (int64_t)(r + bc) * lda] : 0.0;
+        acc = mfma(af, bf, acc);
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int row = tm * 16 + crow(lane, v), col = tn * 16 + lr;
+      const double e = (row == col) ? 1.0 : 0.0;
+      C[row + (int64_t)col * kC] = (!gen && row < r && col < r) ? e - acc[v] : e;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256)
+k_core_compose(double* __restrict__ S0, int64_t lda, int64_t bstride, int r,
+               const double* __restrict__ LU0, const int* __restrict__ ipiv2,
+               const int* __restrict__ perm2, const int* __restrict__ info2,
+               const int* __restrict__ general, int* __restrict__ ipiv0, int* __restrict__ perm0,
+               int* __restrict__ info0) {
+  __shared__ double Bt[128][65];  // a 64-column chunk of B, [row][col]
+  const int b = blockIdx.x, tid = threadIdx.x;
+  if (general[b]) return;  // uniform over the workgroup
+  const int n2 = 2 * r;
+  double* S = S0 + (int64_t)b * bstride;
+  const double* LU = LU0 + (int64_t)b * kC * 128;
+  const int* pv = perm2 + (int64_t)b * 128;
+  // L2 \ U2 into S(r:2r, r:2r)
+  for (int e = tid; e < r * r; e += 256) {
+    const int i = e % r, c = e / r;
+    S[(int64_t)(r + i) + (int64_t)(r + c) * lda] = LU[i + (int64_t)c * kC];
+  }
+  // P2 B into S(r:2r, 0:r): row i takes B's row perm2[i], staged by chunks
+  for (int c0 = 0; c0 < r; c0 += 64) {
+    const int w = min(64, r - c0);
+    for (int e = tid; e < r * w; e += 256) {
+      const int i = e % r, c = e / r;
+      Bt[i][c] = S[(int64_t)(r + i) + (int64_t)(c0 + c) * lda];
+    }
+    __syncthreads();
+    for (int e = tid; e < r * w; e += 256) {
+      const int i = e % r, c = e / r;
+      S[(int64_t)(r + i) + (int64_t)(c0 + c) * lda] = Bt[pv[i]][c];
+    }
+    __syncthreads();
+  }
+  int* ipiv = ipiv0 + (int64_t)b * n2;
+  int* perm = perm0 + (int64_t)b * n2;
+  for (int i = tid; i < r; i += 256) {
+    ipiv[i] = i + 1;
+    perm[i] = i;
+    ipiv[r + i] = r + ipiv2[(int64_t)b * 128 + i];
+    perm[r + i] = r + pv[i];
+  }
+  if (tid == 0) {
+    const int f = info2[b];
+    info0[b] = (f > 0 && f <= r) ? r + f : 0;
+  }
+}
+
 hipError_t launch_getrf_core(int n2, double* S, int64_t lda, int64_t bstride, int batch, int* ipiv,
                              int* perm, int* info, int* general, double* dinv, int* tmask,
-                             hipStream_t st) {
+                             const CoreSchur* cs, hipStream_t st) {
   if (batch <= 0 || n2 <= 0) return hipSuccess;
   if (n2 > 128 || (n2 & 1)) {
-    // every core takes the general LU; all need their inverses formed
+    // every core needs its inverses formed
     hipLaunchKernelGGL(k_fill_int, dim3((batch + 255) / 256), dim3(256), 0, st, tmask, batch, 1);
-    return launch_getrf(n2, S, lda, bstride, batch, ipiv, perm, info, st);
+    const int r = n2 >> 1;
+    if ((n2 & 1) || n2 > 256 || !cs) return launch_getrf(n2, S, lda, bstride, batch, ipiv, perm, info, st);
+    hipLaunchKernelGGL(k_core_schur128, dim3(batch), dim3(256), 0, st, S, lda, bstride, r, cs->C,
+                       general);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = launch_getrf_leaf(128, cs->C, kC, (int64_t)kC * 128, cs->LU, kC, (int64_t)kC * 128, batch,
+                          cs->dinv, cs->ipiv, cs->perm, cs->info, cs->retry, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_core_compose, dim3(batch), dim3(256), 0, st, S, lda, bstride, r, cs->LU,
+                       cs->ipiv, cs->perm, cs->info, general, ipiv, perm, info);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // general cores (max|B| > 1): the pivoted LU of S in place
+    hipLaunchKernelGGL((k_getrf_reg<GETRF_NB, 256, 2, double>), dim3(batch), dim3(256), 0, st, S,
+                       lda, bstride, n2, ipiv, perm, info, (const int*)general,
+                       (const double*)nullptr, (int64_t)0, (int64_t)0);
+    return hipGetLastError();
   }
   hipLaunchKernelGGL(k_getrf_core, dim3(batch), dim3(256), 0, st, S, lda, bstride, n2, ipiv, perm,
                      info, general, dinv, tmask);

@@ -317,14 +317,16 @@ def test_padded_leading_dimension_is_bitwise_identical(gpu):
     assert np.array_equal(outs[0][1], outs[1][1])
 
 
-@pytest.mark.parametrize("scale", [1.0, 200.0])
-def test_core_structured_and_general_lu(gpu, orc, scale):
+@pytest.mark.parametrize("scale,r", [(1.0, 16), (200.0, 16), (1.0, 100), (200.0, 100)])
+def test_core_structured_and_general_lu(gpu, orc, scale, r):
     """Cores S = [I, A; B, I] take the structured LU when max|B| <= 1 (plain
     workload) and the general pivoted LU otherwise (U scaled by 200 makes
-    |B| > 1 at every level); both match the oracle to 1e-10."""
+    |B| > 1 at every level); both match the oracle to 1e-10.  Rank 100 takes
+    the wide-core path (64 < r <= 128: the Schur complement factored by the
+    128-row leaf LU)."""
     F = gpu
-    N, m, r, s = 2048, 128, 16, 2
-    say(f"case: core LU path, U scale {scale}")
+    N, m, s = 2048, 128, 2
+    say(f"case: core LU path, U scale {scale}, rank {r}")
     K, U, V, B = F.workload_gaussian(N, m, r, s, seed=13)
     U.mul_(scale)
     Kh, Uh, Vh, Bh = (to_host(t) for t in (K, U, V, B))
@@ -332,12 +334,18 @@ def test_core_structured_and_general_lu(gpu, orc, scale):
     plan = F.HodlrPlan(N, m, 4, r, nrhs_max=s)
     plan.factor_solve(K, U, V, B)
     assert plan.status()[0] == 0
-    e = relerr(to_host(B), ref)
-    say(f"  rel err {e:.2e}")
-    assert e <= RTOL
+    x = to_host(B)
+    e = relerr(x, ref)
+    # the tolerance: 1e-10, or 4x the oracle's own distance from a dense
+    # solve where the scaled system amplifies rounding more (rank 100, U x 200:
+    # cond 7e4, oracle vs dense 2.4e-10)
+    xd = np.linalg.solve(orc.dense_assemble(Kh, Uh, Vh, m), Bh)
+    tol = max(RTOL, 4.0 * relerr(ref, xd))
+    say(f"  rel err {e:.2e} (tol {tol:.1e})")
+    assert e <= tol and relerr(x, xd) <= tol
     plan.close()
 
-@pytest.mark.parametrize("r", [16, 30, 64])
+@pytest.mark.parametrize("r", [16, 30, 64, 96, 128])
 def test_core_schur_complement_pivots(gpu, orc, r):
     """A root core whose structured LU pivots inside the Schur complement:
     identity leaves (Y = U), V_c = [I; 0], U_0 = [B; 0] with |B| <= 1 (the
