@@ -2273,6 +2273,10 @@ constexpr int kC = 136;  // leading dimension of the padded Schur complements
 __global__ void __launch_bounds__(256)
 k_core_schur128(const double* __restrict__ S0, int64_t lda, int64_t bstride, int r,
                 double* __restrict__ C0, int* __restrict__ general) {
+  // k-chunks of 16: B(:, k0:k0+16) and A(k0:k0+16, :) staged in LDS, the
+  // next chunk's loads in registers under the current chunk's MFMAs
+  __shared__ double Bk[128][17];   // B(row, k0 + k)
+  __shared__ double Ak[16][129];   // A(k0 + k, col)
   __shared__ unsigned long long wmk[4];
   const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
@@ -2291,24 +2295,52 @@ k_core_schur128(const double* __restrict__ S0, int64_t lda, int64_t bstride, int
   mk = umax64(umax64(wmk[0], wmk[1]), umax64(wmk[2], wmk[3]));
   const bool gen = mk > (unsigned long long)__double_as_longlong(1.0);
   if (tid == 0) general[b] = gen ? 1 : 0;
-  // 8 x 8 tiles of 16 x 16; wave w takes tiles w, w + 4, ...
-  for (int t = wid; t < 64; t += 4) {
-    const int tm = t & 7, tn = t >> 3;
-    d4 acc = d4zero();
-    if (!gen && tm * 16 < r && tn * 16 < r) {
-      const int ar = tm * 16 + lr, bc = tn * 16 + lr;
-      for (int k0 = 0; k0 < r; k0 += 4) {
-        const int k = k0 + lk;
-        const double af = (ar < r && k < r) ? S[(int64_t)(r + ar) + (int64_t)k * lda] : 0.0;
-        const double bf = (bc < r && k < r) ? S[(int64_t)k + (int64_t)(r + bc) * lda] : 0.0;
-        acc = mfma(af, bf, acc);
+  const int nt = (r + 15) >> 4;  // tiles per side (<= 8)
+  // wave w owns tiles t = w + 4q (q < 16): tm = t & 7, tn = t >> 3
+  d4 acc[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) acc[q] = d4zero();
+  // staging: thread e loads B(e & 127, k0 + (e >> 7) + 2j) and
+  // A(k0 + (e >> 7) + 2j, e & 127), j < 8
+  const int sr = tid & 127, sk = tid >> 7;
+  double pb[8], pa[8];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int k = k0 + sk + 2 * j;
+      const bool ok = !gen && k < r && sr < r;
+      pb[j] = ok ? S[(int64_t)(r + sr) + (int64_t)k * lda] : 0.0;
+      pa[j] = ok ? S[(int64_t)k + (int64_t)(r + sr) * lda] : 0.0;
+    }
+  };
+  if (!gen) fetch(0);
+  for (int k0 = 0; k0 < r && !gen; k0 += 16) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      Bk[sr][sk + 2 * j] = pb[j];
+      Ak[sk + 2 * j][sr] = pa[j];
+    }
+    __syncthreads();
+    if (k0 + 16 < r) fetch(k0 + 16);
+#pragma unroll
+    for (int s2 = 0; s2 < 4; ++s2) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int t = wid + 4 * q, tm = t & 7, tn = t >> 3;
+        if (tm < nt && tn < nt)
+          acc[q] = mfma(Bk[tm * 16 + lr][4 * s2 + lk], Ak[4 * s2 + lk][tn * 16 + lr], acc[q]);
       }
     }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int t = wid + 4 * q, tm = t & 7, tn = t >> 3;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int row = tm * 16 + crow(lane, v), col = tn * 16 + lr;
       const double e = (row == col) ? 1.0 : 0.0;
-      C[row + (int64_t)col * kC] = (!gen && row < r && col < r) ? e - acc[v] : e;
+      C[row + (int64_t)col * kC] = (!gen && row < r && col < r) ? e - acc[q][v] : e;
     }
   }
 }
