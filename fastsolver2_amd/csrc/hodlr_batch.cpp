@@ -294,16 +294,24 @@ int hodlr_batch_tune(hodlr_batch b, double* const* K, const int64_t* ldk, const 
     for (int tile : kTiles) {
       b->lanes = lanes;
       b->tile = tile;
-      // first call captures (and runs), the second is the timed replay
-      rc = hodlr_batch_factor_solve(b, K, ldk, U, ldu, V, ldv, B, ldb, nrhs, stream);
+      // the first call captures (and runs), the second is an untimed warm
+      // replay (a graph's first launch uploads it), then the fastest of
+      // two timed replays
+      for (int w = 0; w < 2 && !rc; ++w)
+        rc = hodlr_batch_factor_solve(b, K, ldk, U, ldu, V, ldv, B, ldb, nrhs, stream);
       if (rc) break;
-      (void)hipEventRecord(t0, stream);
-      rc = hodlr_batch_factor_solve(b, K, ldk, U, ldu, V, ldv, B, ldb, nrhs, stream);
-      (void)hipEventRecord(t1, stream);
+      float ms = 1e30f;
+      for (int rep = 0; rep < 2 && !rc; ++rep) {
+        (void)hipEventRecord(t0, stream);
+        rc = hodlr_batch_factor_solve(b, K, ldk, U, ldu, V, ldv, B, ldb, nrhs, stream);
+        (void)hipEventRecord(t1, stream);
+        if (rc) break;
+        if (hipEventSynchronize(t1) != hipSuccess) { rc = HODLR_ERR_HIP; break; }
+        float m = 0.f;
+        (void)hipEventElapsedTime(&m, t0, t1);
+        ms = m < ms ? m : ms;
+      }
       if (rc) break;
-      if (hipEventSynchronize(t1) != hipSuccess) { rc = HODLR_ERR_HIP; break; }
-      float ms = 0.f;
-      (void)hipEventElapsedTime(&ms, t0, t1);
       if (ms < best) { best = ms; bl = lanes; bt = tile; }
     }
     if (rc) break;

@@ -240,9 +240,13 @@ struct hodlr_plan_s {
   double* leaf_dinv = nullptr;
   // Y = Y_{L-1} = K^-1 U (N x r, ld N)
   double* Y = nullptr;
-  // W_t,i (r x r, ld r) at W + (t*P + i)*r*r; Q_i = V_i^T Y_i at Q + i*r*r
+  // W_t,i (r x r, ld r) at W + (t*P + i)*r*r for t < L-1 (W_{L-1} = I is
+  // not stored: the products take a null operand as the identity)
   double* W = nullptr;
-  double* Q = nullptr;
+  // the leaf projections of the factorization, V_i^T [Y_i | B_i] (r x nq per
+  // leaf): their first r columns are Q_i = V_i^T Y_i, kept for the solves
+  double* Pq = nullptr;
+  int nq = 0;
   // cores, all levels; level t at offsets *_off[t]
   double* coreS = nullptr;
   int* core_ipiv = nullptr;
@@ -468,13 +472,13 @@ int level_phase(hodlr_plan p, double* B, int64_t ldb, int nrhs, hipStream_t st) 
   ColSeg x;
   x.inA = p->Y; x.outA = p->Y; x.ldiA = p->N; x.ldoA = p->N; x.nA = r;
   x.inB = B; x.outB = B; x.ldiB = ldb; x.ldoB = ldb;
-  double* Pc = p->Pb[0];
-  double* Pn = p->Pb[1];
+  // the leaf projections go to Pq, which keeps them (Q_i = its first r
+  // columns); the level updates then alternate between Pb[0] and Pb[1]
+  double* Pc = p->Pq;
+  double* Pn = p->Pb[0];
   int rc = leaf_projections(p, p->V, p->ldv, x, nc, Pc, st);
   if (rc) return rc;
-  // Q_i = V_i^T Y_i: the first r columns of each projection
-  HIP_TRY(hodlr::launch_copy_cols((int64_t)r * r, (int)P, Pc, (int64_t)r * nc, p->Q, p->wst, st));
-  HIP_TRY(hodlr::launch_eye((int)P, r, p->wst, W_at(p, L - 1), st));
+  p->nq = nc;
   for (int t = L - 1; t >= 0; --t) {
     const int nnode = 1 << t;
     const int shift = L - 1 - t;  // leaf -> child at level t
@@ -492,11 +496,13 @@ int level_phase(hodlr_plan p, double* B, int64_t ldb, int nrhs, hipStream_t st) 
     // W_t E_c -> W_{t-1} = W_t - W_t E^Y, G (+)= W_t E^B
     hodlr::BGemmArgs g;
     g.batch = (int)P; g.M = r; g.K = r; g.Nc = nc; g.csplit = r;
-    g.A = W_at(p, t); g.lda = r; g.sa = p->wst;
+    // W_{L-1} = I: a null operand
+    double* Wt = (t == L - 1) ? nullptr : W_at(p, t);
+    g.A = Wt; g.lda = r; g.sa = p->wst;
     g.B = c.R; g.ldb = 2 * r; g.bshift = shift; g.bs2 = ens; g.bs1 = r;
     if (t > 0) {
       g.alpha1 = -1.0; g.beta1 = 1.0;
-      g.D1 = W_at(p, t); g.ldd1 = r; g.sd1 = p->wst;
+      g.D1 = Wt; g.ldd1 = r; g.sd1 = p->wst;
       g.C1 = W_at(p, t - 1); g.ldc1 = r; g.sc1 = p->wst;
     }
     if (nrhs > 0) {
@@ -518,7 +524,8 @@ int level_phase(hodlr_plan p, double* B, int64_t ldb, int nrhs, hipStream_t st) 
         HIP_TRY(hodlr::launch_bgemm2(g, h, st));
       else
         HIP_TRY(hodlr::launch_bgemm(h, st));
-      std::swap(Pc, Pn);
+      Pc = Pn;
+      Pn = (Pn == p->Pb[0]) ? p->Pb[1] : p->Pb[0];
     } else if (g.C1 || g.C2) {
       HIP_TRY(hodlr::launch_bgemm(g, st));
     }
@@ -660,7 +667,7 @@ static int plan_create_sizes(hodlr_plan* plan, const hodlr_desc& d, const std::v
   rc = rc ? rc : palloc(p, &p->Y, p->L > 0 ? p->N * r : 0);
   p->wst = (int64_t)r * r;
   rc = rc ? rc : palloc(p, &p->W, (int64_t)p->L * p->P * p->wst);
-  rc = rc ? rc : palloc(p, &p->Q, p->L > 0 ? p->P * p->wst : 0);
+  rc = rc ? rc : palloc(p, &p->Pq, p->L > 0 ? p->P * r * (r + (p->smax > 0 ? p->smax : 1)) : 0);
   int64_t so = 0, po = 0, dof = 0, io = 0;
   for (int t = 0; t < p->L; ++t) {
     const int64_t nn = (int64_t)1 << t;
@@ -828,7 +835,7 @@ int hodlr_solve(hodlr_plan p, double* B, int64_t ldb, int nrhs, hipStream_t st) 
       if (t < L - 1) {
         hodlr::BGemmArgs h;
         h.batch = (int)P; h.M = r; h.K = r; h.Nc = nc; h.csplit = nc;
-        h.A = p->Q; h.lda = r; h.sa = p->wst;
+        h.A = p->Pq; h.lda = r; h.sa = (int64_t)r * p->nq;
         h.B = p->G; h.ldb = r; h.bs1 = ps; h.bs2 = 2 * ps;
         h.alpha1 = -1.0; h.beta1 = 1.0;
         h.D1 = p->Pb[0]; h.ldd1 = r; h.sd1 = ps;
@@ -843,7 +850,7 @@ int hodlr_solve(hodlr_plan p, double* B, int64_t ldb, int nrhs, hipStream_t st) 
       // G_i (+)= W_t,i eta_c
       hodlr::BGemmArgs g2;
       g2.batch = (int)P; g2.M = r; g2.K = r; g2.Nc = nc; g2.csplit = nc;
-      g2.A = W_at(p, t); g2.lda = r; g2.sa = p->wst;
+      g2.A = (t == L - 1) ? nullptr : W_at(p, t); g2.lda = r; g2.sa = p->wst;
       g2.B = c.R; g2.ldb = 2 * r; g2.bshift = shift; g2.bs2 = (int64_t)2 * r * nc; g2.bs1 = r;
       g2.alpha1 = 1.0; g2.beta1 = (t == L - 1) ? 0.0 : 1.0;
       g2.D1 = p->G; g2.ldd1 = r; g2.sd1 = ps;
@@ -881,7 +888,7 @@ int hodlr_solve_adjoint(hodlr_plan p, double* G, int64_t ldg, int nrhs, hipStrea
         if (t > 0) {
           hodlr::BGemmArgs h;
           h.batch = (int)P; h.M = r; h.K = r; h.Nc = nc; h.csplit = nc; h.transA = 1;
-          h.A = p->Q; h.lda = r; h.sa = p->wst;
+          h.A = p->Pq; h.lda = r; h.sa = (int64_t)r * p->nq;
           h.B = p->G; h.ldb = r; h.bs1 = ps; h.bs2 = 2 * ps;
           h.alpha1 = -1.0; h.beta1 = 1.0;
           h.D1 = p->Pb[0]; h.ldd1 = r; h.sd1 = ps;
@@ -891,7 +898,7 @@ int hodlr_solve_adjoint(hodlr_plan p, double* G, int64_t ldg, int nrhs, hipStrea
         }
         hodlr::BGemmArgs z;
         z.batch = (int)P; z.M = r; z.K = r; z.Nc = nc; z.csplit = nc; z.transA = 1;
-        z.A = W_at(p, t); z.lda = r; z.sa = p->wst;
+        z.A = (t == L - 1) ? nullptr : W_at(p, t); z.lda = r; z.sa = p->wst;
         z.B = u; z.ldb = r; z.bs1 = ps; z.bs2 = 2 * ps;
         z.alpha1 = 1.0; z.beta1 = 0.0;
         z.C1 = p->Pb[2]; z.ldc1 = r; z.sc1 = ps;

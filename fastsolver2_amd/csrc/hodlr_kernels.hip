@@ -4027,7 +4027,7 @@ __global__ void k_copy_cols(int64_t rows, int cols, const double* __restrict__ s
 // written to (C1, D1, alpha1, beta1) for j < csplit and to (C2, D2,
 // alpha2, beta2) at column j - csplit otherwise; a null C skips its
 // columns.  A_i = A + i*sa (M x K, or K x M read transposed; null A is the
-// identity); B_i = B + (c>>1)*bs2 + ((c&1)^bflip)*bs1 with c = i >> bshift,
+// identity; a null D with a nonzero beta is the identity too); B_i = B + (c>>1)*bs2 + ((c&1)^bflip)*bs1 with c = i >> bshift,
 // which selects the half of a node's core solution that belongs to the
 // leaf's child at a given level.  16 x 16 output tiles are dealt to the
 // four waves.  D may alias C; A and B must not.
@@ -4127,10 +4127,14 @@ __global__ void __launch_bounds__(256) k_bgemm(BGemmArgs g0, BGemmArgs g1) {
       for (int j = 0; j < GN; ++j) {
         const int ocol = (tn0 + j) * 16 + lr;
         const double* dp = ocol < g.csplit ? g.D1 : g.D2;
+        const int dcol = ocol < g.csplit ? ocol : ocol - g.csplit;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int orow = (tm0 + a) * 16 + crow(lane, v);
-          dv[a][j][v] = (colok[j] && tm0 + a < mt && orow < g.M && be[j] != 0.0) ? dp[dof[j] + orow] : 0.0;
+          // a null D with beta != 0 is the identity
+          dv[a][j][v] = (colok[j] && tm0 + a < mt && orow < g.M && be[j] != 0.0)
+                            ? (dp ? dp[dof[j] + orow] : (orow == dcol ? 1.0 : 0.0))
+                            : 0.0;
         }
       }
 #pragma unroll

@@ -209,9 +209,10 @@ int hodlr_batch_status(hodlr_batch batch, hipStream_t stream, int *sys, int *whe
  * column tile (0 = by width, else 1, 2 or 5 MFMA tiles per workgroup). */
 int hodlr_batch_set_tuning(hodlr_batch batch, int lanes, int tile);
 int hodlr_batch_get_tuning(hodlr_batch batch, int *lanes, int *tile);
-/* Autotuner: times a graph replay of the batch for every (lanes, tile)
- * pair on the given operands and keeps the fastest (*best_ms).  The
- * operands are overwritten many times: pass scratch copies. */
+/* Autotuner: for every (lanes, tile) pair, captures the batch, replays it
+ * once untimed (a graph's first launch uploads it), times the faster of two
+ * replays, and keeps the fastest pair (*best_ms).  The operands are
+ * overwritten many times: pass scratch copies. */
 int hodlr_batch_tune(hodlr_batch batch, double *const *K, const int64_t *ldk,
                      const double *const *U, const int64_t *ldu,
                      const double *const *V, const int64_t *ldv,
