@@ -82,8 +82,8 @@ hipError_t launch_add_cols(int64_t rows, int cols, const double* X, int64_t ldx,
 // diagonal inverses (dinv, dinv_elems(n2) per core) written here; tmask[b]
 // is set for every core whose inverses still have to be formed.
 // Scratch for cores of rank 64 < r <= 128 (per core: the padded 128 x 128
-// Schur complement and its LU, ld 136, their inverses, pivots, INFO and
-// retry flag); null for other ranks (then the general LU runs).
+// Schur complement and its LU, ld 136, the LU's diagonal inverses, pivots,
+// INFO and retry flag); null for other ranks (then the general LU runs).
 struct CoreSchur {
   double* C = nullptr;
   double* LU = nullptr;

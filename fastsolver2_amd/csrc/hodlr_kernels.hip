@@ -2269,10 +2269,15 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
 // [1..r, r + ipiv2] and INFO.  General cores are factored by the pivoted
 // kernel in a masked launch.
 constexpr int kC = 136;  // leading dimension of the padded Schur complements
+template <typename T>
+static hipError_t getrf_leaf_impl(int n, const double* K, int64_t ldk, int64_t kbs, T* LU,
+                                  int64_t ldl, int64_t lbs, int batch, T* dinv, int* ipiv,
+                                  int* perm, int* info, int* retry, hipStream_t st,
+                                  bool fallback_inverses);
 
 __global__ void __launch_bounds__(256)
 k_core_schur128(const double* __restrict__ S0, int64_t lda, int64_t bstride, int r,
-                double* __restrict__ C0, int* __restrict__ general) {
+                double* __restrict__ C0, int* __restrict__ general, int* __restrict__ tmask) {
   // k-chunks of 16: B(:, k0:k0+16) and A(k0:k0+16, :) staged in LDS, the
   // next chunk's loads in registers under the current chunk's MFMAs
   __shared__ double Bk[128][17];   // B(row, k0 + k)
@@ -2294,7 +2299,10 @@ k_core_schur128(const double* __restrict__ S0, int64_t lda, int64_t bstride, int
   __syncthreads();
   mk = umax64(umax64(wmk[0], wmk[1]), umax64(wmk[2], wmk[3]));
   const bool gen = mk > (unsigned long long)__double_as_longlong(1.0);
-  if (tid == 0) general[b] = gen ? 1 : 0;
+  if (tid == 0) {
+    general[b] = gen ? 1 : 0;
+    tmask[b] = 1;  // every wide core's inverses are formed afterwards
+  }
   const int nt = (r + 15) >> 4;  // tiles per side (<= 8)
   // wave w owns tiles t = w + 4q (q < 16): tm = t & 7, tn = t >> 3
   d4 acc[16];
@@ -2396,16 +2404,22 @@ hipError_t launch_getrf_core(int n2, double* S, int64_t lda, int64_t bstride, in
                              const CoreSchur* cs, hipStream_t st) {
   if (batch <= 0 || n2 <= 0) return hipSuccess;
   if (n2 > 128 || (n2 & 1)) {
-    // every core needs its inverses formed
-    hipLaunchKernelGGL(k_fill_int, dim3((batch + 255) / 256), dim3(256), 0, st, tmask, batch, 1);
     const int r = n2 >> 1;
-    if ((n2 & 1) || n2 > 256 || !cs) return launch_getrf(n2, S, lda, bstride, batch, ipiv, perm, info, st);
+    if ((n2 & 1) || n2 > 256 || !cs) {
+      // every core needs its inverses formed
+      hipLaunchKernelGGL(k_fill_int, dim3((batch + 255) / 256), dim3(256), 0, st, tmask, batch, 1);
+      return launch_getrf(n2, S, lda, bstride, batch, ipiv, perm, info, st);
+    }
     hipLaunchKernelGGL(k_core_schur128, dim3(batch), dim3(256), 0, st, S, lda, bstride, r, cs->C,
-                       general);
+                       general, tmask);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    e = launch_getrf_leaf(128, cs->C, kC, (int64_t)kC * 128, cs->LU, kC, (int64_t)kC * 128, batch,
-                          cs->dinv, cs->ipiv, cs->perm, cs->info, cs->retry, st);
+    // C's diagonal inverses serve the fast kernel's own chunk solves; those
+    // of the blocks refactored with interchanges are not needed (S's
+    // inverses are formed afterwards)
+    e = getrf_leaf_impl<double>(128, cs->C, kC, (int64_t)kC * 128, cs->LU, kC, (int64_t)kC * 128,
+                                batch, cs->dinv, cs->ipiv, cs->perm, cs->info, cs->retry, st,
+                                false);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_core_compose, dim3(batch), dim3(256), 0, st, S, lda, bstride, r, cs->LU,
                        cs->ipiv, cs->perm, cs->info, general, ipiv, perm, info);
@@ -3006,7 +3020,8 @@ hipError_t fallback_getrf<float>(int n, const double* K, int64_t ldk, int64_t kb
 template <typename T>
 static hipError_t getrf_leaf_impl(int n, const double* K, int64_t ldk, int64_t kbs, T* LU,
                                   int64_t ldl, int64_t lbs, int batch, T* dinv, int* ipiv,
-                                  int* perm, int* info, int* retry, hipStream_t st) {
+                                  int* perm, int* info, int* retry, hipStream_t st,
+                                  bool fallback_inverses) {
   if (batch <= 0 || n <= 0) return hipSuccess;
   // the fast kernel addresses a slab of K and a block of LU with 32-bit
   // byte offsets
@@ -3025,7 +3040,7 @@ static hipError_t getrf_leaf_impl(int n, const double* K, int64_t ldk, int64_t k
   }
   const int* only = fast ? retry : nullptr;
   hipError_t e = fallback_getrf<T>(n, K, ldk, kbs, LU, ldl, lbs, batch, ipiv, perm, info, only, st);
-  if (e != hipSuccess) return e;
+  if (e != hipSuccess || !fallback_inverses) return e;
   hipLaunchKernelGGL(k_trtri_diag<T>, dim3(batch * ntri(n)), dim3(64), 0, st, LU, ldl, lbs, n,
                      dinv, only);
   return hipGetLastError();
@@ -3035,13 +3050,13 @@ hipError_t launch_getrf_leaf(int n, const double* K, int64_t ldk, int64_t kbs, d
                              int64_t ldl, int64_t lbs, int batch, double* dinv, int* ipiv,
                              int* perm, int* info, int* retry, hipStream_t st) {
   return getrf_leaf_impl<double>(n, K, ldk, kbs, LU, ldl, lbs, batch, dinv, ipiv, perm, info,
-                                 retry, st);
+                                 retry, st, true);
 }
 hipError_t launch_getrf_leaf_f32(int n, const double* K, int64_t ldk, int64_t kbs, float* LU,
                                  int64_t ldl, int64_t lbs, int batch, float* dinv, int* ipiv,
                                  int* perm, int* info, int* retry, hipStream_t st) {
   return getrf_leaf_impl<float>(n, K, ldk, kbs, LU, ldl, lbs, batch, dinv, ipiv, perm, info,
-                                retry, st);
+                                retry, st, true);
 }
 
 hipError_t launch_getrs_f32(int n, const float* LU, int64_t lda, int64_t bstride, int batch,
