@@ -460,7 +460,7 @@ class HodlrBatch:
         return [C.cast(a, C.c_void_p) for a in arr], arr
 
     def factor_solve(self, Ks, Us, Vs, Bs):
-        """Every system: K_i overwritten by its leaf LU, B_i by the solution."""
+        """Every system: K_i is only read (its leaf LU is kept in system i's plan), B_i is overwritten by the solution."""
         ptrs, keep = self._arrays(Ks, Us, Vs, Bs)
         _call(lib().hodlr_batch_factor_solve, self._h, *ptrs, _stream(),
               where="hodlr_batch_factor_solve")
@@ -620,7 +620,7 @@ class HodlrSolveFunction(torch.autograd.Function):
         N = K.shape[0]
         levels = int(round(math.log2(N // leaf)))
         s = B.shape[1] if B.dim() == 2 else 1
-        Kf = fortran_empty(N, leaf, device=K.device)  # overwritten by the leaf LU
+        Kf = fortran_empty(N, leaf, device=K.device)  # column-major copy; the plan only reads it
         Kf.copy_(K.detach())
         Uf, Vf = fortran(U.detach()), fortran(V.detach())
         X = fortran_empty(N, s, device=B.device)
