@@ -54,15 +54,19 @@ def flops_alg(N, m, r, s, L=None):
 
 
 def flops_exec(N, m, r, s):
-    """FLOPs this library executes (one Y_t block per level, no copies)."""
+    """FLOPs this library executes (node-sum level loop): leaf LU, leaf solve
+    on [U | B], leaf projections, per node a 2r x 2r core LU and solve on
+    r + s columns, the top-down pass (L products r x r by r x s per leaf)
+    and the final B -= Y G."""
     P = N // m
     L = int(round(math.log2(P)))
     nc = r + s
     f = P * (2.0 / 3.0) * m ** 3 + 2.0 * P * m * m * (nc if L > 0 else s)
-    for t in range(L):
-        f += 2.0 * N * r * nc                              # reduce
-        f += 2 ** t * ((2.0 / 3.0) * (2 * r) ** 3 + 2.0 * (2 * r) ** 2 * nc)  # cores
-        f += 2.0 * N * r * (nc if t > 0 else s)            # update
+    if L > 0:
+        f += 2.0 * N * r * nc                                            # projections
+        f += (P - 1) * ((2.0 / 3.0) * (2 * r) ** 3 + 2.0 * (2 * r) ** 2 * nc)  # cores
+        f += P * L * 2.0 * r * r * s                                     # top-down pass
+        f += 2.0 * N * r * s                                             # B -= Y G
     return f
 
 

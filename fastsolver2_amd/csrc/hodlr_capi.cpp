@@ -226,7 +226,6 @@ struct hodlr_plan_s {
   // leaves as runs of equal size (one run for a balanced tree of equal
   // leaves; empty leaf slots form runs of size 0)
   std::vector<hodlr::LeafRun> runs;
-  int64_t wst = 0;  // per-leaf stride of the r x r blocks W_t,i and Q_i (r*r)
   // leaf factorization, out of place: leaf i's LU factors in the
   // contiguous block LU + i*lbs (ld ldl = leaf + pad), FP64 or (mixed
   // precision) FP32; the caller's K is only read
@@ -240,12 +239,10 @@ struct hodlr_plan_s {
   double* leaf_dinv = nullptr;
   // Y = Y_{L-1} = K^-1 U (N x r, ld N)
   double* Y = nullptr;
-  // W_t,i (r x r, ld r) at W + (t*P + i)*r*r for t < L-1 (W_{L-1} = I is
-  // not stored: the products take a null operand as the identity)
-  double* W = nullptr;
-  // the leaf projections of the factorization, V_i^T [Y_i | B_i] (r x nq per
-  // leaf): their first r columns are Q_i = V_i^T Y_i, kept for the solves
-  double* Pq = nullptr;
+  // the factorization's core solutions E_n = S_n^-1 R_n (2r x nq, ld 2r)
+  // of every node, heap order (node n of level t at (2^t - 1 + n)*2r*nq):
+  // their first r columns, E^Y, serve the solves
+  double* E = nullptr;
   int nq = 0;
   // cores, all levels; level t at offsets *_off[t]
   double* coreS = nullptr;
@@ -257,10 +254,13 @@ struct hodlr_plan_s {
   int* core_tmask = nullptr;    // cores left without diagonal inverses (largest level)
   hodlr::CoreSchur schur;       // Schur-complement scratch for ranks 64 < r <= 128
   std::vector<int64_t> s_off, p_off, d_off, i_off;
-  // workspace: per-leaf projections (two, ping-pong), accumulators, core RHS
-  double* Pb[3] = {nullptr, nullptr, nullptr};
+  // workspace: per-leaf projections and accumulators G_i (P x r x ncmax),
+  // a solve's core solutions (heap order, 2r x ncmax per node) and the
+  // adjoint's per-node sums (heap order, leaves included: r x ncmax)
+  double* Pb = nullptr;
   double* G = nullptr;
-  double* R = nullptr;
+  double* EB = nullptr;
+  double* Hb = nullptr;
   int ncmax = 0;
   // mixed precision: FP32 leaf factors (LU32) and their diagonal-block
   // inverses
@@ -368,7 +368,7 @@ double leaf_pow_sum(hodlr_plan p, int k) {
 CoreBufs core_level(hodlr_plan p, int t) {
   CoreBufs c;
   c.S = p->coreS + p->s_off[t];
-  c.R = p->R;
+  c.R = nullptr;  // set by the caller: the level's slot of a heap-ordered buffer
   c.ipiv = p->core_ipiv + p->p_off[t];
   c.perm = p->core_perm + p->p_off[t];
   c.info = p->core_info + p->i_off[t];
@@ -420,7 +420,8 @@ int leaf_phase(hodlr_plan p, const double* K, int64_t ldk, const double* U, int6
   return HODLR_OK;
 }
 
-double* W_at(hodlr_plan p, int t) { return p->W + (int64_t)t * p->P * p->wst; }
+// first heap index of level t (level L: the leaves)
+int64_t heap0(int t) { return ((int64_t)1 << t) - 1; }
 
 // Per-leaf projections part_i = V_i^T X_i (r x nc, ld r, stride r*nc).
 int leaf_projections(hodlr_plan p, const double* Vm, int64_t ldv, const ColSeg& x, int nc,
@@ -441,9 +442,12 @@ int leaf_projections(hodlr_plan p, const double* Vm, int64_t ldv, const ColSeg& 
   return HODLR_OK;
 }
 
-// B_i -= Y_i G_i for every leaf (G_i: r x nc, ld r, stride r*nc).
-int final_update(hodlr_plan p, const double* Ym, int64_t ldy, const double* Gm, int nc, double* B,
-                 int64_t ldb, hipStream_t st) {
+// B_i -= Y_i G_i for every leaf; G_c for leaf c at Gm + (c>>1)*gs2 +
+// ((c&1)^gflip)*gs1 (r x nc, ld ldg): per-leaf blocks (ldg = r, gs1 = r*nc,
+// gs2 = 2*gs1, gflip = 0) or the halves of the lowest level's core solutions.
+int final_update(hodlr_plan p, const double* Ym, int64_t ldy, const double* Gm, int64_t ldg,
+                 int64_t gs2, int64_t gs1, int gflip, int nc, double* B, int64_t ldb,
+                 hipStream_t st) {
   ProfScope ps(HODLR_PROF_UPDATE, st, 2.0 * (double)p->N * p->r * nc,
                8.0 * (double)p->N * (p->r + 2.0 * nc));
   for (const hodlr::LeafRun& R : p->runs) {
@@ -451,8 +455,8 @@ int final_update(hodlr_plan p, const double* Ym, int64_t ldy, const double* Gm, 
     hodlr::BGemmArgs g;
     g.batch = (int)R.count; g.M = R.m; g.K = p->r; g.Nc = nc; g.csplit = nc;
     g.A = Ym + R.row0; g.lda = ldy; g.sa = R.m;
-    g.B = Gm + R.first * p->r * nc; g.ldb = p->r; g.bshift = 0; g.bs1 = (int64_t)p->r * nc;
-    g.bs2 = 2 * g.bs1;
+    g.B = Gm; g.ldb = ldg; g.bshift = 0; g.bs1 = gs1; g.bs2 = gs2; g.bflip = gflip;
+    g.ioff = (int)R.first;
     g.alpha1 = -1.0; g.beta1 = 1.0;
     g.D1 = B + R.row0; g.ldd1 = ldb; g.sd1 = R.m;
     g.C1 = B + R.row0; g.ldc1 = ldb; g.sc1 = R.m;
@@ -461,77 +465,61 @@ int final_update(hodlr_plan p, const double* Ym, int64_t ldy, const double* Gm, 
   return HODLR_OK;
 }
 
-// Level loop of factor (+ solve, nrhs > 0) on per-leaf r x r blocks.
-// Input: [Y | B] after the leaf solve.  Output: W_t for every level, Q, the
-// core factorizations, and B overwritten by the solution.
+// B_i -= Y_i G_i with G_i in per-leaf blocks (r x nc, ld r).
+int final_update_leaf(hodlr_plan p, const double* Gm, int nc, double* B, int64_t ldb,
+                      hipStream_t st) {
+  const int64_t gs1 = (int64_t)p->r * nc;
+  return final_update(p, p->Y, p->N, Gm, p->r, 2 * gs1, gs1, 0, nc, B, ldb, st);
+}
+
+// Top-down pass: G_i for every non-empty leaf from the core solutions
+// (E^Y at EY, E^B at EB, heap order, node strides eys / ebs).
+int path_down(hodlr_plan p, const double* EY, int64_t eys, const double* EB, int64_t ebs, int nc,
+              hipStream_t st) {
+  const int r = p->r;
+  ProfScope ps(HODLR_PROF_LEAF_LEVEL, st, (double)p->P * p->L * 2.0 * r * r * nc,
+               8.0 * (double)p->P * p->L * r * (r + nc));
+  for (const hodlr::LeafRun& R : p->runs) {
+    if (R.m == 0) continue;
+    HIP_TRY(hodlr::launch_path_down(EY, eys, EB, ebs, p->L, r, nc, R.first, R.count, p->G, st));
+  }
+  return HODLR_OK;
+}
+
+// Level loop of factor (+ solve, nrhs > 0) in node-sum form.  Input:
+// [Y | B] after the leaf solve.  Output: every node's core factorization
+// and core solution E_n (heap order in p->E), and B overwritten by the
+// solution.
 int level_phase(hodlr_plan p, double* B, int64_t ldb, int nrhs, hipStream_t st) {
-  const int r = p->r, L = p->L;
-  const int64_t P = p->P;
+  const int r = p->r, L = p->L, n2 = 2 * r;
   const int nc = r + nrhs;
   if (L == 0) return HODLR_OK;
   ColSeg x;
   x.inA = p->Y; x.outA = p->Y; x.ldiA = p->N; x.ldoA = p->N; x.nA = r;
   x.inB = B; x.outB = B; x.ldiB = ldb; x.ldoB = ldb;
-  // the leaf projections go to Pq, which keeps them (Q_i = its first r
-  // columns); the level updates then alternate between Pb[0] and Pb[1]
-  double* Pc = p->Pq;
-  double* Pn = p->Pb[0];
-  int rc = leaf_projections(p, p->V, p->ldv, x, nc, Pc, st);
+  int rc = leaf_projections(p, p->V, p->ldv, x, nc, p->Pb, st);
   if (rc) return rc;
   p->nq = nc;
+  const int64_t es = (int64_t)n2 * nc;  // node stride of E
   for (int t = L - 1; t >= 0; --t) {
     const int nnode = 1 << t;
-    const int shift = L - 1 - t;  // leaf -> child at level t
     CoreBufs c = core_level(p, t);
+    c.R = p->E + heap0(t) * es;
     {
-      ProfScope ps(HODLR_PROF_ASSEMBLE, st, 0.0, 8.0 * (double)P * r * nc);
-      HIP_TRY(hodlr::launch_assemble(Pc, 1 << shift, r, nc, 0, 0, nc, nnode, c.S, c.R, st));
+      ProfScope ps(HODLR_PROF_ASSEMBLE, st, 0.0,
+                   8.0 * (double)nnode * (n2 * n2 + 2.0 * n2 * nc));
+      if (t == L - 1)  // the children are leaves: their projections
+        HIP_TRY(hodlr::launch_assemble(p->Pb, 1, r, nc, 0, 0, nc, nnode, c.S, c.R, st));
+      else
+        HIP_TRY(hodlr::launch_assemble_pair(p->E + heap0(t + 1) * es, es, r, 0, 0, nc, nnode, c.S,
+                                            c.R, es, st));
     }
     rc = core_factor_solve(nnode, r, c, nc, st);
     if (rc) return rc;
-    const int64_t ens = (int64_t)2 * r * nc;
-    ProfScope ps(HODLR_PROF_LEAF_LEVEL, st,
-                 (double)P * 2.0 * r * r * ((t > 0 ? 2.0 * nc : 0.0) + nrhs),
-                 8.0 * (double)P * r * (2.0 * nc + 2.0 * r + 2.0 * nrhs));
-    // W_t E_c -> W_{t-1} = W_t - W_t E^Y, G (+)= W_t E^B
-    hodlr::BGemmArgs g;
-    g.batch = (int)P; g.M = r; g.K = r; g.Nc = nc; g.csplit = r;
-    // W_{L-1} = I: a null operand
-    double* Wt = (t == L - 1) ? nullptr : W_at(p, t);
-    g.A = Wt; g.lda = r; g.sa = p->wst;
-    g.B = c.R; g.ldb = 2 * r; g.bshift = shift; g.bs2 = ens; g.bs1 = r;
-    if (t > 0) {
-      g.alpha1 = -1.0; g.beta1 = 1.0;
-      g.D1 = Wt; g.ldd1 = r; g.sd1 = p->wst;
-      g.C1 = W_at(p, t - 1); g.ldc1 = r; g.sc1 = p->wst;
-    }
-    if (nrhs > 0) {
-      g.alpha2 = 1.0; g.beta2 = (t == L - 1) ? 0.0 : 1.0;
-      g.D2 = p->G; g.ldd2 = r; g.sd2 = (int64_t)r * nrhs;
-      g.C2 = p->G; g.ldc2 = r; g.sc2 = (int64_t)r * nrhs;
-    }
-    if (t > 0) {
-      // P_{t-1,i} = P_t,i - P_t,i(:, 0:r) E_c, independent of the W/G
-      // product above: both go in one launch
-      hodlr::BGemmArgs h;
-      h.batch = (int)P; h.M = r; h.K = r; h.Nc = nc; h.csplit = nc;
-      h.A = Pc; h.lda = r; h.sa = (int64_t)r * nc;
-      h.B = c.R; h.ldb = 2 * r; h.bshift = shift; h.bs2 = ens; h.bs1 = r;
-      h.alpha1 = -1.0; h.beta1 = 1.0;
-      h.D1 = Pc; h.ldd1 = r; h.sd1 = (int64_t)r * nc;
-      h.C1 = Pn; h.ldc1 = r; h.sc1 = (int64_t)r * nc;
-      if (g.C1 || g.C2)
-        HIP_TRY(hodlr::launch_bgemm2(g, h, st));
-      else
-        HIP_TRY(hodlr::launch_bgemm(h, st));
-      Pc = Pn;
-      Pn = (Pn == p->Pb[0]) ? p->Pb[1] : p->Pb[0];
-    } else if (g.C1 || g.C2) {
-      HIP_TRY(hodlr::launch_bgemm(g, st));
-    }
   }
-  if (nrhs > 0) return final_update(p, p->Y, p->N, p->G, nrhs, B, ldb, st);
-  return HODLR_OK;
+  if (nrhs == 0) return HODLR_OK;
+  rc = path_down(p, p->E, es, p->E + (int64_t)r * n2, es, nrhs, st);
+  return rc ? rc : final_update_leaf(p, p->G, nrhs, B, ldb, st);
 }
 
 }  // namespace
@@ -665,9 +653,6 @@ static int plan_create_sizes(hodlr_plan* plan, const hodlr_desc& d, const std::v
       rc = HODLR_ERR_HIP;
   }
   rc = rc ? rc : palloc(p, &p->Y, p->L > 0 ? p->N * r : 0);
-  p->wst = (int64_t)r * r;
-  rc = rc ? rc : palloc(p, &p->W, (int64_t)p->L * p->P * p->wst);
-  rc = rc ? rc : palloc(p, &p->Pq, p->L > 0 ? p->P * r * (r + (p->smax > 0 ? p->smax : 1)) : 0);
   int64_t so = 0, po = 0, dof = 0, io = 0;
   for (int t = 0; t < p->L; ++t) {
     const int64_t nn = (int64_t)1 << t;
@@ -691,14 +676,18 @@ static int plan_create_sizes(hodlr_plan* plan, const hodlr_desc& d, const std::v
     rc = rc ? rc : palloc(p, &p->schur.info, nn);
     rc = rc ? rc : palloc(p, &p->schur.retry, nn);
   }
-  // per-leaf projections / accumulators and core RHS for the widest
-  // operand: [Y | B] with nrhs_max columns, or a solve chunk of that width
+  // per-leaf projections / accumulators, core solutions and per-node sums
+  // for the widest operand: [Y | B] with nrhs_max columns, or a solve chunk
+  // of that width
   p->ncmax = r + (p->smax > 0 ? p->smax : 1);
   const int64_t pe = p->L > 0 ? p->P * r * p->ncmax : 0;
-  const int64_t re = p->L > 0 ? ((int64_t)1 << (p->L - 1)) * n2 * p->ncmax : 0;
-  for (int k = 0; k < 3; ++k) rc = rc ? rc : palloc(p, &p->Pb[k], pe);
+  const int64_t ee = p->L > 0 ? (p->P - 1) * n2 * p->ncmax : 0;
+  const int64_t he = p->L > 0 ? (2 * p->P - 1) * r * p->ncmax : 0;
+  rc = rc ? rc : palloc(p, &p->Pb, pe);
   rc = rc ? rc : palloc(p, &p->G, pe);
-  rc = rc ? rc : palloc(p, &p->R, re);
+  rc = rc ? rc : palloc(p, &p->E, ee);
+  rc = rc ? rc : palloc(p, &p->EB, ee);
+  rc = rc ? rc : palloc(p, &p->Hb, he);
   if (rc) {
     plan_free(p);
     delete p;
@@ -816,7 +805,6 @@ int hodlr_solve(hodlr_plan p, double* B, int64_t ldb, int nrhs, hipStream_t st) 
   if (check_ld(ldb, p->N)) return HODLR_ERR_ARG;
   DeviceGuard g(p->d.device);
   const int r = p->r, L = p->L;
-  const int64_t P = p->P;
   for (int c0 = 0; c0 < nrhs; c0 += p->ncmax) {
     const int nc = std::min(p->ncmax, nrhs - c0);
     double* Bc = B + (int64_t)c0 * ldb;
@@ -824,102 +812,71 @@ int hodlr_solve(hodlr_plan p, double* B, int64_t ldb, int nrhs, hipStream_t st) 
     int rc0 = leaf_solve(p, x, nc, false, st);
     if (rc0) return rc0;
     if (L == 0) continue;
-    // p_i = V_i^T B_i; per level the current projection is p_i - Q_i G_i
-    int rc = leaf_projections(p, p->V, p->ldv, x, nc, p->Pb[0], st);
+    // p_i = V_i^T B_i; each level's right-hand sides from the level below's
+    // core solutions (node-sum form), then G_i root first
+    int rc = leaf_projections(p, p->V, p->ldv, x, nc, p->Pb, st);
     if (rc) return rc;
-    const int64_t ps = (int64_t)r * nc;
+    const int n2 = 2 * r;
+    const int64_t bs = (int64_t)n2 * nc;
     for (int t = L - 1; t >= 0; --t) {
-      const int nnode = 1 << t, shift = L - 1 - t;
+      const int nnode = 1 << t;
       CoreBufs c = core_level(p, t);
-      const double* cur = p->Pb[0];
-      if (t < L - 1) {
-        hodlr::BGemmArgs h;
-        h.batch = (int)P; h.M = r; h.K = r; h.Nc = nc; h.csplit = nc;
-        h.A = p->Pq; h.lda = r; h.sa = (int64_t)r * p->nq;
-        h.B = p->G; h.ldb = r; h.bs1 = ps; h.bs2 = 2 * ps;
-        h.alpha1 = -1.0; h.beta1 = 1.0;
-        h.D1 = p->Pb[0]; h.ldd1 = r; h.sd1 = ps;
-        h.C1 = p->Pb[1]; h.ldc1 = r; h.sc1 = ps;
-        HIP_TRY(hodlr::launch_bgemm(h, st));
-        cur = p->Pb[1];
-      }
-      HIP_TRY(hodlr::launch_assemble(cur, 1 << shift, r, nc, 0, 0, nc, nnode, nullptr, c.R, st));
-      ColSeg rx = hodlr::seg_single(c.R, 2 * r);
-      HIP_TRY(hodlr::launch_getrs(2 * r, c.S, 2 * r, (int64_t)4 * r * r, nnode, c.perm, c.dinv,
-                                  rx, (int64_t)2 * r * nc, nc, st));
-      // G_i (+)= W_t,i eta_c
-      hodlr::BGemmArgs g2;
-      g2.batch = (int)P; g2.M = r; g2.K = r; g2.Nc = nc; g2.csplit = nc;
-      g2.A = (t == L - 1) ? nullptr : W_at(p, t); g2.lda = r; g2.sa = p->wst;
-      g2.B = c.R; g2.ldb = 2 * r; g2.bshift = shift; g2.bs2 = (int64_t)2 * r * nc; g2.bs1 = r;
-      g2.alpha1 = 1.0; g2.beta1 = (t == L - 1) ? 0.0 : 1.0;
-      g2.D1 = p->G; g2.ldd1 = r; g2.sd1 = ps;
-      g2.C1 = p->G; g2.ldc1 = r; g2.sc1 = ps;
-      HIP_TRY(hodlr::launch_bgemm(g2, st));
+      double* Rt = p->EB + heap0(t) * bs;
+      if (t == L - 1)
+        HIP_TRY(hodlr::launch_assemble(p->Pb, 1, r, nc, 0, 0, nc, nnode, nullptr, Rt, st));
+      else
+        HIP_TRY(hodlr::launch_assemble_pair(p->EB + heap0(t + 1) * bs, bs, r, 0, 0, nc, nnode,
+                                            nullptr, Rt, bs, st));
+      ColSeg rx = hodlr::seg_single(Rt, n2);
+      HIP_TRY(hodlr::launch_getrs(n2, c.S, n2, (int64_t)n2 * n2, nnode, c.perm, c.dinv, rx, bs, nc,
+                                  st));
     }
-    rc = final_update(p, p->Y, p->N, p->G, nc, Bc, ldb, st);
+    rc = path_down(p, p->E, (int64_t)n2 * p->nq, p->EB, bs, nc, st);
+    if (rc) return rc;
+    rc = final_update_leaf(p, p->G, nc, Bc, ldb, st);
     if (rc) return rc;
   }
   return HODLR_OK;
 }
 
-// Transpose of hodlr_solve's operator, levels root first, then K^-T:
-// with h_i = Y_i^T g_i and F_i the sum of the y halves applied so far,
-// u_t,i^T g_i = W_t,i^T (h_i - Q_i^T F_i); finally g_i -= V_i F_i.
+// Transpose of hodlr_solve's operator (node-sum form), then K^-T.  With
+// Hb_i = Y_i^T g_i at the leaves: bottom-up Hb_n = sum_c (I - E^Y_{n,c})^T
+// Hb_c (the transpose of the top-down pass); top-down each node's core
+// takes [Hb_c0 + Tb_n; Hb_c1 + Tb_n] (Tb_n: its half of the parent's
+// transposed core solve) through S_n^-T; the leaves' halves Tb_i give
+// g_i -= V_i Tb_i.
 int hodlr_solve_adjoint(hodlr_plan p, double* G, int64_t ldg, int nrhs, hipStream_t st) {
   if (!p || !G || nrhs < 0) return HODLR_ERR_ARG;
   if (!p->factored) return HODLR_ERR_STATE;
   if (check_ld(ldg, p->N)) return HODLR_ERR_ARG;
   DeviceGuard g(p->d.device);
-  const int r = p->r, L = p->L;
-  const int64_t P = p->P;
+  const int r = p->r, L = p->L, n2 = 2 * r;
+  const int64_t es = (int64_t)n2 * p->nq;
   for (int c0 = 0; c0 < nrhs; c0 += p->ncmax) {
     const int nc = std::min(p->ncmax, nrhs - c0);
     double* Gc = G + (int64_t)c0 * ldg;
     ColSeg x = hodlr::seg_single(Gc, ldg);
     if (L > 0) {
-      int rc = leaf_projections(p, p->Y, p->N, x, nc, p->Pb[0], st);
+      const int64_t hs = (int64_t)r * nc, bs = (int64_t)n2 * nc;
+      int rc = leaf_projections(p, p->Y, p->N, x, nc, p->Hb + heap0(L) * hs, st);
       if (rc) return rc;
-      const int64_t ps = (int64_t)r * nc;
+      for (int t = L - 1; t >= 1; --t)
+        HIP_TRY(hodlr::launch_adj_up(p->E + heap0(t) * es, es, p->Hb + heap0(t + 1) * hs, r, nc,
+                                     1 << t, p->Hb + heap0(t) * hs, st));
       for (int t = 0; t < L; ++t) {
-        const int nnode = 1 << t, shift = L - 1 - t;
+        const int nnode = 1 << t;
         CoreBufs c = core_level(p, t);
-        const double* u = p->Pb[0];
-        if (t > 0) {
-          hodlr::BGemmArgs h;
-          h.batch = (int)P; h.M = r; h.K = r; h.Nc = nc; h.csplit = nc; h.transA = 1;
-          h.A = p->Pq; h.lda = r; h.sa = (int64_t)r * p->nq;
-          h.B = p->G; h.ldb = r; h.bs1 = ps; h.bs2 = 2 * ps;
-          h.alpha1 = -1.0; h.beta1 = 1.0;
-          h.D1 = p->Pb[0]; h.ldd1 = r; h.sd1 = ps;
-          h.C1 = p->Pb[1]; h.ldc1 = r; h.sc1 = ps;
-          HIP_TRY(hodlr::launch_bgemm(h, st));
-          u = p->Pb[1];
-        }
-        hodlr::BGemmArgs z;
-        z.batch = (int)P; z.M = r; z.K = r; z.Nc = nc; z.csplit = nc; z.transA = 1;
-        z.A = (t == L - 1) ? nullptr : W_at(p, t); z.lda = r; z.sa = p->wst;
-        z.B = u; z.ldb = r; z.bs1 = ps; z.bs2 = 2 * ps;
-        z.alpha1 = 1.0; z.beta1 = 0.0;
-        z.C1 = p->Pb[2]; z.ldc1 = r; z.sc1 = ps;
-        HIP_TRY(hodlr::launch_bgemm(z, st));
-        HIP_TRY(hodlr::launch_assemble(p->Pb[2], 1 << shift, r, nc, 0, 0, nc, nnode, nullptr, c.R,
-                                       st, /*swap=*/false));
-        ColSeg rx = hodlr::seg_single(c.R, 2 * r);
-        HIP_TRY(hodlr::launch_getrs_t(2 * r, c.S, 2 * r, (int64_t)4 * r * r, nnode, c.perm,
-                                      c.dinv, rx, (int64_t)2 * r * nc, nc, st));
-        // F_i (+)= y of the other child's half
-        hodlr::BGemmArgs f;
-        f.batch = (int)P; f.M = r; f.K = r; f.Nc = nc; f.csplit = nc;
-        f.A = nullptr;
-        f.B = c.R; f.ldb = 2 * r; f.bshift = shift; f.bs2 = (int64_t)2 * r * nc; f.bs1 = r;
-        f.bflip = 1;
-        f.alpha1 = 1.0; f.beta1 = (t == 0) ? 0.0 : 1.0;
-        f.D1 = p->G; f.ldd1 = r; f.sd1 = ps;
-        f.C1 = p->G; f.ldc1 = r; f.sc1 = ps;
-        HIP_TRY(hodlr::launch_bgemm(f, st));
+        double* Rt = p->EB + heap0(t) * bs;
+        HIP_TRY(hodlr::launch_adj_assemble(p->Hb + heap0(t + 1) * hs,
+                                           t > 0 ? p->EB + heap0(t - 1) * bs : nullptr, bs, r, nc,
+                                           nnode, Rt, bs, st));
+        ColSeg rx = hodlr::seg_single(Rt, n2);
+        HIP_TRY(hodlr::launch_getrs_t(n2, c.S, n2, (int64_t)n2 * n2, nnode, c.perm, c.dinv, rx,
+                                      bs, nc, st));
       }
-      rc = final_update(p, p->V, p->ldv, p->G, nc, Gc, ldg, st);
+      // leaf c's Tb: rows [r, 2r) of its parent's solution for an even c,
+      // [0, r) for an odd one
+      rc = final_update(p, p->V, p->ldv, p->EB + heap0(L - 1) * bs, n2, bs, r, 1, nc, Gc, ldg, st);
       if (rc) return rc;
     }
     int rc1 = leaf_solve(p, x, nc, true, st);

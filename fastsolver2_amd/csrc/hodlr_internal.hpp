@@ -136,6 +136,17 @@ int64_t reduce_chunks_per_child(int64_t n_child, int64_t rc);
 //   R_k = [H_1; H_0]       (2r x nh, ld 2r)  with H_c = part cols [hcol, hcol+nh)
 // (R_k = [H_0; H_1] when swap is false: the adjoint's u_c^T g_c stacking).
 // S may be null (solve with a stored core factorization).
+// The level loop in node-sum form (kernels in hodlr_kernels.hip): assembly of
+// a level from the level below's core solutions, the top-down pass that
+// forms G_i for the leaves [i0, i0 + count), and the adjoint's two passes.
+hipError_t launch_assemble_pair(const double* Ec, int64_t ecs, int r, int gcol, int hcol, int nh,
+                                int nnode, double* S, double* R, int64_t rs, hipStream_t st);
+hipError_t launch_path_down(const double* EY, int64_t eys, const double* EB, int64_t ebs, int L,
+                            int r, int s, int64_t i0, int64_t count, double* G, hipStream_t st);
+hipError_t launch_adj_up(const double* EY, int64_t eys, const double* Hc, int r, int s, int nnode,
+                         double* Hn, hipStream_t st);
+hipError_t launch_adj_assemble(const double* Hc, const double* Rp, int64_t rps, int r, int s,
+                               int nnode, double* R, int64_t rs, hipStream_t st);
 hipError_t launch_assemble(const double* part, int cpc, int r, int ncols_part, int gcol,
                            int hcol, int nh, int nnode, double* S, double* R, hipStream_t st,
                            bool swap = true);
@@ -228,7 +239,7 @@ hipError_t launch_fill_scale(int64_t rows, int cols, double* A, int64_t ld, int 
                              hipStream_t st);
 // Batched small GEMM over leaves, see k_bgemm.
 struct BGemmArgs {
-  int batch = 0, M = 0, K = 0, Nc = 0, csplit = 0, transA = 0, bshift = 0, bflip = 0;
+  int batch = 0, M = 0, K = 0, Nc = 0, csplit = 0, transA = 0, bshift = 0, bflip = 0, ioff = 0;
   const double* A = nullptr;
   int64_t lda = 0, sa = 0;
   const double* B = nullptr;

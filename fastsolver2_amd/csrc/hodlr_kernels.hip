@@ -3357,6 +3357,241 @@ hipError_t launch_assemble(const double* part, int cpc, int r, int ncols_part, i
 }
 
 // =====================================================================
+// The level loop in node-sum form.  Node n of level t solves
+//   S_n E_n = R_n,  S_n = [I, T_{c1}^Y; T_{c0}^Y, I],  R_n = [T_{c1}; T_{c0}]
+// (c0 = 2n, c1 = 2n + 1 its children, T_c the sums over c's leaves of the
+// current projections V_i^T [Y_i | B_i]).  The per-leaf update of a level,
+// P_i <- P_i - P_i^Y E_{n,c(i)}, is linear in P_i, so a child's updated sum
+// is T_c - T_c^Y E_{n,c}, and the rows of S_n E_n = R_n say exactly
+//   T_{c0} - T_{c0}^Y E_n(0:r) = E_n(r:2r),  T_{c1} - T_{c1}^Y E_n(r:2r) = E_n(0:r):
+// the sum a node hands to its parent is the sum of the two halves of its
+// own core solution.  Every level above the leaves is therefore assembled
+// from the level below's solutions alone, and the per-leaf products
+// (G_i = sum_t W_t,i E^B_t, W_{t-1} = W_t (I - E^Y_t)) are evaluated once,
+// root first, along each leaf's path.  E of node (t, n) is stored at heap
+// index 2^t - 1 + n.
+// =====================================================================
+
+// Core assembly from the level below: T_c = E_c(0:r, :) + E_c(r:2r, :)
+// (E_c at Ec + c*ecs, ld 2r); node n's S = [I, T_{2n+1}(:, gcol..); T_{2n}
+// (:, gcol..), I] (skipped when S is null) and R = [T_{2n+1}; T_{2n}]
+// (columns hcol .. hcol + nh; at R + n*rs, ld 2r).
+__global__ void __launch_bounds__(256)
+k_assemble_pair(const double* __restrict__ Ec, int64_t ecs, int r, int gcol, int hcol, int nh,
+                double* __restrict__ S, double* __restrict__ R, int64_t rs) {
+  const int node = blockIdx.y;
+  const int n2 = 2 * r;
+  const int64_t nS = S ? (int64_t)n2 * n2 : 0;
+  const int64_t total = nS + (int64_t)n2 * nh;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * 256) {
+    int c, a, col;
+    double* dst;
+    if (e < nS) {
+      const int i = (int)(e % n2), j = (int)(e / n2);
+      dst = S + (int64_t)node * nS + i + (int64_t)j * n2;
+      if ((i < r) == (j < r)) {
+        *dst = (i == j) ? 1.0 : 0.0;
+        continue;
+      }
+      if (i >= r) { c = 2 * node; a = i - r; col = gcol + j; }           // S(r+a, j) = T_c0^Y
+      else        { c = 2 * node + 1; a = i; col = gcol + j - r; }       // S(a, r+j) = T_c1^Y
+    } else {
+      const int64_t f = e - nS;
+      const int i = (int)(f % n2), j = (int)(f / n2);
+      const int hi = i >= r;
+      dst = R + (int64_t)node * rs + i + (int64_t)j * n2;
+      c = 2 * node + 1 - hi; a = i - hi * r; col = hcol + j;
+    }
+    const double* ep = Ec + (int64_t)c * ecs + a + (int64_t)col * n2;
+    *dst = ep[0] + ep[r];
+  }
+}
+
+hipError_t launch_assemble_pair(const double* Ec, int64_t ecs, int r, int gcol, int hcol, int nh,
+                                int nnode, double* S, double* R, int64_t rs, hipStream_t st) {
+  if (nnode <= 0) return hipSuccess;
+  const int64_t total = (S ? 4LL * r * r : 0) + 2LL * r * nh;
+  int gx = (int)((total + 255) / 256);
+  if (gx > 256) gx = 256;
+  if (gx < 1) gx = 1;
+  hipLaunchKernelGGL(k_assemble_pair, dim3(gx, nnode), dim3(256), 0, st, Ec, ecs, r, gcol, hcol,
+                     nh, S, R, rs);
+  return hipGetLastError();
+}
+
+// Top-down pass, one workgroup per leaf slot i (leaves i0, i0 + 1, ...):
+//   H = 0;  for t = 0 .. L-1:  H <- E^B + H - E^Y H;   G_i = H
+// with E^Y (r x r) and E^B (r x s) the rows of node (t, i >> (L-t))'s core
+// solution that belong to its child holding leaf i (rows [0, r) for the
+// even child, [r, 2r) for the odd one): G_i = sum_t W_t,i E^B_t evaluated
+// by Horner from the root.  Node g's blocks at EY + g*eys and EB + g*ebs
+// (ld 2r); G_i at G + i*r*s (ld r).  H lives in LDS, double-buffered, in
+// chunks of cw columns (cw * (r + ...) sized by the launcher); the E^Y
+// fragments of a row tile (up to 128 k) are loaded together before its
+// MFMA chain.
+__global__ void __launch_bounds__(256)
+k_path_down(const double* __restrict__ EY, int64_t eys, const double* __restrict__ EB,
+            int64_t ebs, int L, int r, int s, int cw, int64_t i0, double* __restrict__ G) {
+  extern __shared__ double hsm[];
+  const int64_t i = i0 + blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int n2 = 2 * r, mt = (r + 15) >> 4;
+  const int hld = cw + 1;  // LDS row stride of H (row-major, r x cw)
+  double* Hb[2] = {hsm, hsm + (int64_t)r * hld};
+  for (int c0 = 0; c0 < s; c0 += cw) {
+    const int sc = min(cw, s - c0);
+    for (int e = tid; e < r * hld; e += 256) Hb[0][e] = 0.0;
+    __syncthreads();
+    int cur = 0;
+    for (int t = 0; t < L; ++t) {
+      const int64_t ch = i >> (L - 1 - t);
+      const int64_t g = ((int64_t)1 << t) - 1 + (ch >> 1);
+      const int64_t hoff = (ch & 1) * (int64_t)r;
+      const double* ey = EY + g * eys + hoff;
+      const double* eb = EB + g * ebs + hoff + (int64_t)c0 * n2;
+      const double* hc = Hb[cur];
+      double* hn = Hb[cur ^ 1];
+      for (int tm = wid; tm < mt; tm += 4) {
+        const int arow = tm * 16 + lr;
+        double bv[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int orow = tm * 16 + crow(lane, v);
+          bv[v] = (orow < r && lr < sc) ? eb[orow + (int64_t)lr * n2] : 0.0;
+        }
+        d4 acc = d4zero();
+        for (int kb = 0; kb < r; kb += 128) {
+          double af[32];
+#pragma unroll
+          for (int q = 0; q < 32; ++q) {
+            const int k = kb + q * 4 + lk;
+            af[q] = (arow < r && k < r) ? ey[arow + (int64_t)k * n2] : 0.0;
+          }
+#pragma unroll
+          for (int q = 0; q < 32; ++q) {
+            if (kb + q * 4 < r) {  // wave-uniform
+              const int k = kb + q * 4 + lk;
+              const double hv = (k < r && lr < cw) ? hc[k * hld + lr] : 0.0;
+              acc = mfma(af[q], hv, acc);
+            }
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int orow = tm * 16 + crow(lane, v);
+          if (orow < r && lr < cw) hn[orow * hld + lr] = bv[v] + hc[orow * hld + lr] - acc[v];
+        }
+      }
+      __syncthreads();
+      cur ^= 1;
+    }
+    for (int e = tid; e < r * sc; e += 256) {
+      const int a = e % r, j = e / r;
+      G[i * r * s + a + (int64_t)(c0 + j) * r] = Hb[cur][a * hld + j];
+    }
+    __syncthreads();
+  }
+}
+
+// LDS of k_path_down: two r x (cw + 1) blocks, cw = 16, 8 or 4 so that they
+// stay within 64 KiB.
+static int path_cw(int r) {
+  for (int cw = 16; cw >= 4; cw >>= 1)
+    if ((size_t)2 * r * (cw + 1) * sizeof(double) <= 65536) return cw;
+  return 0;
+}
+
+hipError_t launch_path_down(const double* EY, int64_t eys, const double* EB, int64_t ebs, int L,
+                            int r, int s, int64_t i0, int64_t count, double* G, hipStream_t st) {
+  if (count <= 0 || s <= 0 || L <= 0) return hipSuccess;
+  const int cw = path_cw(r);
+  if (cw == 0) return hipErrorInvalidValue;
+  const size_t lds = (size_t)2 * r * (cw + 1) * sizeof(double);
+  hipLaunchKernelGGL(k_path_down, dim3((unsigned)count), dim3(256), lds, st, EY, eys, EB, ebs, L, r,
+                     s, cw, i0, G);
+  return hipGetLastError();
+}
+
+// Adjoint of the top-down pass, one level (one workgroup per node n):
+//   Hn_n = Hc_{2n} + Hc_{2n+1} - (E_n^Y)^T [Hc_{2n}; Hc_{2n+1}]
+// E_n^Y: the first r columns of node n's core solution (2r x r at
+// EY + n*eys, ld 2r); Hc_c at Hc + c*r*s and Hn_n at Hn + n*r*s (r x s,
+// ld r).
+__global__ void __launch_bounds__(256)
+k_adj_up(const double* __restrict__ EY, int64_t eys, const double* __restrict__ Hc, int r, int s,
+         double* __restrict__ Hn) {
+  const int64_t n = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int n2 = 2 * r;
+  const int64_t hs = (int64_t)r * s;
+  const double* ey = EY + n * eys;
+  const double* h0 = Hc + 2 * n * hs;
+  const double* h1 = h0 + hs;
+  double* out = Hn + n * hs;
+  const int mt = (r + 15) >> 4, nt = (s + 15) >> 4;
+  for (int task = wid; task < mt * nt; task += 4) {
+    const int tm = task % mt, tn = task / mt;
+    const int arow = tm * 16 + lr, bcol = tn * 16 + lr;
+    d4 acc = d4zero();
+    for (int k0 = 0; k0 < n2; k0 += 4) {
+      const int k = k0 + lk;
+      const double av = (arow < r && k < n2) ? ey[k + (int64_t)arow * n2] : 0.0;
+      double bv = 0.0;
+      if (bcol < s && k < n2) bv = k < r ? h0[k + (int64_t)bcol * r] : h1[k - r + (int64_t)bcol * r];
+      acc = mfma(av, bv, acc);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int orow = tm * 16 + crow(lane, v), col = tn * 16 + lr;
+      if (orow < r && col < s) {
+        const int64_t o = orow + (int64_t)col * r;
+        out[o] = h0[o] + h1[o] - acc[v];
+      }
+    }
+  }
+}
+
+hipError_t launch_adj_up(const double* EY, int64_t eys, const double* Hc, int r, int s, int nnode,
+                         double* Hn, hipStream_t st) {
+  if (nnode <= 0 || s <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_adj_up, dim3((unsigned)nnode), dim3(256), 0, st, EY, eys, Hc, r, s, Hn);
+  return hipGetLastError();
+}
+
+// Adjoint of the bottom-up assembly, one level: node n's right-hand side
+//   [Hc_{2n} + Tbar_n; Hc_{2n+1} + Tbar_n]   (2r x s at R + n*rs, ld 2r)
+// where Tbar_n is the half of the parent's solution that the parent's R
+// took from node n: rows [r, 2r) of Rp_{n/2} for an even n, [0, r) for an
+// odd one (Rp null at the root: Tbar = 0).  Hc_c at Hc + c*r*s (ld r).
+__global__ void __launch_bounds__(256)
+k_adj_assemble(const double* __restrict__ Hc, const double* __restrict__ Rp, int64_t rps, int r,
+               int s, double* __restrict__ R, int64_t rs) {
+  const int64_t node = blockIdx.y;
+  const int n2 = 2 * r;
+  const int64_t total = (int64_t)n2 * s;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * 256) {
+    const int i = (int)(e % n2), j = (int)(e / n2);
+    const int hi = i >= r, a = i - hi * r;
+    double v = Hc[(2 * node + hi) * (int64_t)r * s + a + (int64_t)j * r];
+    if (Rp) v += Rp[(node >> 1) * rps + (1 - (node & 1)) * (int64_t)r + a + (int64_t)j * n2];
+    R[node * rs + i + (int64_t)j * n2] = v;
+  }
+}
+
+hipError_t launch_adj_assemble(const double* Hc, const double* Rp, int64_t rps, int r, int s,
+                               int nnode, double* R, int64_t rs, hipStream_t st) {
+  if (nnode <= 0 || s <= 0) return hipSuccess;
+  int gx = (int)((2LL * r * s + 255) / 256);
+  if (gx > 256) gx = 256;
+  hipLaunchKernelGGL(k_adj_assemble, dim3(gx, nnode), dim3(256), 0, st, Hc, Rp, rps, r, s, R, rs);
+  return hipGetLastError();
+}
+
+// =====================================================================
 // Level update ("gemmBro", gemm_broadcast.cc:58-82):
 //   X_c[:, j] += alpha * u_c * eta_c[:, j]
 // 64-row groups that never straddle a child (one 16-row MFMA tile per
@@ -4042,7 +4277,7 @@ __global__ void k_copy_cols(int64_t rows, int cols, const double* __restrict__ s
 // written to (C1, D1, alpha1, beta1) for j < csplit and to (C2, D2,
 // alpha2, beta2) at column j - csplit otherwise; a null C skips its
 // columns.  A_i = A + i*sa (M x K, or K x M read transposed; null A is the
-// identity; a null D with a nonzero beta is the identity too); B_i = B + (c>>1)*bs2 + ((c&1)^bflip)*bs1 with c = i >> bshift,
+// identity; a null D with a nonzero beta is the identity too); B_i = B + (c>>1)*bs2 + ((c&1)^bflip)*bs1 with c = (i + ioff) >> bshift,
 // which selects the half of a node's core solution that belongs to the
 // leaf's child at a given level.  16 x 16 output tiles are dealt to the
 // four waves.  D may alias C; A and B must not.
@@ -4056,7 +4291,7 @@ __global__ void __launch_bounds__(256) k_bgemm(BGemmArgs g0, BGemmArgs g1) {
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int lr = lane & 15, lk = lane >> 4;
   const double* A = g.A ? g.A + (int64_t)i * g.sa : nullptr;
-  const int c = i >> g.bshift;
+  const int c = (i + g.ioff) >> g.bshift;
   const double* Bp = g.B + (int64_t)(c >> 1) * g.bs2 + (int64_t)((c & 1) ^ g.bflip) * g.bs1;
   const int mt = (g.M + 15) >> 4, nt = (g.Nc + 15) >> 4;
   // each wave task covers GM x GN tiles of 16 x 16: GM*GN independent MFMA
