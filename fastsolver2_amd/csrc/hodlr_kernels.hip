@@ -1141,21 +1141,29 @@ __device__ __forceinline__ void wave_sync() {
 
 // Unblocked LU without interchanges of a 16x16 block held by lanes 0..15
 // (lane i: row i in a[]), dgetf2's operation order; every multiplier is
-// checked against thr.  Row j is published through Prow (16-byte LDS
-// stores) once it is final and read back by every lane before use.
-template <typename T>
+// checked against thr.  Row j, final after step j - 1, reaches every lane
+// either from lane j's registers by readlane (RL: register-to-scalar moves,
+// no LDS round trip on the serial chain; it halves the step, but its scalar
+// registers change the allocation of a kernel it is inlined into) or
+// through Prow (16-byte LDS stores by lane j, read back after a wave sync).
+template <typename T, bool RL>
 __device__ __forceinline__ void ll_lu16(T (&a)[16], int lane, T* Prow, T thr, bool& fail) {
   const bool own = lane < 16;
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
-    if (lane == j) {
-#pragma unroll
-      for (int k = j; k < 16; ++k) Prow[k] = a[k];
-    }
-    wave_sync();
     T u[16];
+    if (RL) {
 #pragma unroll
-    for (int k = j; k < 16; ++k) u[k] = Prow[k];
+      for (int k = j; k < 16; ++k) u[k] = Elem<T>::lane(a[k], j);
+    } else {
+      if (lane == j) {
+#pragma unroll
+        for (int k = j; k < 16; ++k) Prow[k] = a[k];
+      }
+      wave_sync();
+#pragma unroll
+      for (int k = j; k < 16; ++k) u[k] = Prow[k];
+    }
     const T r = recip_nr(u[j]);
     if (own && lane > j) {
       const T l = a[j] * r;
@@ -1164,7 +1172,7 @@ __device__ __forceinline__ void ll_lu16(T (&a)[16], int lane, T* Prow, T thr, bo
 #pragma unroll
       for (int k = j + 1; k < 16; ++k) a[k] -= l * u[k];
     }
-    wave_sync();
+    if (!RL) wave_sync();
   }
 }
 
@@ -1213,7 +1221,7 @@ __device__ __forceinline__ void ll_inv16(T (*Mt)[16][17], T (*Li)[33], T (*Ui)[3
 //   A11 = L1 U1;  L21 = A21 U1^-1;  U12 = L1^-1 A12;  A22 - L21 U12 = L2 U2;
 //   L^-1 = [L1^-1, 0; -L2^-1 (L21 L1^-1), L2^-1],
 //   U^-1 = [U1^-1, -(U1^-1 U12) U2^-1; 0, U2^-1].
-template <typename T>
+template <typename T, bool RL>
 __device__ __forceinline__ void ll_diag_step(T (*Ds)[33], T (*As12)[33], T (*Li)[33],
                                              T (*Ui)[33], T (*Ms)[32][33], T* Prow,
                                              T* __restrict__ out, bool with_a12, T thr,
@@ -1232,7 +1240,7 @@ __device__ __forceinline__ void ll_diag_step(T (*Ds)[33], T (*As12)[33], T (*Li)
     T a[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) a[k] = Ds[o + i][o + k];
-    ll_lu16<T>(a, lane, Prow, thr, fail);
+    ll_lu16<T, RL>(a, lane, Prow, thr, fail);
     if (lane < 16) {
 #pragma unroll
       for (int k = 0; k < 16; ++k) {
@@ -1533,7 +1541,7 @@ k_getrf_ll(const double* __restrict__ K0, int64_t ldk, int64_t kbs, T* __restric
         // compiler barrier: the parking stores stay here (sunk into the
         // step they would keep the accumulators live)
         asm volatile("" ::: "memory");
-        ll_diag_step<T>(Ds, As12, Li, Ui, Ms, Prow, dinv + (int64_t)ib * 2 * kTri * kTri, p == 0,
+        ll_diag_step<T, false>(Ds, As12, Li, Ui, Ms, Prow, dinv + (int64_t)ib * 2 * kTri * kTri, p == 0,
                         thr, fail, lane);
         asm volatile("" ::: "memory");
 #pragma unroll
@@ -1964,7 +1972,6 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
   __shared__ double As[64 * 65];  // A, [k*65 + col]; then the fast path's packed LU
   // the no-interchange attempt's 32x32 tiles (ll_diag_step)
   __shared__ double Dt[32][33], A12t[32][33], Lit[32][33], Uit[32][33], Mst[2][32][33];
-  __shared__ __align__(16) double Prow[32];
   __shared__ unsigned long long wmk[4];
   const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
   const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -2054,7 +2061,7 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
       if (rp == 64) A12t[i][j] = cpad(i, 32 + j);
     }
     wave_sync();
-    ll_diag_step<double>(Dt, A12t, Lit, Uit, Mst, Prow, inv ? dv + (int64_t)(r / 32) * 2 * kTri * kTri : nullptr,
+    ll_diag_step<double, true>(Dt, A12t, Lit, Uit, Mst, nullptr, inv ? dv + (int64_t)(r / 32) * 2 * kTri * kTri : nullptr,
                          rp == 64, thr, fail, lane);
     wave_sync();
 #pragma unroll
@@ -2097,7 +2104,7 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
           for (int v = 0; v < 4; ++v) Dt[16 * rt + Elem<double>::crow(lane, v)][16 * ct + lr] = c[v];
         }
       wave_sync();
-      ll_diag_step<double>(Dt, A12t, Lit, Uit, Mst, Prow,
+      ll_diag_step<double, true>(Dt, A12t, Lit, Uit, Mst, nullptr,
                            inv ? dv + (int64_t)(r / 32 + 1) * 2 * kTri * kTri : nullptr, false,
                            thr, fail, lane);
       wave_sync();
