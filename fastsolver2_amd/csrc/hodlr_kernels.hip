@@ -3502,6 +3502,82 @@ k_path_down(const double* __restrict__ EY, int64_t eys, const double* __restrict
   }
 }
 
+// k_path_down for r <= 64 (one 16-row tile per wave, the inner dimension
+// in 16 k-steps): the next level's E^Y fragments and E^B values are loaded
+// under the current level's MFMA chain, and the level barrier orders LDS
+// only, so those loads stay in flight across it.  Same arithmetic and order
+// as k_path_down.
+__global__ void __launch_bounds__(256)
+k_path_down64(const double* __restrict__ EY, int64_t eys, const double* __restrict__ EB,
+              int64_t ebs, int L, int r, int s, int cw, int64_t i0, double* __restrict__ G) {
+  extern __shared__ double hsm[];
+  const int64_t i = i0 + blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int n2 = 2 * r, mt = (r + 15) >> 4;
+  const int hld = cw + 1;
+  double* Hb[2] = {hsm, hsm + (int64_t)r * hld};
+  const bool act = wid < mt;
+  const int arow = wid * 16 + lr;
+  for (int c0 = 0; c0 < s; c0 += cw) {
+    const int sc = min(cw, s - c0);
+    auto fetch = [&](int t, double (&af)[16], double (&bv)[4]) {
+      const int64_t ch = i >> (L - 1 - t);
+      const int64_t g = ((int64_t)1 << t) - 1 + (ch >> 1);
+      const int64_t hoff = (ch & 1) * (int64_t)r;
+      const double* ey = EY + g * eys + hoff;
+      const double* eb = EB + g * ebs + hoff + (int64_t)c0 * n2;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const int k = q * 4 + lk;
+        af[q] = (act && arow < r && k < r) ? ey[arow + (int64_t)k * n2] : 0.0;
+      }
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int orow = wid * 16 + crow(lane, v);
+        bv[v] = (act && orow < r && lr < sc) ? eb[orow + (int64_t)lr * n2] : 0.0;
+      }
+    };
+    for (int e = tid; e < r * hld; e += 256) Hb[0][e] = 0.0;
+    double af[16], bv[4], an[16], bn[4];
+    fetch(0, af, bv);
+    __syncthreads();
+    int cur = 0;
+    for (int t = 0; t < L; ++t) {
+      if (t + 1 < L) fetch(t + 1, an, bn);
+      const double* hc = Hb[cur];
+      double* hn = Hb[cur ^ 1];
+      if (act) {
+        d4 acc = d4zero();
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          if (q * 4 < r) {  // wave-uniform
+            const int k = q * 4 + lk;
+            const double hv = (k < r && lr < cw) ? hc[k * hld + lr] : 0.0;
+            acc = mfma(af[q], hv, acc);
+          }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int orow = wid * 16 + crow(lane, v);
+          if (orow < r && lr < cw) hn[orow * hld + lr] = bv[v] + hc[orow * hld + lr] - acc[v];
+        }
+      }
+      lds_barrier();
+      cur ^= 1;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) af[q] = an[q];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) bv[v] = bn[v];
+    }
+    for (int e = tid; e < r * sc; e += 256) {
+      const int a = e % r, j = e / r;
+      G[i * r * s + a + (int64_t)(c0 + j) * r] = Hb[cur][a * hld + j];
+    }
+    __syncthreads();
+  }
+}
+
 // LDS of k_path_down: two r x (cw + 1) blocks, cw = 16, 8 or 4 so that they
 // stay within 64 KiB.
 static int path_cw(int r) {
@@ -3516,8 +3592,12 @@ hipError_t launch_path_down(const double* EY, int64_t eys, const double* EB, int
   const int cw = path_cw(r);
   if (cw == 0) return hipErrorInvalidValue;
   const size_t lds = (size_t)2 * r * (cw + 1) * sizeof(double);
-  hipLaunchKernelGGL(k_path_down, dim3((unsigned)count), dim3(256), lds, st, EY, eys, EB, ebs, L, r,
-                     s, cw, i0, G);
+  if (r <= 64)
+    hipLaunchKernelGGL(k_path_down64, dim3((unsigned)count), dim3(256), lds, st, EY, eys, EB, ebs,
+                       L, r, s, cw, i0, G);
+  else
+    hipLaunchKernelGGL(k_path_down, dim3((unsigned)count), dim3(256), lds, st, EY, eys, EB, ebs, L,
+                       r, s, cw, i0, G);
   return hipGetLastError();
 }
 
