@@ -177,6 +177,25 @@ struct CoreBufs {
 // is factored in place, R (n2 x nh per node) overwritten by S^-1 R.
 double lu_flops(double n) { return 2.0 / 3.0 * n * n * n; }
 
+// R <- S^-1 R for the factored cores of one level (R: n2 x nh per node at
+// stride rs, ld n2): the structured solve where the cores were factored by
+// the structured LU (their flags in c.general; r = 32 or 64) plus the
+// generic solve masked to the cores it flagged general, else the generic
+// solve.
+int core_solve(int nnode, int n2, const CoreBufs& c, double* R, int64_t rs, int nh,
+               hipStream_t st) {
+  const int r = n2 / 2;
+  ColSeg x = hodlr::seg_single(R, n2);
+  if (c.general && (n2 & 1) == 0 && hodlr::core_solve_structured(r)) {
+    HIP_TRY(hodlr::launch_core_solve_s(r, c.S, nnode, c.perm, c.dinv, c.general, R, rs, nh, st));
+    HIP_TRY(hodlr::launch_getrs_masked(n2, c.S, n2, (int64_t)n2 * n2, nnode, c.perm, c.dinv, x, rs,
+                                       nh, c.general, st));
+    return HODLR_OK;
+  }
+  HIP_TRY(hodlr::launch_getrs(n2, c.S, n2, (int64_t)n2 * n2, nnode, c.perm, c.dinv, x, rs, nh, st));
+  return HODLR_OK;
+}
+
 int core_factor_solve_n(int nnode, int n2, const CoreBufs& c, int nh, hipStream_t st) {
   const double blk = 8.0 * n2 * n2 * nnode;
   {
@@ -199,9 +218,7 @@ int core_factor_solve_n(int nnode, int n2, const CoreBufs& c, int nh, hipStream_
   if (nh > 0) {
     ProfScope ps(HODLR_PROF_GETRS_CORE, st, 2.0 * n2 * n2 * nh * nnode,
                  blk + 16.0 * n2 * nh * nnode);
-    ColSeg x = hodlr::seg_single(c.R, n2);
-    HIP_TRY(hodlr::launch_getrs(n2, c.S, n2, (int64_t)n2 * n2, nnode, c.perm, c.dinv, x,
-                                (int64_t)n2 * nh, nh, st));
+    return core_solve(nnode, n2, c, c.R, (int64_t)n2 * nh, nh, st);
   }
   return HODLR_OK;
 }
@@ -250,7 +267,7 @@ struct hodlr_plan_s {
   int* core_perm = nullptr;
   int* core_info = nullptr;
   double* core_dinv = nullptr;
-  int* core_general = nullptr;  // flags of the structured core LU (largest level)
+  int* core_general = nullptr;  // flags of the structured core LU, every level (heap order)
   int* core_tmask = nullptr;    // cores left without diagonal inverses (largest level)
   hodlr::CoreSchur schur;       // Schur-complement scratch for ranks 64 < r <= 128
   std::vector<int64_t> s_off, p_off, d_off, i_off;
@@ -365,6 +382,9 @@ double leaf_pow_sum(hodlr_plan p, int k) {
   return s;
 }
 
+// first heap index of level t (level L: the leaves)
+int64_t heap0(int t) { return ((int64_t)1 << t) - 1; }
+
 CoreBufs core_level(hodlr_plan p, int t) {
   CoreBufs c;
   c.S = p->coreS + p->s_off[t];
@@ -373,7 +393,7 @@ CoreBufs core_level(hodlr_plan p, int t) {
   c.perm = p->core_perm + p->p_off[t];
   c.info = p->core_info + p->i_off[t];
   c.dinv = p->core_dinv + p->d_off[t];
-  c.general = p->core_general;
+  c.general = p->core_general + heap0(t);
   c.tmask = p->core_tmask;
   c.schur = p->schur.C ? &p->schur : nullptr;
   return c;
@@ -419,9 +439,6 @@ int leaf_phase(hodlr_plan p, const double* K, int64_t ldk, const double* U, int6
   }
   return HODLR_OK;
 }
-
-// first heap index of level t (level L: the leaves)
-int64_t heap0(int t) { return ((int64_t)1 << t) - 1; }
 
 // Per-leaf projections part_i = V_i^T X_i (r x nc, ld r, stride r*nc).
 int leaf_projections(hodlr_plan p, const double* Vm, int64_t ldv, const ColSeg& x, int nc,
@@ -664,7 +681,7 @@ static int plan_create_sizes(hodlr_plan* plan, const hodlr_desc& d, const std::v
   rc = rc ? rc : palloc(p, &p->core_perm, po);
   rc = rc ? rc : palloc(p, &p->core_info, io > 0 ? io : 1);
   rc = rc ? rc : palloc(p, &p->core_dinv, dof);
-  rc = rc ? rc : palloc(p, &p->core_general, p->L > 0 ? ((int64_t)1 << (p->L - 1)) : 0);
+  rc = rc ? rc : palloc(p, &p->core_general, p->L > 0 ? p->P - 1 : 0);
   rc = rc ? rc : palloc(p, &p->core_tmask, p->L > 0 ? ((int64_t)1 << (p->L - 1)) : 0);
   if (p->L > 0 && r > 64 && r <= 128) {
     const int64_t nn = (int64_t)1 << (p->L - 1);
@@ -827,9 +844,8 @@ int hodlr_solve(hodlr_plan p, double* B, int64_t ldb, int nrhs, hipStream_t st) 
       else
         HIP_TRY(hodlr::launch_assemble_pair(p->EB + heap0(t + 1) * bs, bs, r, 0, 0, nc, nnode,
                                             nullptr, Rt, bs, st));
-      ColSeg rx = hodlr::seg_single(Rt, n2);
-      HIP_TRY(hodlr::launch_getrs(n2, c.S, n2, (int64_t)n2 * n2, nnode, c.perm, c.dinv, rx, bs, nc,
-                                  st));
+      rc = core_solve(nnode, n2, c, Rt, bs, nc, st);
+      if (rc) return rc;
     }
     rc = path_down(p, p->E, (int64_t)n2 * p->nq, p->EB, bs, nc, st);
     if (rc) return rc;

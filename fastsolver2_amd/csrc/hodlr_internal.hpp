@@ -119,6 +119,18 @@ bool profiling_enabled();
 // no factors until its graph has been launched.
 void plan_mark_unfactored(hodlr_plan_s* p);
 void plan_mark_factored(hodlr_plan_s* p);
+// The solve restricted to the blocks b with only[b] != 0 (blocks of up to
+// 512 rows).
+hipError_t launch_getrs_masked(int n, const double* LU, int64_t lda, int64_t bstride, int batch,
+                               const int* perm, const double* dinv, const ColSeg& X,
+                               int64_t xbstride, int ncols, const int* only, hipStream_t st);
+// Core solve with the structured LU of S = [I, A; B, I] (ld 2r, stride 4r^2,
+// r = 32 or 64: core_solve_structured); cores flagged in `general` are
+// skipped (launch_getrs_masked takes them).
+bool core_solve_structured(int r);
+hipError_t launch_core_solve_s(int r, const double* S, int batch, const int* perm,
+                               const double* dinv, const int* general, double* R, int64_t rs,
+                               int nc, hipStream_t st);
 hipError_t launch_getrs_t(int n, const double* LU, int64_t lda, int64_t bstride, int batch,
                           const int* perm, const double* dinv, const ColSeg& X,
                           int64_t xbstride, int ncols, hipStream_t st);

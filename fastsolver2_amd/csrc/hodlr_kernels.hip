@@ -2677,7 +2677,7 @@ template <int NCT, bool TRANS, typename T>
 __global__ void __launch_bounds__(512)
 k_getrs_reg(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
             const int* __restrict__ perm0, const T* __restrict__ dinv0, ColSeg X,
-            int64_t xbstride, int ncols, int nchunk) {
+            int64_t xbstride, int ncols, int nchunk, const int* __restrict__ only) {
   constexpr int W = NCT * 16;
   typedef typename Elem<T>::V4 V4;
   __shared__ T xa[kTri * (W + 1)];  // raw block (owners -> all)
@@ -2691,6 +2691,7 @@ k_getrs_reg(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
   const int xcd = blockIdx.x & 7;
   const int logical = xcd * gq + min(xcd, grem) + (blockIdx.x >> 3);
   const int b = logical / nchunk, chunk = logical % nchunk;
+  if (only && !only[b]) return;  // (optional) blocks b with only[b] != 0
   const int c0 = chunk * W;
   const T* LU = LU0 + (int64_t)b * bstride;
   const int* perm = perm0 + (int64_t)b * n;
@@ -2862,10 +2863,11 @@ template <typename T> static bool getrs_reg_ld_ok(int64_t lda) {
 template <int NCT, bool TRANS, typename T>
 static void launch_getrs_reg(int n, const T* LU, int64_t lda, int64_t bstride, int batch,
                              const int* perm, const T* dinv, const ColSeg& X,
-                             int64_t xbstride, int ncols, hipStream_t st) {
+                             int64_t xbstride, int ncols, hipStream_t st,
+                             const int* only = nullptr) {
   const int nchunk = (ncols + NCT * 16 - 1) / (NCT * 16);
   hipLaunchKernelGGL((k_getrs_reg<NCT, TRANS, T>), dim3(batch * nchunk), dim3(512), 0, st, LU, lda,
-                     bstride, n, perm, dinv, X, xbstride, ncols, nchunk);
+                     bstride, n, perm, dinv, X, xbstride, ncols, nchunk, only);
 }
 
 int g_getrs_tile = 0;
@@ -2876,9 +2878,11 @@ template <bool TRANS>
 static hipError_t launch_getrs_impl(int n, const double* LU, int64_t lda, int64_t bstride,
                                     int batch, const int* perm, const double* dinv,
                                     const ColSeg& X, int64_t xbstride, int ncols,
-                                    hipStream_t st) {
+                                    hipStream_t st, const int* only = nullptr) {
   if (batch <= 0 || n <= 0 || ncols <= 0) return hipSuccess;
   if (n > kMaxBlock) return hipErrorInvalidValue;
+  // the node mask exists on the register path only (blocks of up to 512)
+  if (only && !(n <= 512 && getrs_reg_ld_ok<double>(lda))) return hipErrorInvalidValue;
   if (n <= 512 && getrs_reg_ld_ok<double>(lda)) {
     // column tile: 16, 32 or 80 right-hand sides per workgroup, by width
     // (16 for blocks of up to 128 rows, the 2r x 2r cores: their 8 steps
@@ -2888,11 +2892,11 @@ static hipError_t launch_getrs_impl(int n, const double* LU, int64_t lda, int64_
     int nct = (ncols <= 16 || n <= 128) ? 1 : (ncols <= 32 ? 2 : 5);
     if (g_getrs_tile == 1 || g_getrs_tile == 2 || g_getrs_tile == 5) nct = g_getrs_tile;
     if (nct == 1)
-      launch_getrs_reg<1, TRANS, double>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
+      launch_getrs_reg<1, TRANS, double>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st, only);
     else if (nct == 2)
-      launch_getrs_reg<2, TRANS, double>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
+      launch_getrs_reg<2, TRANS, double>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st, only);
     else
-      launch_getrs_reg<5, TRANS, double>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
+      launch_getrs_reg<5, TRANS, double>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st, only);
     return hipGetLastError();
   }
   const int npad = ntri(n) * kTri;
@@ -2934,6 +2938,164 @@ hipError_t launch_getrs_t(int n, const double* LU, int64_t lda, int64_t bstride,
                           const int* perm, const double* dinv, const ColSeg& X, int64_t xbstride,
                           int ncols, hipStream_t st) {
   return launch_getrs_impl<true>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st);
+}
+hipError_t launch_getrs_masked(int n, const double* LU, int64_t lda, int64_t bstride, int batch,
+                               const int* perm, const double* dinv, const ColSeg& X,
+                               int64_t xbstride, int ncols, const int* only, hipStream_t st) {
+  return launch_getrs_impl<false>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st,
+                                  only);
+}
+
+// =====================================================================
+// Core solve with the structured LU of S = [I, A; B, I] (r = 32 or 64):
+// with pivots [1..r, r + ipiv2] the factors are
+//   L = [I, 0; P2 B, L2],  U = [I, A; 0, U2],
+// so S x = R is y2 = L2^-1 (P2 R2 - (P2 B) R1), x2 = U2^-1 y2,
+// x1 = R1 - A x2 (the first r rows of y are R1).  L2 and U2 are applied by
+// 32-row blocks with the stored inverses of S's diagonal blocks r/32, ...,
+// the couplings and the two GEMMs on MFMA.  One workgroup per node and 16
+// right-hand-side columns, four waves; every factor fragment a wave uses
+// (56 values per lane at r = 64) is loaded at the start, so the stages
+// wait only on LDS and the MFMA chains (the 16-row tiles: the GEMMs one
+// per wave; block 0 of L2 / U2 waves 0-1, block 1 waves 2-3, each coupling
+// the other pair).  R (2r x nc per node, ld 2r, node stride rs) is
+// overwritten by the solution.  Cores flagged in `general` (factored by the
+// unstructured LU) are skipped: the masked generic solve takes them.
+// =====================================================================
+template <int R>
+__global__ void __launch_bounds__(256)
+k_core_solve_s(const double* __restrict__ S0, const int* __restrict__ perm0,
+               const double* __restrict__ dinv0, const int* __restrict__ general,
+               double* __restrict__ R0, int64_t rs, int nc) {
+  constexpr int N2 = 2 * R, MT = R / 16, NB = R / 32, Q = R / 4;
+  static_assert(R == 32 || R == 64, "structured core solve: r = 32 or 64");
+  __shared__ double Xt[R][17];  // R1, then x1
+  __shared__ double Zb[R][17];  // P2 R2, then Z (consumed block by block)
+  __shared__ double Yb[R][17];  // y2
+  __shared__ double X2[R][17];  // x2
+  const int b = blockIdx.y;
+  if (general[b]) return;
+  const int c0 = blockIdx.x * 16;
+  const int sc = min(16, nc - c0);
+  const double* S = S0 + (int64_t)b * N2 * N2;
+  const int* perm = perm0 + (int64_t)b * N2;
+  const double* dv = dinv0 + (int64_t)b * dinv_elems(N2);
+  double* Rn = R0 + (int64_t)b * rs;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lr = lane & 15, lk = lane >> 4;
+  const bool gw = wid < MT;  // a GEMM tile (rows 16 wid ..)
+  // block k of L2 / U2 is applied by waves 2k, 2k+1 (tile 2k + (wid & 1));
+  // the coupling of block k to the other block by the other pair
+  const int kb = wid >> 1, half = wid & 1;
+  const bool dw = kb < NB;
+  double fz[Q], fx[Q], fli[8], fui[8], fc[8];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int k = 4 * q + lk;
+    fz[q] = gw ? S[(R + 16 * wid + lr) + (int64_t)k * N2] : 0.0;      // P2 B (row, k)
+    fx[q] = gw ? S[(16 * wid + lr) + (int64_t)(R + k) * N2] : 0.0;    // A (row, k)
+  }
+  const double* Di = dv + (int64_t)(NB + kb) * 2 * kTri * kTri;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int k = 4 * q + lk;
+    fli[q] = dw ? Di[(16 * half + lr) + k * 32] : 0.0;
+    fui[q] = dw ? Di[kTri * kTri + (16 * half + lr) + k * 32] : 0.0;
+    // NB == 2: waves 2-3 take L2(block 1, block 0) (rows of tile 2 + half,
+    // columns of block 0), waves 0-1 U2(block 0, block 1)
+    fc[q] = (NB == 2) ? (kb == 1 ? S[(R + 32 + 16 * half + lr) + (int64_t)(R + k) * N2]
+                                 : S[(R + 16 * half + lr) + (int64_t)(R + 32 + k) * N2])
+                      : 0.0;
+  }
+  for (int e = tid; e < N2 * 16; e += 256) {
+    const int i = e % N2, j = e / N2;
+    const double v = j < sc ? Rn[(i < R ? i : perm[i]) + (int64_t)(c0 + j) * N2] : 0.0;
+    if (i < R)
+      Xt[i][j] = v;
+    else
+      Zb[i - R][j] = v;
+  }
+  __syncthreads();
+  auto tile_mm = [&](const double (&af)[8], double (*Bm)[17], int brow0) {
+    d4 acc = d4zero();
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc = mfma(af[q], Bm[brow0 + 4 * q + lk][lr], acc);
+    return acc;
+  };
+  // Z = P2 R2 - (P2 B) R1
+  if (gw) {
+    d4 acc = d4zero();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) acc = mfma(fz[q], Xt[4 * q + lk][lr], acc);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Zb[16 * wid + crow(lane, v)][lr] -= acc[v];
+  }
+  lds_barrier();
+  // forward: y_0 = L2_00^-1 Z_0; Z_1 -= L2_10 y_0; y_1 = L2_11^-1 Z_1
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    if (kb == k) {
+      const d4 acc = tile_mm(fli, Zb, 32 * k);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Yb[32 * k + 16 * half + crow(lane, v)][lr] = acc[v];
+    }
+    lds_barrier();
+    if (k + 1 < NB) {
+      if (kb == k + 1) {
+        const d4 acc = tile_mm(fc, Yb, 32 * k);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Zb[32 * (k + 1) + 16 * half + crow(lane, v)][lr] -= acc[v];
+      }
+      lds_barrier();
+    }
+  }
+  // backward: x2_1 = U2_11^-1 y_1; y_0 -= U2_01 x2_1; x2_0 = U2_00^-1 y_0
+#pragma unroll
+  for (int k = NB - 1; k >= 0; --k) {
+    if (kb == k) {
+      const d4 acc = tile_mm(fui, Yb, 32 * k);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) X2[32 * k + 16 * half + crow(lane, v)][lr] = acc[v];
+    }
+    lds_barrier();
+    if (k > 0) {
+      if (kb == k - 1) {
+        const d4 acc = tile_mm(fc, X2, 32 * k);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Yb[32 * (k - 1) + 16 * half + crow(lane, v)][lr] -= acc[v];
+      }
+      lds_barrier();
+    }
+  }
+  // x1 = R1 - A x2
+  if (gw) {
+    d4 acc = d4zero();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) acc = mfma(fx[q], X2[4 * q + lk][lr], acc);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Xt[16 * wid + crow(lane, v)][lr] -= acc[v];
+  }
+  lds_barrier();
+  for (int e = tid; e < N2 * 16; e += 256) {
+    const int i = e % N2, j = e / N2;
+    if (j < sc) Rn[i + (int64_t)(c0 + j) * N2] = i < R ? Xt[i][j] : X2[i - R][j];
+  }
+}
+
+bool core_solve_structured(int r) { return r == 32 || r == 64; }
+
+hipError_t launch_core_solve_s(int r, const double* S, int batch, const int* perm,
+                               const double* dinv, const int* general, double* R, int64_t rs,
+                               int nc, hipStream_t st) {
+  if (batch <= 0 || nc <= 0) return hipSuccess;
+  if (!core_solve_structured(r) || !general) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)((nc + 15) / 16), (unsigned)batch);
+  if (r == 32)
+    hipLaunchKernelGGL(k_core_solve_s<32>, grid, dim3(256), 0, st, S, perm, dinv, general, R, rs, nc);
+  else
+    hipLaunchKernelGGL(k_core_solve_s<64>, grid, dim3(256), 0, st, S, perm, dinv, general, R, rs, nc);
+  return hipGetLastError();
 }
 
 // FP32 leaf kernels for the mixed-precision path (blocks up to 512).  The

@@ -317,13 +317,17 @@ def test_padded_leading_dimension_is_bitwise_identical(gpu):
     assert np.array_equal(outs[0][1], outs[1][1])
 
 
-@pytest.mark.parametrize("scale,r", [(1.0, 16), (200.0, 16), (1.0, 100), (200.0, 100)])
+@pytest.mark.parametrize("scale,r", [(1.0, 16), (200.0, 16), (1.0, 100), (200.0, 100),
+                                     (200.0, 64), (1.0, 96), (200.0, 96)])
 def test_core_structured_and_general_lu(gpu, orc, scale, r):
     """Cores S = [I, A; B, I] take the structured LU when max|B| <= 1 (plain
     workload) and the general pivoted LU otherwise (U scaled by 200 makes
-    |B| > 1 at every level); both match the oracle to 1e-10.  Rank 100 takes
-    the wide-core path (64 < r <= 128: the Schur complement factored by the
-    128-row leaf LU)."""
+    |B| > 1 at every level); both match the oracle to 1e-10.  Ranks 96 and
+    100 take the wide-core path (64 < r <= 128: the Schur complement factored
+    by the 128-row leaf LU).  At ranks 64 and 96 (multiples of 32) the core
+    solves take the structured solve, with the generic solve masked to the
+    cores the general LU factored -- in the fused factor+solve and in a
+    factor-reuse solve, which reads every level's flags."""
     F = gpu
     N, m, s = 2048, 128, 2
     say(f"case: core LU path, U scale {scale}, rank {r}")
@@ -343,6 +347,9 @@ def test_core_structured_and_general_lu(gpu, orc, scale, r):
     tol = max(RTOL, 4.0 * relerr(ref, xd))
     say(f"  rel err {e:.2e} (tol {tol:.1e})")
     assert e <= tol and relerr(x, xd) <= tol
+    X2 = to_dev(F, Bh)
+    plan.solve(X2)
+    assert relerr(to_host(X2), ref) <= tol
     plan.close()
 
 @pytest.mark.parametrize("r", [16, 30, 64, 96, 128])
