@@ -207,7 +207,8 @@ int core_factor_solve_n(int nnode, int n2, const CoreBufs& c, int nh, hipStream_
       HIP_TRY(hodlr::launch_getrf(n2, c.S, n2, (int64_t)n2 * n2, nnode, c.ipiv, c.perm, c.info,
                                   st));
   }
-  {
+  // cores of up to 128 rows: the LU kernels formed every inverse themselves
+  if (!(c.general && n2 <= 128 && (n2 & 1) == 0)) {
     ProfScope ps(HODLR_PROF_TRTRI, st, 0.0, blk);
     if (c.general)  // only the cores the structured LU left without inverses
       HIP_TRY(hodlr::launch_trtri_diag_masked(n2, c.S, n2, (int64_t)n2 * n2, nnode, c.dinv, c.tmask,

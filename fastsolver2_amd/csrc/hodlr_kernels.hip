@@ -1633,14 +1633,21 @@ k_getrf_ll(const double* __restrict__ K0, int64_t ldk, int64_t kbs, T* __restric
 // Pivot rule, implicit row tracking, U12 and the trailing update are those
 // of k_getrf_reg.
 // =====================================================================
+template <typename T>
+__device__ __forceinline__ void trtri_block(const T* __restrict__ LU, int64_t lda, int n, int ib,
+                                            T* __restrict__ out, T (*M0)[kTri + 1],
+                                            T (*M1)[kTri + 1], int lane);
+
 template <int NB>
 __global__ void __launch_bounds__(64)
 k_getrf_w64(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
             int* __restrict__ ipiv0, int* __restrict__ perm0, int* __restrict__ info0,
             const int* __restrict__ only, const double* __restrict__ src = nullptr,
-            int64_t lds = 0, int64_t sbstride = 0) {
+            int64_t lds = 0, int64_t sbstride = 0, double* __restrict__ dinv0 = nullptr) {
   // `only` (optional): factor block b only where only[b] != 0; `src`
-  // (optional): block b is first copied from src + b*sbstride (ld lds)
+  // (optional): block b is first copied from src + b*sbstride (ld lds);
+  // `dinv0` (optional): the inverses of the factor's 32x32 diagonal blocks
+  // are formed too (dinv_elems(n) per block)
   if (only && !only[blockIdx.x]) return;
   if (src) {
     const double* S0 = src + (int64_t)blockIdx.x * sbstride;
@@ -1893,6 +1900,18 @@ k_getrf_w64(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
   for (int q = 0; q < R; ++q)
     if (q * 64 + lane < n) perm0[(int64_t)b * n + q * 64 + lane] = perm[q * 64 + lane];
   if (lane == 0) info0[b] = info;
+  if (dinv0) {
+    static_assert(NB >= kTri, "the inversion tiles fit L11 and Xi");
+    // the factor is this wave's own stores: the barrier makes every lane's
+    // visible to all of them
+    __syncthreads();
+    double (*const M0)[kTri + 1] = reinterpret_cast<double (*)[kTri + 1]>(L11);
+    double (*const M1)[kTri + 1] = reinterpret_cast<double (*)[kTri + 1]>(Xi);
+    for (int ib = 0; ib < ntri(n); ++ib)
+      trtri_block<double>(A, lda, n, ib,
+                          dinv0 + (int64_t)b * dinv_elems(n) + (int64_t)ib * 2 * kTri * kTri, M0,
+                          M1, lane);
+  }
 }
 
 static size_t getrf_lds_bytes(int nb, int n) {
@@ -1973,6 +1992,7 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
   // the no-interchange attempt's 32x32 tiles (ll_diag_step)
   __shared__ double Dt[32][33], A12t[32][33], Lit[32][33], Uit[32][33], Mst[2][32][33];
   __shared__ unsigned long long wmk[4];
+  __shared__ int needinv;  // the diagonal inverses are still to be formed
   const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
   const int wid = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lr = lane & 15, lk = lane >> 4;
@@ -2003,7 +2023,7 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
   const bool fallback = mk > (unsigned long long)__double_as_longlong(1.0);
   if (tid == 0) {
     general[b] = fallback ? 1 : 0;
-    tmask[b] = 1;  // cleared below when this kernel writes the inverses
+    tmask[b] = 1;  // cleared below once this kernel has formed the inverses
   }
   if (fallback) return;  // uniform over the workgroup
   // (2) C = I - B A on MFMA from LDS: wave w forms 16-row block w of C,
@@ -2034,7 +2054,9 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
       }
   }
   __syncthreads();
-  if (wid != 0) return;
+  // (3) the LU of C by wave 0; waves 1-3 wait at the barrier before (5)
+  if (wid == 0) {
+  bool done = false;
   // (3a) No interchanges when partial pivoting keeps every diagonal: C,
   // padded with the identity to rp = 32 or 64 rows, factored in 32x32
   // blocks by the leaf LU's diagonal step (every multiplier checked
@@ -2141,12 +2163,13 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
       }
       if (lane == 0) {
         info0[b] = 0;
-        if (inv) tmask[b] = 0;
+        needinv = inv ? 0 : 1;
       }
-      return;
+      done = true;
     }
   }
-  // (3) LU of C with implicit pivoting (lane i keeps row i of C; rows
+  if (!done) {
+  // (3b) LU of C with implicit pivoting (lane i keeps row i of C; rows
   // never move, pos is a row's LAPACK position), blocked like dgetrf: a
   // 16-column panel is factored with rank-1 updates inside it (the pivot
   // row's panel elements broadcast by readlane), then the panel's pivot
@@ -2261,7 +2284,34 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
     ipiv[r + lane] = r + mypiv + 1;
     perm[r + pos] = r + lane;
   }
-  if (lane == 0) info0[b] = info;
+  if (lane == 0) {
+    info0[b] = info;
+    needinv = 1;
+  }
+  }  // !done
+  }  // wave 0
+  // (5) the inverses of S's 32x32 diagonal blocks where (3) did not form
+  // them, one block per wave, from S's LU as (3) stored it; each wave's
+  // tiles are LDS the factorization no longer uses
+  __syncthreads();
+  if (needinv) {
+    double(*M0)[33];
+    double(*M1)[33];
+    if (wid == 0) {
+      M0 = Mst[0]; M1 = Mst[1];
+    } else if (wid == 1) {
+      M0 = Dt; M1 = A12t;
+    } else if (wid == 2) {
+      M0 = Lit; M1 = Uit;
+    } else {
+      M0 = reinterpret_cast<double (*)[33]>(Cs);
+      M1 = reinterpret_cast<double (*)[33]>(Cs + 32 * 33);
+    }
+    double* dv = dinv0 + (int64_t)b * dinv_elems(n2);
+    for (int ib = wid; ib < ntri(n2); ib += 4)
+      trtri_block<double>(S, lda, n2, ib, dv + (int64_t)ib * 2 * kTri * kTri, M0, M1, lane);
+  }
+  if (tid == 0) tmask[b] = 0;
 }
 
 // Cores of rank 64 < r <= 128 (S = [I, A; B, I], n2 = 2r <= 256).  The
@@ -2439,8 +2489,10 @@ hipError_t launch_getrf_core(int n2, double* S, int64_t lda, int64_t bstride, in
   }
   hipLaunchKernelGGL(k_getrf_core, dim3(batch), dim3(256), 0, st, S, lda, bstride, n2, ipiv, perm,
                      info, general, dinv, tmask);
+  // general cores: the pivoted LU of S and its diagonal inverses
   hipLaunchKernelGGL(k_getrf_w64<32>, dim3(batch), dim3(64), 0, st, S, lda, bstride, n2, ipiv, perm,
-                     info, (const int*)general);
+                     info, (const int*)general, (const double*)nullptr, (int64_t)0, (int64_t)0,
+                     dinv);
   return hipGetLastError();
 }
 
@@ -2451,38 +2503,35 @@ hipError_t launch_getrf_core(int n2, double* S, int64_t lda, int64_t bstride, in
 // last block is padded with the identity.  Lane c < 32 computes column c
 // of Linv, lane 32 + c column c of Uinv (substitution on e_c).
 // =====================================================================
+// One wave: the inverses of diagonal block ib of the LU at LU (ld lda, n
+// rows) into out ([Linv | Uinv], 32x32 column-major each), through the two
+// LDS tiles M0, M1 ([32][33] each).  Both triangles go in as
+// lower-triangular tiles, [col][row]: M0 = L with its unit diagonal, M1 =
+// U index-reversed (M1[31-c][31-r] = U[r][c]), so lanes 0..31 (columns of
+// Linv) and 32..63 (columns of Uinv, reversed) run the same forward
+// substitution side by side.  Only the calling wave touches the tiles:
+// its LDS accesses execute in order, so wave syncs order them.
 template <typename T>
-__global__ void __launch_bounds__(64)
-k_trtri_diag(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
-             T* __restrict__ dinv0, const int* __restrict__ only = nullptr) {
-  // Both triangles as lower-triangular tiles, [col][row]: Ms[0] = L with
-  // its unit diagonal, Ms[1] = U index-reversed (Ms[1][31-c][31-r] =
-  // U[r][c]), so lanes 0..31 (columns of Linv) and 32..63 (columns of
-  // Uinv, reversed) run the same forward substitution side by side
-  __shared__ T Ms[2][kTri][kTri + 1];
-  const int nblk = ntri(n);
-  const int b = blockIdx.x / nblk, ib = blockIdx.x % nblk;
-  if (only && !only[b]) return;  // (optional) blocks b with only[b] != 0
+__device__ __forceinline__ void trtri_block(const T* __restrict__ LU, int64_t lda, int n, int ib,
+                                            T* __restrict__ out, T (*M0)[kTri + 1],
+                                            T (*M1)[kTri + 1], int lane) {
   const int i0 = ib * kTri;
   const int bs = min(kTri, n - i0);
-  const T* LU = LU0 + (int64_t)b * bstride;
-  const int tid = threadIdx.x;
   // all 16 loads of a lane in flight together
   T v[kTri * kTri / 64];
 #pragma unroll
   for (int q = 0; q < kTri * kTri / 64; ++q) {
-    const int idx = tid + 64 * q, c = idx / kTri, r = idx % kTri;
+    const int idx = lane + 64 * q, c = idx / kTri, r = idx % kTri;
     v[q] = (r < bs && c < bs) ? LU[(int64_t)(i0 + r) + (int64_t)(i0 + c) * lda] : ((r == c) ? T(1) : T(0));
   }
 #pragma unroll
   for (int q = 0; q < kTri * kTri / 64; ++q) {
-    const int idx = tid + 64 * q, c = idx / kTri, r = idx % kTri;
-    Ms[0][c][r] = (r > c) ? v[q] : ((r == c) ? T(1) : T(0));
-    Ms[1][kTri - 1 - c][kTri - 1 - r] = (r <= c) ? v[q] : T(0);
+    const int idx = lane + 64 * q, c = idx / kTri, r = idx % kTri;
+    M0[c][r] = (r > c) ? v[q] : ((r == c) ? T(1) : T(0));
+    M1[kTri - 1 - c][kTri - 1 - r] = (r <= c) ? v[q] : T(0);
   }
-  __syncthreads();
-  T* out = dinv0 + (int64_t)b * dinv_elems(n) + (int64_t)ib * 2 * kTri * kTri;
-  const int h = tid >> 5, c = tid & 31;
+  wave_sync();
+  const int h = lane >> 5, c = lane & 31;
   // column c of Linv solves L x = e_c; column c of Uinv, reversed, solves
   // the reversed U with the unit vector at 31 - c
   const int c0 = h ? kTri - 1 - c : c;
@@ -2492,7 +2541,7 @@ k_trtri_diag(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
   T x[kTri];
 #pragma unroll
   for (int i = 0; i < kTri; ++i) x[i] = (i == c0) ? T(1) : T(0);
-  const T* const tile = &Ms[h][0][0];
+  const T* const tile = h ? &M1[0][0] : &M0[0][0];
 #pragma unroll
   for (int k = 0; k < kTri; ++k) {
     // an opaque column offset per step keeps the step's reads in the step
@@ -2507,21 +2556,36 @@ k_trtri_diag(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
   }
   // the columns go out through an LDS tile (Linv, then Uinv), so every
   // store of the wave covers 512 consecutive bytes
-  T (*const Ts)[kTri + 1] = Ms[0];
+  T (*const Ts)[kTri + 1] = M0;
 #pragma unroll
   for (int hh = 0; hh < 2; ++hh) {
-    __syncthreads();  // the tile's last readers are done
+    wave_sync();  // the tile's last readers are done
     if (h == hh) {
 #pragma unroll
       for (int i = 0; i < kTri; ++i) Ts[c][i] = hh ? x[kTri - 1 - i] : x[i];
     }
-    __syncthreads();
+    wave_sync();
 #pragma unroll
     for (int q = 0; q < kTri * kTri / 64; ++q) {
-      const int idx = tid + 64 * q;
+      const int idx = lane + 64 * q;
       out[hh * kTri * kTri + idx] = Ts[idx / kTri][idx % kTri];
     }
   }
+}
+
+// Inverses of the 32x32 diagonal blocks of L (unit lower) and U (upper),
+// one wave per block.
+template <typename T>
+__global__ void __launch_bounds__(64)
+k_trtri_diag(const T* __restrict__ LU0, int64_t lda, int64_t bstride, int n,
+             T* __restrict__ dinv0, const int* __restrict__ only = nullptr) {
+  __shared__ T Ms[2][kTri][kTri + 1];
+  const int nblk = ntri(n);
+  const int b = blockIdx.x / nblk, ib = blockIdx.x % nblk;
+  if (only && !only[b]) return;  // (optional) blocks b with only[b] != 0
+  trtri_block<T>(LU0 + (int64_t)b * bstride, lda, n, ib,
+                 dinv0 + (int64_t)b * dinv_elems(n) + (int64_t)ib * 2 * kTri * kTri, Ms[0], Ms[1],
+                 (int)threadIdx.x);
 }
 
 hipError_t launch_trtri_diag(int n, const double* LU, int64_t lda, int64_t bstride, int batch,
