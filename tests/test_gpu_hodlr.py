@@ -667,3 +667,32 @@ def test_config2_size_residual_and_oracle_sample(gpu, orc):
                          nthreads=16)
     assert relerr(to_host(B), xo) <= RTOL
     plan.close()
+
+
+@pytest.mark.parametrize("r", [200, 300])
+def test_large_rank_plan(gpu, orc, r):
+    """Ranks above 128: cores of 2r rows take the general pivoted LU and
+    the generic solve (2r up to 1216), and the top-down pass keeps H in
+    narrower LDS column chunks (16 columns at r = 200, 8 at r = 300).  The
+    fused factor+solve and a factor-reuse solve against the oracle at
+    1e-10, the adjoint solve against the dense A^-T."""
+    F = gpu
+    m, L, s = r + 64, 2, 20
+    N = m << L
+    say(f"case: large rank r={r} N={N} leaf={m} nrhs={s}")
+    K, U, V, B = F.workload_gaussian(N, m, r, s, seed=21)
+    Kh, Uh, Vh, Bh = (to_host(t) for t in (K, U, V, B))
+    ref = orc.hodlr_solve(Kh, Uh, Vh, Bh, m)
+    plan = F.HodlrPlan(N, m, L, r, nrhs_max=s)
+    X = F.fortran_clone(B)
+    plan.factor_solve(K, U, V, X)
+    assert plan.status()[0] == 0
+    assert relerr(to_host(X), ref) <= RTOL
+    X2 = F.fortran_clone(B)
+    plan.solve(X2)
+    assert relerr(to_host(X2), ref) <= RTOL
+    A = orc.dense_assemble(Kh, Uh, Vh, m)
+    G = F.fortran_clone(B)
+    plan.solve_adjoint(G)
+    assert relerr(to_host(G), np.linalg.solve(A.T, Bh)) <= RTOL
+    plan.close()
