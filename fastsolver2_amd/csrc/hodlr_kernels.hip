@@ -1541,7 +1541,7 @@ k_getrf_ll(const double* __restrict__ K0, int64_t ldk, int64_t kbs, T* __restric
         // compiler barrier: the parking stores stay here (sunk into the
         // step they would keep the accumulators live)
         asm volatile("" ::: "memory");
-        ll_diag_step<T, false>(Ds, As12, Li, Ui, Ms, Prow, dinv + (int64_t)ib * 2 * kTri * kTri, p == 0,
+        ll_diag_step<T, (RT < 4)>(Ds, As12, Li, Ui, Ms, Prow, dinv + (int64_t)ib * 2 * kTri * kTri, p == 0,
                         thr, fail, lane);
         asm volatile("" ::: "memory");
 #pragma unroll
@@ -2949,11 +2949,12 @@ static hipError_t launch_getrs_impl(int n, const double* LU, int64_t lda, int64_
   if (only && !(n <= 512 && getrs_reg_ld_ok<double>(lda))) return hipErrorInvalidValue;
   if (n <= 512 && getrs_reg_ld_ok<double>(lda)) {
     // column tile: 16, 32 or 80 right-hand sides per workgroup, by width
-    // (16 for blocks of up to 128 rows, the 2r x 2r cores: their 8 steps
-    // are latency, and more, narrower workgroups shorten them; a block's
-    // factor is re-read per workgroup, which is cheap at that size) unless
+    // (16 for blocks of up to 256 rows -- the 2r x 2r cores and the smaller
+    // leaves: their steps are latency, and more, narrower workgroups
+    // shorten them, while a block's factor re-read per workgroup is cheap
+    // at that size; measured 11 % faster per system at rank 97-98) unless
     // a tuner fixed it
-    int nct = (ncols <= 16 || n <= 128) ? 1 : (ncols <= 32 ? 2 : 5);
+    int nct = (ncols <= 16 || n <= 256) ? 1 : (ncols <= 32 ? 2 : 5);
     if (g_getrs_tile == 1 || g_getrs_tile == 2 || g_getrs_tile == 5) nct = g_getrs_tile;
     if (nct == 1)
       launch_getrs_reg<1, TRANS, double>(n, LU, lda, bstride, batch, perm, dinv, X, xbstride, ncols, st, only);
