@@ -211,6 +211,12 @@ template <typename V> __device__ __forceinline__ V opaque(V x) {
   asm volatile("" : "+v"(x));
   return x;
 }
+// The same for a wave-uniform value kept in a scalar register (a buffer
+// access's scalar offset is built on it inside the loop that uses it).
+template <typename V> __device__ __forceinline__ V opaque_s(V x) {
+  asm volatile("" : "+s"(x));
+  return x;
+}
 // Workgroup barrier that orders LDS only: it waits for this wave's LDS
 // operations, not for its global stores (which the workgroup fence of
 // __syncthreads also covers).  For exchanges through LDS where no wave reads
@@ -1142,28 +1148,19 @@ __device__ __forceinline__ void wave_sync() {
 // Unblocked LU without interchanges of a 16x16 block held by lanes 0..15
 // (lane i: row i in a[]), dgetf2's operation order; every multiplier is
 // checked against thr.  Row j, final after step j - 1, reaches every lane
-// either from lane j's registers by readlane (RL: register-to-scalar moves,
-// no LDS round trip on the serial chain; it halves the step, but its scalar
-// registers change the allocation of a kernel it is inlined into) or
-// through Prow (16-byte LDS stores by lane j, read back after a wave sync).
-template <typename T, bool RL>
-__device__ __forceinline__ void ll_lu16(T (&a)[16], int lane, T* Prow, T thr, bool& fail) {
+// from lane j's registers by readlane (register-to-scalar moves: no LDS
+// round trip and no wave synchronization on the serial chain; half the
+// time of publishing the row through LDS, measured in the leaf and core
+// LUs once the leaf LU's scalar offsets stopped occupying the scalar
+// registers across the kernel).
+template <typename T>
+__device__ __forceinline__ void ll_lu16(T (&a)[16], int lane, T thr, bool& fail) {
   const bool own = lane < 16;
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
     T u[16];
-    if (RL) {
 #pragma unroll
-      for (int k = j; k < 16; ++k) u[k] = Elem<T>::lane(a[k], j);
-    } else {
-      if (lane == j) {
-#pragma unroll
-        for (int k = j; k < 16; ++k) Prow[k] = a[k];
-      }
-      wave_sync();
-#pragma unroll
-      for (int k = j; k < 16; ++k) u[k] = Prow[k];
-    }
+    for (int k = j; k < 16; ++k) u[k] = Elem<T>::lane(a[k], j);
     const T r = recip_nr(u[j]);
     if (own && lane > j) {
       const T l = a[j] * r;
@@ -1172,7 +1169,6 @@ __device__ __forceinline__ void ll_lu16(T (&a)[16], int lane, T* Prow, T thr, bo
 #pragma unroll
       for (int k = j + 1; k < 16; ++k) a[k] -= l * u[k];
     }
-    if (!RL) wave_sync();
   }
 }
 
@@ -1221,9 +1217,9 @@ __device__ __forceinline__ void ll_inv16(T (*Mt)[16][17], T (*Li)[33], T (*Ui)[3
 //   A11 = L1 U1;  L21 = A21 U1^-1;  U12 = L1^-1 A12;  A22 - L21 U12 = L2 U2;
 //   L^-1 = [L1^-1, 0; -L2^-1 (L21 L1^-1), L2^-1],
 //   U^-1 = [U1^-1, -(U1^-1 U12) U2^-1; 0, U2^-1].
-template <typename T, bool RL>
+template <typename T>
 __device__ __forceinline__ void ll_diag_step(T (*Ds)[33], T (*As12)[33], T (*Li)[33],
-                                             T (*Ui)[33], T (*Ms)[32][33], T* Prow,
+                                             T (*Ui)[33], T (*Ms)[32][33],
                                              T* __restrict__ out, bool with_a12, T thr,
                                              bool& fail, int lane_in) {
   typedef typename Elem<T>::V4 V4;
@@ -1240,7 +1236,7 @@ __device__ __forceinline__ void ll_diag_step(T (*Ds)[33], T (*As12)[33], T (*Li)
     T a[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) a[k] = Ds[o + i][o + k];
-    ll_lu16<T, RL>(a, lane, Prow, thr, fail);
+    ll_lu16<T>(a, lane, thr, fail);
     if (lane < 16) {
 #pragma unroll
       for (int k = 0; k < 16; ++k) {
@@ -1367,7 +1363,6 @@ k_getrf_ll(const double* __restrict__ K0, int64_t ldk, int64_t kbs, T* __restric
   __shared__ T Li[32][33];     // L11^-1 [row][col]
   __shared__ T Ui[32][33];     // U11^-1 [row][col]
   __shared__ T Ms[2][32][33];  // inversion tiles
-  __shared__ __align__(16) T Prow[32];  // pivot row broadcast
   __shared__ int bad;
   const int b = blockIdx.x;
   const int tid = threadIdx.x;
@@ -1386,6 +1381,12 @@ k_getrf_ll(const double* __restrict__ K0, int64_t ldk, int64_t kbs, T* __restric
 
   for (int J = 0; J < NS; ++J) {
     const int j0 = 64 * J;
+    // the column strides again, opaque per slab: the scalar offsets built
+    // on them (64 for the slab's loads, 32 for the L fragments) are formed
+    // here instead of being hoisted and held across the kernel, which had
+    // spilled 307 scalar registers (19 vector registers to scratch) and
+    // now spills 145 (13)
+    const uint32_t ldkb_j = opaque_s(ldkb), ldab_j = opaque_s(ldab);
     // Barriers.  The chunk loop exchanges only LDS data (X_c, U_c), so its
     // barriers order LDS alone and the slab's loads of K stay in flight
     // under the first chunks.  Factors a chunk reads from global memory
@@ -1403,14 +1404,14 @@ k_getrf_ll(const double* __restrict__ K0, int64_t ldk, int64_t kbs, T* __restric
       // the slab, all n rows, converted to T: element (q, ct, v) is row
       // 16 (wid + 8q) + lr, column j0 + 16 ct + cr0 + v DV
       const __amdgpu_buffer_rsrc_t rk = buf_rsrc(K + (int64_t)j0 * ldk);
-      const uint32_t vo = lr * 8u + (uint32_t)cr0 * ldkb;
+      const uint32_t vo = lr * 8u + (uint32_t)cr0 * ldkb_j;
 #pragma unroll
       for (int q = 0; q < RT; ++q)
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
           for (int v = 0; v < 4; ++v)
-            acc[q][ct][v] = (T)buf_load<double>(rk, vo, 128u * (wid + 8 * q) + (16 * ct + v * DV) * ldkb);
+            acc[q][ct][v] = (T)buf_load<double>(rk, vo, 128u * (wid + 8 * q) + (16 * ct + v * DV) * ldkb_j);
     }
     if (J > 0 && wid < 2) {
       // X_0 = rows 0..31 (tiles 0, 1: waves 0, 1, q = 0)
@@ -1427,7 +1428,7 @@ k_getrf_ll(const double* __restrict__ K0, int64_t ldk, int64_t kbs, T* __restric
     const int rl = wid & 1, uct = wid >> 1;  // this wave's U tile
     T lcv[8];
     T lf[RT][8];
-    const uint32_t vl = opaque(lr * (uint32_t)sizeof(T) + lk * ldab);
+    const uint32_t vl = opaque(lr * (uint32_t)sizeof(T) + lk * ldab_j);
     auto fetch_lc = [&](int c) {
       const T* Lc = dinv + (int64_t)c * 2 * kTri * kTri + 16 * rl + lr;
 #pragma unroll
@@ -1438,7 +1439,7 @@ k_getrf_ll(const double* __restrict__ K0, int64_t ldk, int64_t kbs, T* __restric
 #pragma unroll
       for (int q = 0; q < RT; ++q) {
         const int rt = wid + 8 * q;
-        if (rt >= 2 * c + 2) lf[q][s2] = buf_load<T>(ra, vl, 16u * sizeof(T) * rt + 4 * s2 * ldab);
+        if (rt >= 2 * c + 2) lf[q][s2] = buf_load<T>(ra, vl, 16u * sizeof(T) * rt + 4 * s2 * ldab_j);
       }
     };
     if (J > 0) {
@@ -1541,7 +1542,7 @@ k_getrf_ll(const double* __restrict__ K0, int64_t ldk, int64_t kbs, T* __restric
         // compiler barrier: the parking stores stay here (sunk into the
         // step they would keep the accumulators live)
         asm volatile("" ::: "memory");
-        ll_diag_step<T, (RT < 4)>(Ds, As12, Li, Ui, Ms, Prow, dinv + (int64_t)ib * 2 * kTri * kTri, p == 0,
+        ll_diag_step<T>(Ds, As12, Li, Ui, Ms, dinv + (int64_t)ib * 2 * kTri * kTri, p == 0,
                         thr, fail, lane);
         asm volatile("" ::: "memory");
 #pragma unroll
@@ -1605,14 +1606,14 @@ k_getrf_ll(const double* __restrict__ K0, int64_t ldk, int64_t kbs, T* __restric
     {
       // the slab's factors
       const __amdgpu_buffer_rsrc_t ra = buf_rsrc(A + (int64_t)j0 * lda);
-      const uint32_t vo = lr * (uint32_t)sizeof(T) + (uint32_t)cr0 * ldab;
+      const uint32_t vo = lr * (uint32_t)sizeof(T) + (uint32_t)cr0 * ldab_j;
 #pragma unroll
       for (int q = 0; q < RT; ++q)
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
           for (int v = 0; v < 4; ++v)
-            buf_store(acc[q][ct][v], ra, vo, 16u * sizeof(T) * (wid + 8 * q) + (16 * ct + v * DV) * ldab);
+            buf_store(acc[q][ct][v], ra, vo, 16u * sizeof(T) * (wid + 8 * q) + (16 * ct + v * DV) * ldab_j);
     }
   }
   for (int t = tid; t < n; t += 512) {
@@ -2083,7 +2084,7 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
       if (rp == 64) A12t[i][j] = cpad(i, 32 + j);
     }
     wave_sync();
-    ll_diag_step<double, true>(Dt, A12t, Lit, Uit, Mst, nullptr, inv ? dv + (int64_t)(r / 32) * 2 * kTri * kTri : nullptr,
+    ll_diag_step<double>(Dt, A12t, Lit, Uit, Mst, inv ? dv + (int64_t)(r / 32) * 2 * kTri * kTri : nullptr,
                          rp == 64, thr, fail, lane);
     wave_sync();
 #pragma unroll
@@ -2126,7 +2127,7 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
           for (int v = 0; v < 4; ++v) Dt[16 * rt + Elem<double>::crow(lane, v)][16 * ct + lr] = c[v];
         }
       wave_sync();
-      ll_diag_step<double, true>(Dt, A12t, Lit, Uit, Mst, nullptr,
+      ll_diag_step<double>(Dt, A12t, Lit, Uit, Mst,
                            inv ? dv + (int64_t)(r / 32 + 1) * 2 * kTri * kTri : nullptr, false,
                            thr, fail, lane);
       wave_sync();
