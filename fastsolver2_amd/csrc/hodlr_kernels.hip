@@ -1172,13 +1172,20 @@ __device__ __forceinline__ void ll_lu16(T (&a)[16], int lane, T thr, bool& fail)
   }
 }
 
+// Column stride of the 16x16 inversion tiles: 16 elements and 16 bytes of
+// padding, so every column starts 16-byte aligned and a step's reads are
+// wide LDS reads off one base address.
+template <typename T> struct InvTile {
+  static constexpr int S = 16 + 16 / (int)sizeof(T);
+};
+
 // Inverses of the unit-lower L and upper U of a 16x16 LU held in the
 // inversion tiles Mt[0] (L with its unit diagonal, [col][row]) and Mt[1]
 // (U index-reversed, also lower triangular): lanes 0..15 solve for the
 // columns of L^-1, lanes 16..31 for those of U^-1 (reversed), by
 // column-oriented substitution; written to Li / Ui at offset (o, o).
 template <typename T>
-__device__ __forceinline__ void ll_inv16(T (*Mt)[16][17], T (*Li)[33], T (*Ui)[33], int o,
+__device__ __forceinline__ void ll_inv16(T (*Mt)[16][InvTile<T>::S], T (*Li)[33], T (*Ui)[33], int o,
                                          int lane) {
   const int h = (lane >> 4) & 1, c = lane & 15;
   const int c0 = h ? 15 - c : c;
@@ -1186,18 +1193,35 @@ __device__ __forceinline__ void ll_inv16(T (*Mt)[16][17], T (*Li)[33], T (*Ui)[3
 #pragma unroll
   for (int q = 0; q < 16; ++q) x[q] = (q == c0) ? T(1) : T(0);
   const T* const tile = &Mt[h][0][0];
+  // The diagonal's reciprocals first (16 independent load + reciprocal
+  // chains, issued together), then the substitution with each step's column
+  // read one step ahead: the scheduling barriers keep the reads of step
+  // k + 1 above the arithmetic of step k, so a step's dependent chain is
+  // its multiply and the next element's update, not an LDS round trip and
+  // a reciprocal (left to the scheduler, each step waited for its own
+  // column and its reciprocal).  The arithmetic is unchanged.
+  T rd[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) rd[k] = recip_nr(tile[k * InvTile<T>::S + k]);
+  T cn[16];
+#pragma unroll
+  for (int q = 1; q < 16; ++q) cn[q] = tile[q];
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
-    const T* const col = tile + k * 17;
-    // the reciprocal is formed inside the step, from the step's own load;
-    // the scheduler may move the steps' independent LDS reads ahead (pinning
-    // them in their steps, for registers, measured 1.3 % slower in the leaf
-    // LU once its scalar offsets were formed per slab)
-    const T d = x[k] * recip_nr(col[k]);
+    T cc[16];
+#pragma unroll
+    for (int q = k + 1; q < 16; ++q) cc[q] = cn[q];
+    if (k + 1 < 16) {
+#pragma unroll
+      for (int q = k + 2; q < 16; ++q) cn[q] = tile[(k + 1) * InvTile<T>::S + q];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    const T d = x[k] * rd[k];
     const T xk = (k < c0) ? T(0) : d;
     x[k] = xk;
 #pragma unroll
-    for (int q = k + 1; q < 16; ++q) x[q] -= col[q] * xk;
+    for (int q = k + 1; q < 16; ++q) x[q] -= cc[q] * xk;
+    __builtin_amdgcn_sched_barrier(0);
   }
   if (lane < 16) {
 #pragma unroll
@@ -1228,8 +1252,8 @@ __device__ __forceinline__ void ll_diag_step(T (*Ds)[33], T (*As12)[33], T (*Li)
   const int lane = opaque(lane_in);
   const int lr = lane & 15, lk = lane >> 4;
   // the inversion tiles and a product scratch, in the space of Ms
-  T (*const Mt)[16][17] = reinterpret_cast<T (*)[16][17]>(&Ms[0][0][0]);
-  T (*const Pt)[16][17] = reinterpret_cast<T (*)[16][17]>(&Ms[0][0][0] + 2 * 16 * 17);
+  T (*const Mt)[16][InvTile<T>::S] = reinterpret_cast<T (*)[16][InvTile<T>::S]>(&Ms[0][0][0]);
+  T (*const Pt)[16][17] = reinterpret_cast<T (*)[16][17]>(&Ms[0][0][0] + 2 * 16 * InvTile<T>::S);
   const int i = lane & 15;
   auto factor16 = [&](int o) {
     // LU of the diagonal 16x16 block at (o, o), then its inversion tiles
@@ -1362,7 +1386,9 @@ k_getrf_ll(const double* __restrict__ K0, int64_t ldk, int64_t kbs, T* __restric
   __shared__ T As12[32][33];   // A12 in, U12 out
   __shared__ T Li[32][33];     // L11^-1 [row][col]
   __shared__ T Ui[32][33];     // U11^-1 [row][col]
-  __shared__ T Ms[2][32][33];  // inversion tiles
+  __shared__ __align__(16) T Ms[2][32][33];  // inversion tiles
+  // landing space of the next slab's cache prefetch (never read)
+  __shared__ __align__(16) char pf_sink[7][1024];
   __shared__ int bad;
   const int b = blockIdx.x;
   const int tid = threadIdx.x;
@@ -1377,6 +1403,8 @@ k_getrf_ll(const double* __restrict__ K0, int64_t ldk, int64_t kbs, T* __restric
   const uint32_t ldab = (uint32_t)lda * (uint32_t)sizeof(T);
   bool fail = false;
   if (tid == 0) bad = 0;
+  // the next slab's cache prefetch takes 16-byte pieces of K's columns
+  const bool pf_ok = ((reinterpret_cast<uintptr_t>(K) & 15) == 0) && ((ldk & 1) == 0);
   V4 acc[RT][4];
 
   for (int J = 0; J < NS; ++J) {
@@ -1551,6 +1579,20 @@ k_getrf_ll(const double* __restrict__ K0, int64_t ldk, int64_t kbs, T* __restric
           for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
             for (int v = 0; v < 4; ++v) acc[q][ct][v] = park[((q * 4 + ct) * 4 + v) * 64];
+      } else if (pf_ok && J + 1 < NS) {
+        // waves 1..7, idle through the diagonal step: half p of the next
+        // slab of K (32 columns of n rows, RT KB each, in 1 KB pieces) is
+        // copied toward this CU (L2 and the memory-side cache; the LDS
+        // copy is dropped), so the next slab's load, which every CU issues
+        // at the same moment, is served from the caches rather than HBM
+        const char* col0 = reinterpret_cast<const char*>(K + (int64_t)(j0 + 64 + 32 * p) * ldk);
+        for (int i = wid - 1; i < 32 * RT; i += 7) {
+          const int c = i / RT, part = i - c * RT;
+          const char* src = col0 + (size_t)c * ldkb_j + part * 1024 + lane * 16;
+          __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                           (__attribute__((address_space(3))) void*)&pf_sink[wid - 1][0],
+                                           16, 0, 0);
+        }
       }
       __syncthreads();
 #pragma unroll
@@ -1991,7 +2033,8 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
   __shared__ double Bs[64 * 65];  // B, [row*65 + k]
   __shared__ double As[64 * 65];  // A, [k*65 + col]; then the fast path's packed LU
   // the no-interchange attempt's 32x32 tiles (ll_diag_step)
-  __shared__ double Dt[32][33], A12t[32][33], Lit[32][33], Uit[32][33], Mst[2][32][33];
+  __shared__ double Dt[32][33], A12t[32][33], Lit[32][33], Uit[32][33];
+  __shared__ __align__(16) double Mst[2][32][33];
   __shared__ unsigned long long wmk[4];
   __shared__ int needinv;  // the diagonal inverses are still to be formed
   const int b = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
