@@ -1570,8 +1570,8 @@ k_getrf_ll(const double* __restrict__ K0, int64_t ldk, int64_t kbs, T* __restric
         // compiler barrier: the parking stores stay here (sunk into the
         // step they would keep the accumulators live)
         asm volatile("" ::: "memory");
-        ll_diag_step<T>(Ds, As12, Li, Ui, Ms, dinv + (int64_t)ib * 2 * kTri * kTri, p == 0,
-                        thr, fail, lane);
+        // (the stored inverses go out from all waves after the barrier)
+        ll_diag_step<T>(Ds, As12, Li, Ui, Ms, nullptr, p == 0, thr, fail, lane);
         asm volatile("" ::: "memory");
 #pragma unroll
         for (int q = 0; q < RT; ++q)
@@ -1595,6 +1595,21 @@ k_getrf_ll(const double* __restrict__ K0, int64_t ldk, int64_t kbs, T* __restric
         }
       }
       __syncthreads();
+      {
+        // the diagonal block's stored inverses [Linv | Uinv], column-major,
+        // from every wave (off the diagonal step's serial path; Li and Ui
+        // are next written by a diagonal step several barriers later)
+        // (one element in flight at a time: the slab's accumulators are
+        // live here)
+        T* const out = dinv + (int64_t)ib * 2 * kTri * kTri;
+        const int t0 = opaque(tid);
+#pragma unroll 1
+        for (int q = 0; q < 4; ++q) {
+          const int idx = t0 + 512 * q;
+          const int e = idx & 1023;
+          out[idx] = (q < 2 ? &Li[0][0] : &Ui[0][0])[(e % 32) * 33 + e / 32];
+        }
+      }
 #pragma unroll
       for (int q = 0; q < RT; ++q) {
         const int rt = wid + 8 * q;
