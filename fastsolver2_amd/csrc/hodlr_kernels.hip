@@ -1387,8 +1387,6 @@ k_getrf_ll(const double* __restrict__ K0, int64_t ldk, int64_t kbs, T* __restric
   __shared__ T Li[32][33];     // L11^-1 [row][col]
   __shared__ T Ui[32][33];     // U11^-1 [row][col]
   __shared__ __align__(16) T Ms[2][32][33];  // inversion tiles
-  // landing space of the next slab's cache prefetch (never read)
-  __shared__ __align__(16) char pf_sink[7][1024];
   __shared__ int bad;
   const int b = blockIdx.x;
   const int tid = threadIdx.x;
@@ -1403,8 +1401,6 @@ k_getrf_ll(const double* __restrict__ K0, int64_t ldk, int64_t kbs, T* __restric
   const uint32_t ldab = (uint32_t)lda * (uint32_t)sizeof(T);
   bool fail = false;
   if (tid == 0) bad = 0;
-  // the next slab's cache prefetch takes 16-byte pieces of K's columns
-  const bool pf_ok = ((reinterpret_cast<uintptr_t>(K) & 15) == 0) && ((ldk & 1) == 0);
   V4 acc[RT][4];
 
   for (int J = 0; J < NS; ++J) {
@@ -1579,20 +1575,6 @@ k_getrf_ll(const double* __restrict__ K0, int64_t ldk, int64_t kbs, T* __restric
           for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
             for (int v = 0; v < 4; ++v) acc[q][ct][v] = park[((q * 4 + ct) * 4 + v) * 64];
-      } else if (pf_ok && J + 1 < NS) {
-        // waves 1..7, idle through the diagonal step: half p of the next
-        // slab of K (32 columns of n rows, RT KB each, in 1 KB pieces) is
-        // copied toward this CU (L2 and the memory-side cache; the LDS
-        // copy is dropped), so the next slab's load, which every CU issues
-        // at the same moment, is served from the caches rather than HBM
-        const char* col0 = reinterpret_cast<const char*>(K + (int64_t)(j0 + 64 + 32 * p) * ldk);
-        for (int i = wid - 1; i < 32 * RT; i += 7) {
-          const int c = i / RT, part = i - c * RT;
-          const char* src = col0 + (size_t)c * ldkb_j + part * 1024 + lane * 16;
-          __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
-                                           (__attribute__((address_space(3))) void*)&pf_sink[wid - 1][0],
-                                           16, 0, 0);
-        }
       }
       __syncthreads();
       {
