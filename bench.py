@@ -212,7 +212,6 @@ def bench_batch(F, torch, W, K_, log, nsys=256):
     from fastsolver2_amd.workloads import batch_operands, batch_systems
     systems = batch_systems(nsys, seed=0)
     Ks, Us, Vs, Bs = batch_operands(systems, seed=0, pad=LD_PAD)
-    K0 = [F.fortran_clone(k) for k in Ks]
     B0 = [F.fortran_clone(x) for x in Bs]
     b = F.HodlrBatch(systems)
     t0 = time.perf_counter()
@@ -222,9 +221,7 @@ def bench_batch(F, torch, W, K_, log, nsys=256):
     ev1 = torch.cuda.Event(enable_timing=True)
     ts = []
     for i in range(W + K_):
-        for k, k0 in zip(Ks, K0):
-            k.copy_(k0)
-        for x, x0 in zip(Bs, B0):
+        for x, x0 in zip(Bs, B0):  # B is overwritten by the solution; K is only read
             x.copy_(x0)
         torch.cuda.synchronize()
         ev0.record()
@@ -237,7 +234,7 @@ def bench_batch(F, torch, W, K_, log, nsys=256):
     if st[0] != 0:
         log(f"batch status {st}")
     worst = 0.0
-    for sy, K, U, V, X, B in zip(systems, K0, Us, Vs, Bs, B0):
+    for sy, K, U, V, X, B in zip(systems, Ks, Us, Vs, Bs, B0):
         R = F.matvec(K, U, V, X, sy["leaf"], alpha=-1.0)
         R.add_(B)
         worst = max(worst, (R.abs().max() / B.abs().max()).item())
@@ -254,7 +251,7 @@ def bench_batch(F, torch, W, K_, log, nsys=256):
     log(f"batch of {nsys}: {ms:.2f} ms ({nsys * 1000.0 / ms:.0f} systems/s, {fl / ms / 1e6:.0f} GFLOP/s), "
         f"tuned lanes={lanes} tile={tile}, worst residual {worst:.2e}")
     b.close()
-    del Ks, Us, Vs, Bs, K0, B0
+    del Ks, Us, Vs, Bs, B0
     return out
 
 

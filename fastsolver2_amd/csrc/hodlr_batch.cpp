@@ -302,6 +302,10 @@ int hodlr_batch_tune(hodlr_batch b, double* const* K, const int64_t* ldk, const 
       if (rc) break;
       float ms = 1e30f;
       for (int rep = 0; rep < 2 && !rc; ++rep) {
+        // from an idle stream, as a caller that waits for each batch sees
+        // it (replays queued behind a running one overlap its tail and
+        // favour fewer lanes)
+        if (hipStreamSynchronize(stream) != hipSuccess) { rc = HODLR_ERR_HIP; break; }
         (void)hipEventRecord(t0, stream);
         rc = hodlr_batch_factor_solve(b, K, ldk, U, ldu, V, ldv, B, ldb, nrhs, stream);
         (void)hipEventRecord(t1, stream);

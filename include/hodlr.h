@@ -211,8 +211,8 @@ int hodlr_batch_set_tuning(hodlr_batch batch, int lanes, int tile);
 int hodlr_batch_get_tuning(hodlr_batch batch, int *lanes, int *tile);
 /* Autotuner: for every (lanes, tile) pair, captures the batch, replays it
  * once untimed (a graph's first launch uploads it), times the faster of two
- * replays, and keeps the fastest pair (*best_ms).  The operands are
- * overwritten many times: pass scratch copies. */
+ * replays, each started on an idle stream, and keeps the fastest pair
+ * (*best_ms).  Every B is overwritten many times: pass scratch copies. */
 int hodlr_batch_tune(hodlr_batch batch, double *const *K, const int64_t *ldk,
                      const double *const *U, const int64_t *ldu,
                      const double *const *V, const int64_t *ldv,
