@@ -1185,8 +1185,8 @@ template <typename T> struct InvTile {
 // columns of L^-1, lanes 16..31 for those of U^-1 (reversed), by
 // column-oriented substitution; written to Li / Ui at offset (o, o).
 template <typename T>
-__device__ __forceinline__ void ll_inv16(T (*Mt)[16][InvTile<T>::S], T (*Li)[33], T (*Ui)[33], int o,
-                                         int lane) {
+__device__ __forceinline__ void ll_inv16(T (*Mt)[16][InvTile<T>::S], T (*Li)[33], T (*Ui)[33],
+                                         int o, int lane) {
   const int h = (lane >> 4) & 1, c = lane & 15;
   const int c0 = h ? 15 - c : c;
   T x[16];
@@ -1580,9 +1580,9 @@ k_getrf_ll(const double* __restrict__ K0, int64_t ldk, int64_t kbs, T* __restric
       {
         // the diagonal block's stored inverses [Linv | Uinv], column-major,
         // from every wave (off the diagonal step's serial path; Li and Ui
-        // are next written by a diagonal step several barriers later)
-        // (one element in flight at a time: the slab's accumulators are
-        // live here)
+        // are next written by a diagonal step several barriers later), one
+        // element in flight per lane at a time: the slab's accumulators
+        // are live here
         T* const out = dinv + (int64_t)ib * 2 * kTri * kTri;
         const int t0 = opaque(tid);
 #pragma unroll 1
