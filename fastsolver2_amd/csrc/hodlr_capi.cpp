@@ -171,6 +171,12 @@ struct CoreBufs {
   int* general = nullptr;  // per-core flags of the structured LU (null: general LU)
   int* tmask = nullptr;    // cores whose diagonal inverses are still to be formed
   const hodlr::CoreSchur* schur = nullptr;  // scratch for ranks 64 < r <= 128
+  // the level above (null at the root): its cores' S (stride 4r^2) and
+  // right-hand sides (the node stride of R), formed by the structured
+  // solve's epilogue; `assembled` reports whether it did
+  double* PS = nullptr;
+  double* PR = nullptr;
+  bool* assembled = nullptr;
 };
 
 // Core step shared by every level: S (already assembled, n2 x n2 per node)
@@ -186,10 +192,16 @@ int core_solve(int nnode, int n2, const CoreBufs& c, double* R, int64_t rs, int 
                hipStream_t st) {
   const int r = n2 / 2;
   ColSeg x = hodlr::seg_single(R, n2);
+  if (c.assembled) *c.assembled = false;
   if (c.general && (n2 & 1) == 0 && hodlr::core_solve_structured(r)) {
-    HIP_TRY(hodlr::launch_core_solve_s(r, c.S, nnode, c.perm, c.dinv, c.general, R, rs, nh, st));
+    // the general cores first: the structured kernel hands every node's
+    // solution, theirs included, to the level above
     HIP_TRY(hodlr::launch_getrs_masked(n2, c.S, n2, (int64_t)n2 * n2, nnode, c.perm, c.dinv, x, rs,
                                        nh, c.general, st));
+    const bool up = c.PR && nnode >= 2;
+    HIP_TRY(hodlr::launch_core_solve_s(r, c.S, nnode, c.perm, c.dinv, c.general, R, rs, nh,
+                                       up ? c.PS : nullptr, up ? c.PR : nullptr, st));
+    if (c.assembled) *c.assembled = up;
     return HODLR_OK;
   }
   HIP_TRY(hodlr::launch_getrs(n2, c.S, n2, (int64_t)n2 * n2, nnode, c.perm, c.dinv, x, rs, nh, st));
@@ -519,11 +531,18 @@ int level_phase(hodlr_plan p, double* B, int64_t ldb, int nrhs, hipStream_t st) 
   if (rc) return rc;
   p->nq = nc;
   const int64_t es = (int64_t)n2 * nc;  // node stride of E
+  bool assembled = false;  // this level's S and R formed by the solve below
   for (int t = L - 1; t >= 0; --t) {
     const int nnode = 1 << t;
     CoreBufs c = core_level(p, t);
     c.R = p->E + heap0(t) * es;
-    {
+    if (t > 0) {
+      c.PS = p->coreS + p->s_off[t - 1];
+      c.PR = p->E + heap0(t - 1) * es;
+    }
+    bool up = false;
+    c.assembled = &up;
+    if (!assembled) {
       ProfScope ps(HODLR_PROF_ASSEMBLE, st, 0.0,
                    8.0 * (double)nnode * (n2 * n2 + 2.0 * n2 * nc));
       if (t == L - 1)  // the children are leaves: their projections
@@ -534,6 +553,7 @@ int level_phase(hodlr_plan p, double* B, int64_t ldb, int nrhs, hipStream_t st) 
     }
     rc = core_factor_solve(nnode, r, c, nc, st);
     if (rc) return rc;
+    assembled = up;
   }
   if (nrhs == 0) return HODLR_OK;
   rc = path_down(p, p->E, es, p->E + (int64_t)r * n2, es, nrhs, st);
@@ -836,17 +856,22 @@ int hodlr_solve(hodlr_plan p, double* B, int64_t ldb, int nrhs, hipStream_t st) 
     if (rc) return rc;
     const int n2 = 2 * r;
     const int64_t bs = (int64_t)n2 * nc;
+    bool assembled = false;  // this level's right-hand sides formed by the solve below
     for (int t = L - 1; t >= 0; --t) {
       const int nnode = 1 << t;
       CoreBufs c = core_level(p, t);
       double* Rt = p->EB + heap0(t) * bs;
+      if (t > 0) c.PR = p->EB + heap0(t - 1) * bs;
+      bool up = false;
+      c.assembled = &up;
       if (t == L - 1)
         HIP_TRY(hodlr::launch_assemble(p->Pb, 1, r, nc, 0, 0, nc, nnode, nullptr, Rt, st));
-      else
+      else if (!assembled)
         HIP_TRY(hodlr::launch_assemble_pair(p->EB + heap0(t + 1) * bs, bs, r, 0, 0, nc, nnode,
                                             nullptr, Rt, bs, st));
       rc = core_solve(nnode, n2, c, Rt, bs, nc, st);
       if (rc) return rc;
+      assembled = up;
     }
     rc = path_down(p, p->E, (int64_t)n2 * p->nq, p->EB, bs, nc, st);
     if (rc) return rc;

@@ -126,11 +126,15 @@ hipError_t launch_getrs_masked(int n, const double* LU, int64_t lda, int64_t bst
                                int64_t xbstride, int ncols, const int* only, hipStream_t st);
 // Core solve with the structured LU of S = [I, A; B, I] (ld 2r, stride 4r^2,
 // r = 32 or 64: core_solve_structured); cores flagged in `general` are
-// skipped (launch_getrs_masked takes them).
+// solved by launch_getrs_masked, launched before this.  With PR non-null
+// (batch even) every node b also hands T_b = E_b(0:r) + E_b(r:2r) to its
+// parent b/2 at the level above: R_p = [T_c1; T_c0] at PR + p*rs (ld 2r,
+// nc columns) and, with PS non-null, S_p = [I, T_c1; T_c0, I] at
+// PS + p*4r^2 from the first r columns (what launch_assemble_pair forms).
 bool core_solve_structured(int r);
 hipError_t launch_core_solve_s(int r, const double* S, int batch, const int* perm,
                                const double* dinv, const int* general, double* R, int64_t rs,
-                               int nc, hipStream_t st);
+                               int nc, double* PS, double* PR, hipStream_t st);
 hipError_t launch_getrs_t(int n, const double* LU, int64_t lda, int64_t bstride, int batch,
                           const int* perm, const double* dinv, const ColSeg& X,
                           int64_t xbstride, int ncols, hipStream_t st);

@@ -3072,7 +3072,8 @@ template <int R>
 __global__ void __launch_bounds__(256)
 k_core_solve_s(const double* __restrict__ S0, const int* __restrict__ perm0,
                const double* __restrict__ dinv0, const int* __restrict__ general,
-               double* __restrict__ R0, int64_t rs, int nc) {
+               double* __restrict__ R0, int64_t rs, int nc, double* __restrict__ PS,
+               double* __restrict__ PR) {
   constexpr int N2 = 2 * R, MT = R / 16, NB = R / 32, Q = R / 4;
   static_assert(R == 32 || R == 64, "structured core solve: r = 32 or 64");
   __shared__ double Xt[R][17];  // R1, then x1
@@ -3080,7 +3081,8 @@ k_core_solve_s(const double* __restrict__ S0, const int* __restrict__ perm0,
   __shared__ double Yb[R][17];  // y2
   __shared__ double X2[R][17];  // x2
   const int b = blockIdx.y;
-  if (general[b]) return;
+  const bool gen = general[b] != 0;  // solved by the generic kernel before this one
+  if (gen && !PR) return;
   const int c0 = blockIdx.x * 16;
   const int sc = min(16, nc - c0);
   const double* S = S0 + (int64_t)b * N2 * N2;
@@ -3095,97 +3097,137 @@ k_core_solve_s(const double* __restrict__ S0, const int* __restrict__ perm0,
   // the coupling of block k to the other block by the other pair
   const int kb = wid >> 1, half = wid & 1;
   const bool dw = kb < NB;
-  double fz[Q], fx[Q], fli[8], fui[8], fc[8];
+  if (!gen) {
+    double fz[Q], fx[Q], fli[8], fui[8], fc[8];
 #pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const int k = 4 * q + lk;
-    fz[q] = gw ? S[(R + 16 * wid + lr) + (int64_t)k * N2] : 0.0;      // P2 B (row, k)
-    fx[q] = gw ? S[(16 * wid + lr) + (int64_t)(R + k) * N2] : 0.0;    // A (row, k)
-  }
-  const double* Di = dv + (int64_t)(NB + kb) * 2 * kTri * kTri;
+    for (int q = 0; q < Q; ++q) {
+      const int k = 4 * q + lk;
+      fz[q] = gw ? S[(R + 16 * wid + lr) + (int64_t)k * N2] : 0.0;      // P2 B (row, k)
+      fx[q] = gw ? S[(16 * wid + lr) + (int64_t)(R + k) * N2] : 0.0;    // A (row, k)
+    }
+    const double* Di = dv + (int64_t)(NB + kb) * 2 * kTri * kTri;
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int k = 4 * q + lk;
-    fli[q] = dw ? Di[(16 * half + lr) + k * 32] : 0.0;
-    fui[q] = dw ? Di[kTri * kTri + (16 * half + lr) + k * 32] : 0.0;
-    // NB == 2: waves 2-3 take L2(block 1, block 0) (rows of tile 2 + half,
-    // columns of block 0), waves 0-1 U2(block 0, block 1)
-    fc[q] = (NB == 2) ? (kb == 1 ? S[(R + 32 + 16 * half + lr) + (int64_t)(R + k) * N2]
-                                 : S[(R + 16 * half + lr) + (int64_t)(R + 32 + k) * N2])
-                      : 0.0;
-  }
-  for (int e = tid; e < N2 * 16; e += 256) {
-    const int i = e % N2, j = e / N2;
-    const double v = j < sc ? Rn[(i < R ? i : perm[i]) + (int64_t)(c0 + j) * N2] : 0.0;
-    if (i < R)
-      Xt[i][j] = v;
-    else
-      Zb[i - R][j] = v;
-  }
-  __syncthreads();
-  auto tile_mm = [&](const double (&af)[8], double (*Bm)[17], int brow0) {
-    d4 acc = d4zero();
+    for (int q = 0; q < 8; ++q) {
+      const int k = 4 * q + lk;
+      fli[q] = dw ? Di[(16 * half + lr) + k * 32] : 0.0;
+      fui[q] = dw ? Di[kTri * kTri + (16 * half + lr) + k * 32] : 0.0;
+      // NB == 2: waves 2-3 take L2(block 1, block 0) (rows of tile 2 + half,
+      // columns of block 0), waves 0-1 U2(block 0, block 1)
+      fc[q] = (NB == 2) ? (kb == 1 ? S[(R + 32 + 16 * half + lr) + (int64_t)(R + k) * N2]
+                                   : S[(R + 16 * half + lr) + (int64_t)(R + 32 + k) * N2])
+                        : 0.0;
+    }
+    for (int e = tid; e < N2 * 16; e += 256) {
+      const int i = e % N2, j = e / N2;
+      const double v = j < sc ? Rn[(i < R ? i : perm[i]) + (int64_t)(c0 + j) * N2] : 0.0;
+      if (i < R)
+        Xt[i][j] = v;
+      else
+        Zb[i - R][j] = v;
+    }
+    __syncthreads();
+    auto tile_mm = [&](const double (&af)[8], double (*Bm)[17], int brow0) {
+      d4 acc = d4zero();
 #pragma unroll
-    for (int q = 0; q < 8; ++q) acc = mfma(af[q], Bm[brow0 + 4 * q + lk][lr], acc);
-    return acc;
-  };
-  // Z = P2 R2 - (P2 B) R1
-  if (gw) {
-    d4 acc = d4zero();
+      for (int q = 0; q < 8; ++q) acc = mfma(af[q], Bm[brow0 + 4 * q + lk][lr], acc);
+      return acc;
+    };
+    // Z = P2 R2 - (P2 B) R1
+    if (gw) {
+      d4 acc = d4zero();
 #pragma unroll
-    for (int q = 0; q < Q; ++q) acc = mfma(fz[q], Xt[4 * q + lk][lr], acc);
+      for (int q = 0; q < Q; ++q) acc = mfma(fz[q], Xt[4 * q + lk][lr], acc);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) Zb[16 * wid + crow(lane, v)][lr] -= acc[v];
-  }
-  lds_barrier();
-  // forward: y_0 = L2_00^-1 Z_0; Z_1 -= L2_10 y_0; y_1 = L2_11^-1 Z_1
-#pragma unroll
-  for (int k = 0; k < NB; ++k) {
-    if (kb == k) {
-      const d4 acc = tile_mm(fli, Zb, 32 * k);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) Yb[32 * k + 16 * half + crow(lane, v)][lr] = acc[v];
+      for (int v = 0; v < 4; ++v) Zb[16 * wid + crow(lane, v)][lr] -= acc[v];
     }
     lds_barrier();
-    if (k + 1 < NB) {
-      if (kb == k + 1) {
-        const d4 acc = tile_mm(fc, Yb, 32 * k);
+    // forward: y_0 = L2_00^-1 Z_0; Z_1 -= L2_10 y_0; y_1 = L2_11^-1 Z_1
 #pragma unroll
-        for (int v = 0; v < 4; ++v) Zb[32 * (k + 1) + 16 * half + crow(lane, v)][lr] -= acc[v];
+    for (int k = 0; k < NB; ++k) {
+      if (kb == k) {
+        const d4 acc = tile_mm(fli, Zb, 32 * k);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Yb[32 * k + 16 * half + crow(lane, v)][lr] = acc[v];
       }
       lds_barrier();
+      if (k + 1 < NB) {
+        if (kb == k + 1) {
+          const d4 acc = tile_mm(fc, Yb, 32 * k);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) Zb[32 * (k + 1) + 16 * half + crow(lane, v)][lr] -= acc[v];
+        }
+        lds_barrier();
+      }
     }
-  }
-  // backward: x2_1 = U2_11^-1 y_1; y_0 -= U2_01 x2_1; x2_0 = U2_00^-1 y_0
+    // backward: x2_1 = U2_11^-1 y_1; y_0 -= U2_01 x2_1; x2_0 = U2_00^-1 y_0
 #pragma unroll
-  for (int k = NB - 1; k >= 0; --k) {
-    if (kb == k) {
-      const d4 acc = tile_mm(fui, Yb, 32 * k);
+    for (int k = NB - 1; k >= 0; --k) {
+      if (kb == k) {
+        const d4 acc = tile_mm(fui, Yb, 32 * k);
 #pragma unroll
-      for (int v = 0; v < 4; ++v) X2[32 * k + 16 * half + crow(lane, v)][lr] = acc[v];
-    }
-    lds_barrier();
-    if (k > 0) {
-      if (kb == k - 1) {
-        const d4 acc = tile_mm(fc, X2, 32 * k);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) Yb[32 * (k - 1) + 16 * half + crow(lane, v)][lr] -= acc[v];
+        for (int v = 0; v < 4; ++v) X2[32 * k + 16 * half + crow(lane, v)][lr] = acc[v];
       }
       lds_barrier();
+      if (k > 0) {
+        if (kb == k - 1) {
+          const d4 acc = tile_mm(fc, X2, 32 * k);
+#pragma unroll
+          for (int v = 0; v < 4; ++v) Yb[32 * (k - 1) + 16 * half + crow(lane, v)][lr] -= acc[v];
+        }
+        lds_barrier();
+      }
     }
-  }
-  // x1 = R1 - A x2
-  if (gw) {
-    d4 acc = d4zero();
+    // x1 = R1 - A x2
+    if (gw) {
+      d4 acc = d4zero();
 #pragma unroll
-    for (int q = 0; q < Q; ++q) acc = mfma(fx[q], X2[4 * q + lk][lr], acc);
+      for (int q = 0; q < Q; ++q) acc = mfma(fx[q], X2[4 * q + lk][lr], acc);
 #pragma unroll
-    for (int v = 0; v < 4; ++v) Xt[16 * wid + crow(lane, v)][lr] -= acc[v];
-  }
+      for (int v = 0; v < 4; ++v) Xt[16 * wid + crow(lane, v)][lr] -= acc[v];
+    }
   lds_barrier();
   for (int e = tid; e < N2 * 16; e += 256) {
     const int i = e % N2, j = e / N2;
     if (j < sc) Rn[i + (int64_t)(c0 + j) * N2] = i < R ? Xt[i][j] : X2[i - R][j];
+  }
+  } else {
+    // a general core: its solution E_b, written by the generic solve
+    for (int e = tid; e < N2 * 16; e += 256) {
+      const int i = e % N2, j = e / N2;
+      const double v = j < sc ? Rn[i + (int64_t)(c0 + j) * N2] : 0.0;
+      if (i < R)
+        Xt[i][j] = v;
+      else
+        X2[i - R][j] = v;
+    }
+    __syncthreads();
+  }
+  if (PR) {
+    // The parent's core from this node's solution (node-sum form): the
+    // sum this child hands up is T = E_b(0:r) + E_b(r:2r).  Parent p = b/2
+    // takes R_p = [T_c1; T_c0] and, in its first r columns (those of Y),
+    // S_p = [I, T_c1; T_c0, I] -- what k_assemble_pair forms, written here
+    // so the level above needs no assembly launch.
+    const int pn = b >> 1, odd = b & 1;
+    double* Rp = PR + (int64_t)pn * rs;
+    double* Sp = PS ? PS + (int64_t)pn * N2 * N2 : nullptr;
+    for (int e = tid; e < R * 16; e += 256) {
+      const int a = e % R, j = e / R;
+      if (j < sc) {
+        const int col = c0 + j;
+        const double t = Xt[a][j] + X2[a][j];
+        Rp[(odd ? a : R + a) + (int64_t)col * N2] = t;
+        if (Sp && col < R) {
+          if (odd) {
+            Sp[a + (int64_t)(R + col) * N2] = t;
+            Sp[(R + a) + (int64_t)(R + col) * N2] = (a == col) ? 1.0 : 0.0;
+          } else {
+            Sp[(R + a) + (int64_t)col * N2] = t;
+            Sp[a + (int64_t)col * N2] = (a == col) ? 1.0 : 0.0;
+          }
+        }
+      }
+    }
   }
 }
 
@@ -3193,14 +3235,18 @@ bool core_solve_structured(int r) { return r == 32 || r == 64; }
 
 hipError_t launch_core_solve_s(int r, const double* S, int batch, const int* perm,
                                const double* dinv, const int* general, double* R, int64_t rs,
-                               int nc, hipStream_t st) {
+                               int nc, double* PS, double* PR, hipStream_t st) {
   if (batch <= 0 || nc <= 0) return hipSuccess;
   if (!core_solve_structured(r) || !general) return hipErrorInvalidValue;
+  // the parent's assembly pairs the nodes: a whole number of parents
+  if (PR && (batch & 1)) return hipErrorInvalidValue;
   const dim3 grid((unsigned)((nc + 15) / 16), (unsigned)batch);
   if (r == 32)
-    hipLaunchKernelGGL(k_core_solve_s<32>, grid, dim3(256), 0, st, S, perm, dinv, general, R, rs, nc);
+    hipLaunchKernelGGL(k_core_solve_s<32>, grid, dim3(256), 0, st, S, perm, dinv, general, R, rs, nc,
+                       PS, PR);
   else
-    hipLaunchKernelGGL(k_core_solve_s<64>, grid, dim3(256), 0, st, S, perm, dinv, general, R, rs, nc);
+    hipLaunchKernelGGL(k_core_solve_s<64>, grid, dim3(256), 0, st, S, perm, dinv, general, R, rs, nc,
+                       PS, PR);
   return hipGetLastError();
 }
 

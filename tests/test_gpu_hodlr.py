@@ -318,7 +318,8 @@ def test_padded_leading_dimension_is_bitwise_identical(gpu):
 
 
 @pytest.mark.parametrize("scale,r", [(1.0, 16), (200.0, 16), (1.0, 100), (200.0, 100),
-                                     (200.0, 64), (1.0, 96), (200.0, 96)])
+                                     (1.0, 32), (200.0, 32), (200.0, 64), (1.0, 96),
+                                     (200.0, 96)])
 def test_core_structured_and_general_lu(gpu, orc, scale, r):
     """Cores S = [I, A; B, I] take the structured LU when max|B| <= 1 (plain
     workload) and the general pivoted LU otherwise (U scaled by 200 makes
@@ -327,7 +328,9 @@ def test_core_structured_and_general_lu(gpu, orc, scale, r):
     by the 128-row leaf LU).  At ranks 64 and 96 (multiples of 32) the core
     solves take the structured solve, with the generic solve masked to the
     cores the general LU factored -- in the fused factor+solve and in a
-    factor-reuse solve, which reads every level's flags."""
+    factor-reuse solve, which reads every level's flags.  At ranks 32 and
+    64 the structured solve also forms the level above's cores from every
+    node's solution, the general cores' included."""
     F = gpu
     N, m, s = 2048, 128, 2
     say(f"case: core LU path, U scale {scale}, rank {r}")
