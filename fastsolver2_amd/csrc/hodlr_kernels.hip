@@ -1678,37 +1678,47 @@ __device__ __forceinline__ void trtri_block(const T* __restrict__ LU, int64_t ld
                                             T* __restrict__ out, T (*M0)[kTri + 1],
                                             T (*M1)[kTri + 1], int lane);
 
+// Orders the accesses of a workgroup's one running wave (LDS and global):
+// the fences of a workgroup barrier without the barrier, so the LU below
+// may also run in one wave of a wider workgroup whose other waves have left.
+__device__ __forceinline__ void w64_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// LDS of the one-wave pivoted LU (w64_factor).
+struct W64Lds {
+  double* prow;  // NB
+  int* pivs;     // NB
+  double* L11;   // NB (NB + 1), also an inversion tile
+  double* Xi;    // NB (NB + 1), also an inversion tile
+  double* Tt;    // NB (NB + 1)
+  int* perm;     // 128
+  int* mv;       // 2 NB: the panel's moving rows (move_rows)
+  int* mcount;
+};
+
+// Pivoted LU of one block of n <= 128 rows by ONE wave (the body of
+// k_getrf_w64): rows lane and lane + 64 in registers, per pivot column a
+// DPP argmax and an LDS broadcast of the pivot row; ipiv (1-based), permo,
+// *infoo; with dinv non-null the inverses of the factor's 32x32 diagonal
+// blocks as well.
 template <int NB>
-__global__ void __launch_bounds__(64)
-k_getrf_w64(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
-            int* __restrict__ ipiv0, int* __restrict__ perm0, int* __restrict__ info0,
-            const int* __restrict__ only, const double* __restrict__ src = nullptr,
-            int64_t lds = 0, int64_t sbstride = 0, double* __restrict__ dinv0 = nullptr) {
-  // `only` (optional): factor block b only where only[b] != 0; `src`
-  // (optional): block b is first copied from src + b*sbstride (ld lds);
-  // `dinv0` (optional): the inverses of the factor's 32x32 diagonal blocks
-  // are formed too (dinv_elems(n) per block)
-  if (only && !only[blockIdx.x]) return;
-  if (src) {
-    const double* S0 = src + (int64_t)blockIdx.x * sbstride;
-    double* D0 = A0 + (int64_t)blockIdx.x * bstride;
-    for (int c = 0; c < n; ++c)
-      for (int r = threadIdx.x; r < n; r += 64) D0[(int64_t)r + (int64_t)c * lda] = S0[(int64_t)r + (int64_t)c * lds];
-    __syncthreads();
-  }
-  constexpr int R = 2;  // rows per lane
-  __shared__ double prow[NB];
-  __shared__ int pivs[NB];
-  __shared__ double L11[NB * (NB + 1)];
-  __shared__ double Xi[NB * (NB + 1)];
-  __shared__ double Tt[NB * (NB + 1)];
-  __shared__ int perm[128];
-  __shared__ int mv[2 * NB];  // the panel's moving rows (move_rows)
+__device__ __forceinline__ void w64_factor(double* __restrict__ A, int64_t lda, int n,
+                                           int* __restrict__ ipiv, int* __restrict__ permo,
+                                           int* __restrict__ infoo, double* __restrict__ dinv,
+                                           const W64Lds& sm, int lane) {
   static_assert(2 * NB <= 64, "a moving row per lane");
-  __shared__ int mcount;
-  const int b = blockIdx.x;
-  const int lane = threadIdx.x;
-  double* A = A0 + (int64_t)b * bstride;
+  constexpr int R = 2;  // rows per lane
+  double* const prow = sm.prow;
+  int* const pivs = sm.pivs;
+  double* const L11 = sm.L11;
+  double* const Xi = sm.Xi;
+  double* const Tt = sm.Tt;
+  int* const perm = sm.perm;
+  int* const mv = sm.mv;
+  int& mcount = *sm.mcount;
   int info = 0;  // lane 0
   for (int q = 0; q < R; ++q)
     if (q * 64 + lane < n) perm[q * 64 + lane] = q * 64 + lane;
@@ -1770,7 +1780,7 @@ k_getrf_w64(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
           }
         }
         if (lane == 0) pivs[jj] = bpos;
-        __syncthreads();  // one wave: orders the LDS broadcast
+        w64_sync();  // one wave: orders the LDS broadcast
         const double pv = prow[jj];
 #pragma unroll
         for (int q = 0; q < R; ++q) {
@@ -1797,7 +1807,7 @@ k_getrf_w64(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
         } else if (lane == 0 && info == 0) {
           info = j0 + jj + 1;
         }
-        __syncthreads();  // prow is rewritten by the next column
+        w64_sync();  // prow is rewritten by the next column
       }
     }
     // rows to their final positions; L11 (padded) to LDS
@@ -1818,7 +1828,7 @@ k_getrf_w64(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
     for (int t = nb + lane; t < NB; t += 64)
 #pragma unroll
       for (int c = 0; c < NB; ++c) L11[t * (NB + 1) + c] = 0.0;
-    if (lane < nb) ipiv0[(int64_t)b * n + j0 + lane] = j0 + pivs[lane] + 1;
+    if (lane < nb) ipiv[j0 + lane] = j0 + pivs[lane] + 1;
     if (lane == 0) {
       for (int jj = 0; jj < nb; ++jj) {
         const int pp = pivs[jj];
@@ -1834,7 +1844,7 @@ k_getrf_w64(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
       const int rr = q * 64 + lane;
       if (rr < rows && pos[q] != rr) mv[atomicAdd(&mcount, 1)] = rr | (pos[q] << 16);
     }
-    __syncthreads();
+    w64_sync();
     const int m2 = n - j0 - nb;
     move_rows<32>(A, lda, n, j0, nb, mv, mcount, 0, 1, lane);
     if (m2 > 0) {
@@ -1842,7 +1852,7 @@ k_getrf_w64(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
         const int i = e / NB, j = e % NB;
         Xi[i * (NB + 1) + j] = (i == j) ? 1.0 : 0.0;
       }
-      __syncthreads();
+      w64_sync();
       for (int h = 1; h < NB; h <<= 1) {
         for (int e = lane; e < NB * h / 2; e += 64) {
           const int q = e / (h * h), r = e % (h * h);
@@ -1851,7 +1861,7 @@ k_getrf_w64(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
           for (int k = j; k < s0 + h; ++k) acc += L11[i * (NB + 1) + k] * Xi[k * (NB + 1) + j];
           Tt[i * (NB + 1) + j] = acc;
         }
-        __syncthreads();
+        w64_sync();
         for (int e = lane; e < NB * h / 2; e += 64) {
           const int q = e / (h * h), r = e % (h * h);
           const int s0 = q * 2 * h, i = s0 + h + r / h, j = s0 + r % h;
@@ -1859,7 +1869,7 @@ k_getrf_w64(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
           for (int k = s0 + h; k <= i; ++k) acc += Xi[i * (NB + 1) + k] * Tt[k * (NB + 1) + j];
           Xi[i * (NB + 1) + j] = -acc;
         }
-        __syncthreads();
+        w64_sync();
       }
       const int o = j0 + nb;
       const int nct = (m2 + 15) >> 4;
@@ -1884,7 +1894,7 @@ k_getrf_w64(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
             if (col < m2 && r < nb) A[(int64_t)(j0 + r) + (int64_t)(o + col) * lda] = u[q][v];
           }
       }
-      __syncthreads();
+      w64_sync();
       // trailing update, 32x32 blocks
       const int t32 = (m2 + 31) >> 5;
       for (int blk = 0; blk < t32 * t32; ++blk) {
@@ -1935,23 +1945,56 @@ k_getrf_w64(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
           }
       }
     }
-    __syncthreads();
+    w64_sync();
   }
   for (int q = 0; q < R; ++q)
-    if (q * 64 + lane < n) perm0[(int64_t)b * n + q * 64 + lane] = perm[q * 64 + lane];
-  if (lane == 0) info0[b] = info;
-  if (dinv0) {
+    if (q * 64 + lane < n) permo[q * 64 + lane] = perm[q * 64 + lane];
+  if (lane == 0) *infoo = info;
+  if (dinv) {
     static_assert(NB >= kTri, "the inversion tiles fit L11 and Xi");
     // the factor is this wave's own stores: the barrier makes every lane's
     // visible to all of them
-    __syncthreads();
+    w64_sync();
     double (*const M0)[kTri + 1] = reinterpret_cast<double (*)[kTri + 1]>(L11);
     double (*const M1)[kTri + 1] = reinterpret_cast<double (*)[kTri + 1]>(Xi);
     for (int ib = 0; ib < ntri(n); ++ib)
       trtri_block<double>(A, lda, n, ib,
-                          dinv0 + (int64_t)b * dinv_elems(n) + (int64_t)ib * 2 * kTri * kTri, M0,
+                          dinv + (int64_t)ib * 2 * kTri * kTri, M0,
                           M1, lane);
   }
+}
+
+template <int NB>
+__global__ void __launch_bounds__(64)
+k_getrf_w64(double* __restrict__ A0, int64_t lda, int64_t bstride, int n,
+            int* __restrict__ ipiv0, int* __restrict__ perm0, int* __restrict__ info0,
+            const int* __restrict__ only, const double* __restrict__ src = nullptr,
+            int64_t lds = 0, int64_t sbstride = 0, double* __restrict__ dinv0 = nullptr) {
+  // `only` (optional): factor block b only where only[b] != 0; `src`
+  // (optional): block b is first copied from src + b*sbstride (ld lds);
+  // `dinv0` (optional): the inverses of the factor's 32x32 diagonal blocks
+  // are formed too (dinv_elems(n) per block)
+  if (only && !only[blockIdx.x]) return;
+  if (src) {
+    const double* S0 = src + (int64_t)blockIdx.x * sbstride;
+    double* D0 = A0 + (int64_t)blockIdx.x * bstride;
+    for (int c = 0; c < n; ++c)
+      for (int r = threadIdx.x; r < n; r += 64) D0[(int64_t)r + (int64_t)c * lda] = S0[(int64_t)r + (int64_t)c * lds];
+    __syncthreads();
+  }
+  __shared__ double prow[NB];
+  __shared__ int pivs[NB];
+  __shared__ double L11[NB * (NB + 1)];
+  __shared__ double Xi[NB * (NB + 1)];
+  __shared__ double Tt[NB * (NB + 1)];
+  __shared__ int perm[128];
+  __shared__ int mv[2 * NB];
+  __shared__ int mcount;
+  const int b = blockIdx.x;
+  const W64Lds sm{prow, pivs, L11, Xi, Tt, perm, mv, &mcount};
+  w64_factor<NB>(A0 + (int64_t)b * bstride, lda, n, ipiv0 + (int64_t)b * n, perm0 + (int64_t)b * n,
+                 info0 + b, dinv0 ? dinv0 + (int64_t)b * dinv_elems(n) : nullptr, sm,
+                 (int)threadIdx.x);
 }
 
 static size_t getrf_lds_bytes(int nb, int n) {
@@ -2066,7 +2109,18 @@ k_getrf_core(double* __restrict__ A0, int64_t lda, int64_t bstride, int n2,
     general[b] = fallback ? 1 : 0;
     tmask[b] = 1;  // cleared below once this kernel has formed the inverses
   }
-  if (fallback) return;  // uniform over the workgroup
+  if (fallback) {  // uniform over the workgroup
+    // a general core: the pivoted LU of S and its diagonal inverses by wave
+    // 0 alone (the others leave; it synchronizes only itself), in the LDS
+    // of the structured path (B and A are read again from S)
+    if (wid != 0) return;
+    const W64Lds sm{Cs, reinterpret_cast<int*>(Bs), Cs + 64, Cs + 64 + 32 * 33,
+                    Cs + 64 + 2 * 32 * 33, reinterpret_cast<int*>(Bs) + 32,
+                    reinterpret_cast<int*>(Bs) + 160, reinterpret_cast<int*>(Bs) + 224};
+    w64_factor<32>(S, lda, n2, ipiv0 + (int64_t)b * n2, perm0 + (int64_t)b * n2, info0 + b,
+                   dinv0 + (int64_t)b * dinv_elems(n2), sm, lane);
+    return;
+  }
   // (2) C = I - B A on MFMA from LDS: wave w forms 16-row block w of C,
   // its four column tiles side by side (independent MFMA chains)
   const int nt = (r + 15) >> 4;
@@ -2528,12 +2582,9 @@ hipError_t launch_getrf_core(int n2, double* S, int64_t lda, int64_t bstride, in
                        (const double*)nullptr, (int64_t)0, (int64_t)0);
     return hipGetLastError();
   }
+  // (general cores take the pivoted LU inside the kernel)
   hipLaunchKernelGGL(k_getrf_core, dim3(batch), dim3(256), 0, st, S, lda, bstride, n2, ipiv, perm,
                      info, general, dinv, tmask);
-  // general cores: the pivoted LU of S and its diagonal inverses
-  hipLaunchKernelGGL(k_getrf_w64<32>, dim3(batch), dim3(64), 0, st, S, lda, bstride, n2, ipiv, perm,
-                     info, (const int*)general, (const double*)nullptr, (int64_t)0, (int64_t)0,
-                     dinv);
   return hipGetLastError();
 }
 
